@@ -1,0 +1,196 @@
+/*
+ * wrt.h -- C-ABI of the MI355X-native 3D projection hot path.
+ *
+ * Drop-in for the numerical core of the reference programs radon_grid, ray_grid,
+ * radon_reduction and radon3d_ajoint (devhliu/wrt-project).  The reference computes one
+ * output element per call of a pure C function over a jagged double*** array; this ABI
+ * takes whole grids as flat row-major arrays (same index order as the reference's files)
+ * and runs them as HIP kernels on one AMD Instinct MI355X (gfx950).
+ *
+ * Layouts (x-major volumes, z fastest, as read by tfunction_values_init):
+ *   volume   vol[x][y][z]            n^3       radon_grid/src/init.c:120-124
+ *   radon    out[s][phi][theta]                radon_grid/src/radon_grid.c:248-258
+ *   rays     out[z][s][phi]                    ray_grid/src/ray_grid.c:244-255
+ *   adjoint  out[z][x][y]                      radon_inversion/backprojection/C/src/radon3d_ajoint.c:240-254
+ *
+ * Conventions:
+ *   - every function returns WRT_OK (0) or a negative WRT_E_* code, never aborts;
+ *   - the caller owns every host buffer; a wrt_ctx owns all device memory it allocates;
+ *   - one submitting host thread per wrt_ctx;
+ *   - precision WRT_F64 is the plain form: the reference's arithmetic and index decisions
+ *     in IEEE binary64 (relative error vs the reference <= 1e-10);
+ *     WRT_F32 / WRT_BF16 are the fast forms (FP32 or BF16 values, FP32 compensated sums);
+ *     their error against WRT_F64 is reported, not assumed.
+ */
+#ifndef WRT_H
+#define WRT_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ error codes --- */
+#define WRT_OK 0
+#define WRT_E_ARG (-1)     /* invalid size, grid or pointer                           */
+#define WRT_E_NODEV (-2)   /* no usable HIP device                                    */
+#define WRT_E_HIP (-3)     /* HIP runtime error                                       */
+#define WRT_E_NOMEM (-4)   /* device or host allocation failed                        */
+#define WRT_E_ADMIT (-5)   /* serve loop: the job does not fit the free-HBM budget    */
+#define WRT_E_IO (-6)      /* file open / parse error (config, volume, data files)    */
+#define WRT_E_STATE (-7)   /* call out of order (e.g. run before load)                */
+
+typedef enum { WRT_F64 = 0, WRT_F32 = 1, WRT_BF16 = 2 } wrt_prec;
+
+typedef enum {
+  WRT_OP_RADON = 0,     /* plane integrals       (radon_grid)                      */
+  WRT_OP_RAY = 1,       /* layer-by-layer rays   (ray_grid)                        */
+  WRT_OP_REDUCTION = 2, /* rays -> Radon         (radon_reduction)                 */
+  WRT_OP_ADJOINT = 3,   /* backprojection        (radon3d_ajoint)                  */
+  WRT_OP_RADON_T = 4,   /* exact transpose of WRT_OP_RADON (no reference counterpart) */
+  WRT_OP_RAY_T = 5      /* exact transpose of WRT_OP_RAY   (no reference counterpart) */
+} wrt_op;
+
+typedef struct wrt_ctx wrt_ctx;
+
+/* Timing and work accounting of one call.  seconds: device time from HIP events on the
+ * context stream around the whole device part of the call (no host transfers);
+ * kernel_seconds: the dominant kernel only.  logical_samples counts the reference's
+ * samples (plane: 16 n^2 per plane, ray: 4n+1 per ray, reduction: 4n+1 per plane,
+ * adjoint: nphi*ntheta per voxel); alg_bytes is the algorithmic byte count of the
+ * dominant kernel. */
+typedef struct wrt_stats {
+  double seconds;
+  double kernel_seconds;
+  double logical_samples;
+  double alg_bytes;
+  long long outputs;
+} wrt_stats;
+
+/* -------------------------------------------------------------------- context --- */
+int wrt_init(wrt_ctx** ctx, int device);
+void wrt_free(wrt_ctx* ctx);
+const char* wrt_strerror(int code);
+/* free / total device memory as reported by the HIP runtime (admission input) */
+int wrt_mem_info(wrt_ctx* ctx, size_t* free_bytes, size_t* total_bytes);
+
+/* ---------------------------------------------------------------------- grids --- */
+/* n-point Gauss-Legendre rule on [-1,1], ascending nodes, computed on the GPU in
+ * double-double and rounded.  Replaces gsl_integration_glfixed_table_alloc/point at
+ * radon_grid/src/init.c:180-187, radon_reduction/init.c:213-220,
+ * radon_inversion/backprojection/C/src/init.c:186-194. */
+int wrt_gl_nodes(wrt_ctx* ctx, int n, double* x, double* w);
+
+/* phi_k = (2pi/nphi)k, theta_i = acos(x_i), theta_w[i] = w_i (may be NULL),
+ * s_j = -1 + j*2/(nshift-1).  Replaces radon_grid_init (radon_grid/src/init.c:165-190)
+ * and its weighted copy (radon_inversion/backprojection/C/src/init.c:170-197). */
+int wrt_radon_grid_init(wrt_ctx* ctx, int nphi, int ntheta, int nshift, double* phi,
+                        double* theta, double* theta_w, double* shift);
+/* s_j as above, phi_k = (2pi/nphi)k.  Replaces ray_grid_init (ray_grid/src/init.c:163-177). */
+int wrt_ray_grid_init(int nshift, int nphi, double* shift, double* phi);
+/* z_l = -1 + l*2/(nslices-1) (ray_grid/src/ray_grid.c:240-247). */
+int wrt_zslice_grid(int nslices, double* z);
+
+/* ------------------------------------------------- host-buffer entry points --- */
+/* Plane integrals over every (shift, phi, theta): replaces plane_integral
+ * (radon_grid/src/integral.c:26-59) called from radon_grid/src/radon_grid.c:248-258.
+ * out[s][phi][theta]. */
+int wrt_radon_grid(wrt_ctx* ctx, const double* vol, int n, const double* phi, int nphi,
+                   const double* theta, int ntheta, const double* shift, int nshift,
+                   wrt_prec prec, double* out, wrt_stats* st);
+
+/* Line integrals in z-slices: replaces ray_transform (ray_grid/src/integral.c:11-50)
+ * called from ray_grid/src/ray_grid.c:242-255.  out[z][s][phi]. */
+int wrt_ray_grid(wrt_ctx* ctx, const double* vol, int n, const double* z, int nz,
+                 const double* shift, int nshift, const double* phi, int nphi, wrt_prec prec,
+                 double* out, wrt_stats* st);
+
+/* Ray data -> Radon data: replaces radon_reduction + ray_interpolate
+ * (radon_reduction/slicing.c:5-119) called from radon_reduction/reduction.c:213-229.
+ * rays[z][s][phi] share the output's shift and phi grids; nz is the number of z-slices
+ * (the reference's "ngrid").  out[s][phi][theta].  FP64 only. */
+int wrt_radon_reduction(wrt_ctx* ctx, const double* rays, int nz, const double* shift,
+                        int nshift, const double* phi, int nphi, const double* theta,
+                        int ntheta, double* out, wrt_stats* st);
+
+/* Backprojection at the n^3 voxel centres: replaces radon3d_adjoint + radon_interpolate_s
+ * (radon_inversion/backprojection/C/src/integral.c:6-24, interp.c:5-28) called from
+ * radon3d_ajoint.c:240-254.  radon[s][phi][theta]; out[z][x][y]. */
+int wrt_radon_adjoint(wrt_ctx* ctx, const double* radon, const double* phi, int nphi,
+                      const double* theta, const double* theta_w, int ntheta,
+                      const double* shift, int nshift, int n, wrt_prec prec, double* out,
+                      wrt_stats* st);
+
+/* Exact transposes of wrt_radon_grid and wrt_ray_grid (no reference counterpart): the
+ * volume v with <A f, g> = <f, v> for every f.  g in the forward's output layout;
+ * out[x][y][z]. */
+int wrt_radon_grid_T(wrt_ctx* ctx, const double* g, int n, const double* phi, int nphi,
+                     const double* theta, int ntheta, const double* shift, int nshift,
+                     wrt_prec prec, double* out, wrt_stats* st);
+int wrt_ray_grid_T(wrt_ctx* ctx, const double* g, int n, const double* z, int nz,
+                   const double* shift, int nshift, const double* phi, int nphi, wrt_prec prec,
+                   double* out, wrt_stats* st);
+
+/* ---------------------------------------------- device-resident jobs (batch/serve) --- */
+/* A job owns its device input, output and tables; load/run/fetch may repeat.  The grids
+ * passed at creation are copied.  Unused grid fields may be NULL/0 for an op:
+ *   RADON, RADON_T : n, phi, theta, shift        RAY, RAY_T : n, z, shift, phi
+ *   REDUCTION      : nz, shift, phi, theta       ADJOINT    : n, phi, theta, theta_w, shift */
+typedef struct wrt_job_desc {
+  wrt_op op;
+  wrt_prec prec;
+  int n;
+  const double* phi;
+  int nphi;
+  const double* theta;
+  const double* theta_w;
+  int ntheta;
+  const double* shift;
+  int nshift;
+  const double* z;
+  int nz;
+} wrt_job_desc;
+
+typedef struct wrt_job wrt_job;
+
+/* device bytes a job of this description holds (admission control) */
+size_t wrt_job_bytes(const wrt_job_desc* d);
+size_t wrt_job_input_count(const wrt_job_desc* d);
+size_t wrt_job_output_count(const wrt_job_desc* d);
+int wrt_job_create(wrt_ctx* ctx, const wrt_job_desc* d, wrt_job** job);
+/* host -> device input (converted to the job's precision/layout) */
+int wrt_job_load(wrt_job* job, const double* in);
+/* synthetic input from the seeded generators below: an ellipsoid phantom for RADON (cell
+ * centred) and RAY (node centred), Gaussian data for the other ops */
+int wrt_job_load_synthetic(wrt_job* job, unsigned long long seed, int n_ellipsoids,
+                           double texture_sigma);
+/* enqueue on the context stream and wait; fills st when non-NULL */
+int wrt_job_run(wrt_job* job, wrt_stats* st);
+int wrt_job_fetch(wrt_job* job, double* out);
+void wrt_job_destroy(wrt_job* job);
+
+/* Run many jobs (forward and adjoint, uneven sizes) as one persistent launch captured in
+ * a HIP graph; the graph is replayed on later calls with the same job list.  Each job's
+ * output equals its wrt_job_run output within the documented tolerance. */
+int wrt_batch_run(wrt_ctx* ctx, wrt_job* const* jobs, int njobs, wrt_stats* st);
+
+/* ----------------------------------------------------------- synthetic inputs --- */
+/* Seeded ellipsoid phantom on [-1,1]^3 (no reference counterpart): nell ellipsoids with
+ * centres uniform in |c| < 0.5, semi-axes U(0.05,0.4), uniform random rotations and
+ * densities U(-0.5,1.0), summed and cut to the unit ball, plus N(0, sigma^2) texture inside
+ * the ball when sigma > 0.  cell_centred = 1 samples x_i = -1 + (2i+1)/n (the cells of
+ * radon_grid's zero_interp); 0 samples x_i = -1 + 2i/(n-1) (the nodes of ray_grid's
+ * trilinear interpolation).  vol[x][y][z].  Identical bits on every host. */
+int wrt_phantom(int n, int cell_centred, int nell, unsigned long long seed, double sigma,
+                double* vol);
+/* the ellipsoids wrt_phantom draws: centre[3], axes[3], rot[9] (row-major, columns are
+ * the principal axes), density -- 16 doubles per ellipsoid */
+int wrt_phantom_params(int nell, unsigned long long seed, double* params);
+/* count standard normal deviates from a seeded counter-based generator */
+int wrt_gaussian(size_t count, unsigned long long seed, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WRT_H */

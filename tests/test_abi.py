@@ -1,0 +1,83 @@
+"""CPU tests of the C-ABI library: it loads, exports every symbol include/wrt.h declares,
+and its host-only parts (grids, synthetic inputs, argument checks) behave without a GPU."""
+import ctypes as C
+import math
+import os
+import re
+
+import numpy as np
+
+import orc
+import wrt
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    src = open(os.path.join(ROOT, "include", "wrt.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = re.findall(r"\b(wrt_[A-Za-z0-9_]+)\s*\(", src)
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_symbol():
+    L = wrt.lib()
+    names = declared_symbols()
+    assert len(names) >= 25
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    assert sorted(set(wrt.ABI_SYMBOLS)) == names
+
+
+def test_host_grids_match_oracle_bits():
+    s, p = wrt.ray_grid_init(257, 180)
+    assert np.array_equal(s, orc.shift_grid(257))
+    assert np.array_equal(p, orc.phi_grid(180))
+    assert np.array_equal(wrt.zslice_grid(128), orc.zslice_grid(128))
+
+
+def test_phantom_properties():
+    n = 32
+    v = wrt.phantom(n, cell_centred=True, nell=10, seed=0, sigma=0.0)
+    v2 = wrt.phantom(n, cell_centred=True, nell=10, seed=0, sigma=0.0)
+    assert np.array_equal(v, v2)
+    xs = -1.0 + (2.0 * np.arange(n) + 1.0) / n
+    X, Y, Z = np.meshgrid(xs, xs, xs, indexing="ij")
+    outside = X ** 2 + Y ** 2 + Z ** 2 > 1.0
+    assert np.all(v[outside] == 0.0)
+    assert np.count_nonzero(v) > 0
+    P = wrt.phantom_params(10, seed=0)
+    assert np.all(np.linalg.norm(P[:, 0:3], axis=1) < 0.5)
+    assert np.all((P[:, 3:6] >= 0.05) & (P[:, 3:6] <= 0.4))
+    assert np.all((P[:, 15] >= -0.5) & (P[:, 15] <= 1.0))
+    for e in range(10):
+        R = P[e, 6:15].reshape(3, 3)
+        assert np.allclose(R @ R.T, np.eye(3), atol=1e-12)
+        assert abs(np.linalg.det(R) - 1.0) < 1e-12
+    t = wrt.phantom(n, cell_centred=False, nell=10, seed=0, sigma=0.01)
+    assert not np.array_equal(t, wrt.phantom(n, cell_centred=False, nell=10, seed=0))
+
+
+def test_gaussian_moments():
+    g = wrt.gaussian((200000,), seed=3)
+    assert abs(g.mean()) < 0.01 and abs(g.std() - 1.0) < 0.01
+    assert np.array_equal(g[:100], wrt.gaussian((100,), seed=3))
+
+
+def test_job_sizes():
+    phi = np.zeros(4)
+    th = np.zeros(3)
+    s = np.linspace(-1, 1, 5)
+    assert wrt.job_bytes("radon", "f64", n=8, phi=phi, theta=th, shift=s) > 8 ** 3 * 8
+    assert wrt.job_bytes("radon", "f64", n=0, phi=phi, theta=th, shift=s) == 0
+    assert wrt.job_bytes("reduction", "f32", phi=phi, theta=th, shift=s, nz=8) == 0
+
+
+def test_init_without_device_reports_nodev():
+    h = C.c_void_p()
+    rc = wrt.lib().wrt_init(C.byref(h), 0)
+    if rc == wrt.WRT_OK:  # a GPU is present: this is a CPU-only check
+        wrt.lib().wrt_free(h)
+        return
+    assert rc == wrt.WRT_E_NODEV
+    assert "device" in wrt.strerror(rc)

@@ -1,0 +1,282 @@
+"""Pins the CPU oracle (oracle/wrt_oracle.c) before it is trusted as the checker.
+
+- bit-exact against the reference's own GSL-free functions (golden fixtures made from
+  radon_grid/src/{integral,interp}.c and ray_grid/src/{integral,interp}.c);
+- Gauss-Legendre rule against 60-digit mpmath (correctly rounded) and the README values;
+- natural spline against scipy; the quadratic against exact polynomials;
+- closed forms: ellipsoid plane/line integrals, mass conservation, constant and linear
+  data through the adjoint, a Gaussian through ray -> Radon reduction;
+- the dot-product test for the transposes.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import orc
+from golden_inputs import volume_formula
+
+G = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _npz(name):
+    return np.load(os.path.join(G, name), allow_pickle=False)
+
+
+# --------------------------------------------------------------- reference goldens ---
+def test_plane_integral_matches_reference_bits():
+    f = _npz("ref_radon_grid.npz")
+    keys = [k for k in f.files if k.startswith("radon_") and not k.endswith(("_in", "_n"))]
+    assert len(keys) >= 4
+    for k in keys:
+        n = int(f[k + "_n"])
+        phi, theta, shift = f[k + "_in"]
+        vol = volume_formula(n)
+        got = np.array([orc.plane_integral(vol, p, t, s) for p, t, s in zip(phi, theta, shift)])
+        assert np.array_equal(got, f[k]), k
+
+
+def test_zero_interp_matches_reference_bits():
+    f = _npz("ref_radon_grid.npz")
+    for k in [k for k in f.files if k.startswith("zero_") and not k.endswith(("_in", "_n"))]:
+        n = int(f[k + "_n"])
+        vol = volume_formula(n)
+        got = np.array([orc.zero_interp(vol, *p) for p in f[k + "_in"]])
+        assert np.array_equal(got, f[k]), k
+
+
+def test_ray_transform_matches_reference_bits():
+    f = _npz("ref_ray_grid.npz")
+    keys = [k for k in f.files if k.startswith("ray_") and not k.endswith(("_in", "_n"))]
+    assert "ray_n128" in keys
+    for k in keys:
+        n = int(f[k + "_n"])
+        z, s, phi = f[k + "_in"]
+        vol = volume_formula(n)
+        got = orc.rays(vol, z, s, phi)
+        assert np.array_equal(got, f[k]), k
+
+
+def test_trilinear_matches_reference_bits():
+    f = _npz("ref_ray_grid.npz")
+    for k in [k for k in f.files if k.startswith("tri_") and not k.endswith(("_in", "_n"))]:
+        n = int(f[k + "_n"])
+        vol = volume_formula(n)
+        got = np.array([orc.trilinear(vol, *p) for p in f[k + "_in"]])
+        assert np.array_equal(got, f[k]), k
+
+
+def test_trilinear_index_quirk_is_reproduced():
+    # a quotient that rounds up to an integer k reads cell k with fraction ~1, i.e. ~node k+1
+    n = 9
+    vol = volume_formula(n)
+    h = 2.0 / (n - 1)
+    hits = 0
+    for k in range(1, n - 1):
+        x = np.nextafter(k * h - 1.0, -2.0)
+        a = x + 1.0
+        if int(a / h) == k and math.fmod(a, h) / h > 0.5:
+            hits += 1
+            v = orc.trilinear(vol, x, -1.0, -1.0)
+            assert abs(v - vol[k + 1, 0, 0]) <= 1e-12 * max(1.0, abs(vol[k + 1, 0, 0]))
+    assert hits >= 0
+
+
+# ---------------------------------------------------------------------- grids / GL ---
+def test_gl_nodes_correctly_rounded():
+    f = _npz("gl.npz")
+    for k in [k for k in f.files if k.startswith("x")]:
+        n = int(k[1:])
+        x, w = orc.gl_nodes(n)
+        assert np.array_equal(x, f[k]), n
+        assert np.array_equal(w, f["w%d" % n]), n
+        assert np.array_equal(x, -x[::-1])
+        assert abs(w.sum() - 2.0) < 1e-14
+        assert np.all(np.diff(x) > 0)
+
+
+def test_gl_exactness_on_polynomials():
+    x, w = orc.gl_nodes(12)
+    for k in range(0, 24):
+        exact = 0.0 if k % 2 else 2.0 / (k + 1)
+        assert abs(np.sum(w * x ** k) - exact) < 1e-14
+
+
+def test_readme_grid_values():
+    # radon_analytic/README.md:76-78 (ntheta = 128) and ray_grid/README.md:91-93 (nphi = 256)
+    th, _ = orc.theta_grid(128)
+    assert ["%f" % t for t in th[:3]] == ["3.122878", "3.098635", "3.074249"]
+    phi = orc.phi_grid(256)
+    assert ["%f" % p for p in phi[227:230]] == ["5.571418", "5.595962", "5.620506"]
+    s = orc.shift_grid(129)
+    assert s[0] == -1.0 and s[-1] == 1.0 and s[64] == 0.0
+
+
+# ------------------------------------------------------------ spline / polynomial ---
+def test_natural_spline_vs_scipy():
+    f = _npz("spline.npz")
+    for k in [k for k in f.files if k.startswith("x")]:
+        ns = k[1:]
+        x, y, q, v = f["x" + ns], f["y" + ns], f["q" + ns], f["v" + ns]
+        c = orc.cspline(x, y)
+        assert c[0] == 0.0 and c[-1] == 0.0
+        got = np.array([orc.cspline_eval(x, y, c, t) for t in q])
+        assert np.max(np.abs(got - v)) <= 1e-13 * max(1.0, np.max(np.abs(v))), ns
+
+
+def test_spline_reproduces_linear_data_and_nodes():
+    x = -1.0 + np.arange(17) * (2.0 / 16)
+    y = 0.25 - 1.5 * x
+    c = orc.cspline(x, y)
+    assert np.max(np.abs(c)) < 1e-14
+    for t in np.linspace(-1, 1, 51):
+        assert abs(orc.cspline_eval(x, y, c, t) - (0.25 - 1.5 * t)) < 1e-14
+    y = np.cos(3 * x)
+    c = orc.cspline(x, y)
+    for i, t in enumerate(x):
+        assert abs(orc.cspline_eval(x, y, c, t) - y[i]) < 1e-15
+
+
+def test_poly3_exact_on_quadratics():
+    rng = np.random.default_rng(3)
+    for _ in range(50):
+        a, b, cc = rng.standard_normal(3)
+        xa = np.sort(rng.uniform(-1, 1, 3))
+        ya = a + b * xa + cc * xa ** 2
+        for t in rng.uniform(-1, 1, 5):
+            assert abs(orc.poly3(xa, ya, t) - (a + b * t + cc * t * t)) < 1e-11
+
+
+# -------------------------------------------------------------------- closed forms ---
+def _one_ellipsoid(n, cell):
+    import wrt
+
+    p = wrt.phantom_params(1, seed=5)[0]
+    vol = wrt.phantom(n, cell_centred=cell, nell=1, seed=5)
+    return p, vol
+
+
+def test_radon_ellipsoid_closed_form_and_mass():
+    n = 48
+    p, vol = _one_ellipsoid(n, True)
+    c, ax, R, rho = p[0:3], p[3:6], p[6:15], p[15]
+    rng = np.random.default_rng(1)
+    for _ in range(12):
+        phi, th, s = rng.uniform(0, 2 * math.pi), math.acos(rng.uniform(-1, 1)), rng.uniform(-0.6, 0.6)
+        nr, _, _ = orc.plane_basis(phi, th)
+        got = orc.plane_integral(vol, phi, th, s)
+        want = orc.ellipsoid_plane(c, ax, R, rho, nr, s)
+        assert abs(got - want) <= 0.06 * abs(rho) * math.pi * max(ax) ** 2 + 1e-12
+    # mass: sum_s R(s) ds = sum f h^3 (exact in the continuum, discretisation here)
+    ns = 97
+    shift = orc.shift_grid(ns)
+    R1 = orc.radon_grid(vol, np.array([0.7]), np.array([1.1]), shift)[:, 0, 0]
+    mass = R1.sum() * (2.0 / (ns - 1))
+    assert abs(mass - vol.sum() * (2.0 / n) ** 3) < 0.02 * abs(vol.sum() * (2.0 / n) ** 3)
+
+
+def test_ray_ellipsoid_closed_form():
+    n = 49
+    p, vol = _one_ellipsoid(n, False)
+    c, ax, R, rho = p[0:3], p[3:6], p[6:15], p[15]
+    rng = np.random.default_rng(2)
+    for _ in range(16):
+        z, s, phi = rng.uniform(-0.5, 0.5), rng.uniform(-0.5, 0.5), rng.uniform(0, 2 * math.pi)
+        got = orc.ray_transform(vol, z, s, phi)
+        p0 = np.array([s * math.cos(phi), s * math.sin(phi), z])
+        d = np.array([math.cos(phi + math.pi / 2), math.sin(phi + math.pi / 2), 0.0])
+        want = orc.ellipsoid_line(c, ax, R, rho, p0, d)
+        assert abs(got - want) <= 0.08 * abs(rho) * 2 * max(ax) + 1e-12
+
+
+def test_adjoint_constant_and_linear_data():
+    nphi, nth, ns, n = 12, 6, 17, 6
+    phi = orc.phi_grid(nphi)
+    th, w = orc.theta_grid(nth)
+    s = orc.shift_grid(ns)
+    const = np.full((ns, nphi, nth), 0.75)
+    adj = orc.radon_adjoint(const, phi, th, w, s, n)
+    xs = -1.0 + 1.0 / n + np.arange(n) * (2.0 / n)
+    Z, X, Y = np.meshgrid(xs, xs, xs, indexing="ij")
+    inside = (X ** 2 + Y ** 2 + Z ** 2) <= 1.0
+    assert np.max(np.abs(adj[inside] - 4 * math.pi * 0.75)) < 1e-12
+    lin = np.broadcast_to(s[:, None, None], (ns, nphi, nth)).copy()
+    adj = orc.radon_adjoint(lin, phi, th, w, s, n)
+    assert np.max(np.abs(adj[inside])) < 1e-12
+
+
+def test_reduction_gaussian_closed_form():
+    # f = exp(-|x|^2/sg^2): P(z,s) = sg sqrt(pi) exp(-(z^2+s^2)/sg^2), R(p) = pi sg^2 exp(-p^2/sg^2)
+    sg = 0.3
+    nz, ns, nphi, nth = 65, 65, 4, 6
+    z = orc.zslice_grid(nz)
+    s = orc.shift_grid(ns)
+    phi = orc.phi_grid(nphi)
+    th, _ = orc.theta_grid(nth)
+    rays = sg * math.sqrt(math.pi) * np.exp(-(z[:, None, None] ** 2 + s[None, :, None] ** 2) / sg ** 2)
+    rays = np.broadcast_to(rays, (nz, ns, nphi)).copy()
+    R = orc.radon_reduction(rays, s, phi, th)
+    want = math.pi * sg ** 2 * np.exp(-s ** 2 / sg ** 2)
+    err = np.max(np.abs(R - want[:, None, None]))
+    assert err < 2e-3 * want.max(), err
+
+
+def test_ray_symmetry_phi_plus_pi():
+    n = 17
+    vol = volume_formula(n)
+    rng = np.random.default_rng(4)
+    for _ in range(20):
+        z, s, phi = rng.uniform(-1, 1), rng.uniform(-1, 1), rng.uniform(0, math.pi)
+        a = orc.ray_transform(vol, z, s, phi + math.pi)
+        b = orc.ray_transform(vol, z, -s, phi)
+        assert abs(a - b) <= 1e-10 * max(1.0, abs(b))
+
+
+# ---------------------------------------------------------------------- transposes ---
+def test_transposes_dot_product():
+    rng = np.random.default_rng(9)
+    n = 6
+    phi, s = orc.phi_grid(6), orc.shift_grid(7)
+    th, _ = orc.theta_grid(3)
+    f = rng.standard_normal((n, n, n))
+    g = rng.standard_normal((len(s), len(phi), len(th)))
+    Af = orc.radon_grid(f, phi, th, s)
+    ATg = orc.radon_grid_T(g, n, phi, th, s)
+    lhs, rhs = np.sum(Af * g), np.sum(f * ATg)
+    assert abs(lhs - rhs) <= 1e-12 * np.linalg.norm(Af) * np.linalg.norm(g)
+    n = 7
+    z = orc.zslice_grid(5)
+    s = orc.shift_grid(9)
+    phi = orc.phi_grid(8)
+    f = rng.standard_normal((n, n, n))
+    g = rng.standard_normal((len(z), len(s), len(phi)))
+    Af = orc.ray_grid(f, z, s, phi)
+    ATg = orc.ray_grid_T(g, n, z, s, phi)
+    lhs, rhs = np.sum(Af * g), np.sum(f * ATg)
+    assert abs(lhs - rhs) <= 1e-12 * np.linalg.norm(Af) * np.linalg.norm(g)
+
+
+@pytest.mark.parametrize("planted", ["sign", "transpose", "delta"])
+def test_checks_catch_planted_mistakes(planted):
+    # a mistake planted in the oracle's answer must fail the closed-form check on some plane
+    n = 48
+    p, vol = _one_ellipsoid(n, True)
+    c, ax, R, rho = p[0:3], p[3:6], p[6:15], p[15]
+    tol = 0.06 * abs(rho) * math.pi * max(ax) ** 2
+    volT = np.ascontiguousarray(vol.transpose(2, 1, 0))
+    rng = np.random.default_rng(1)
+    worst = 0.0
+    for _ in range(12):
+        phi, th = rng.uniform(0, 2 * math.pi), math.acos(rng.uniform(-1, 1))
+        nr, _, _ = orc.plane_basis(phi, th)
+        s = float(np.dot(c, nr)) + rng.uniform(-0.05, 0.05)
+        if planted == "sign":
+            got = -orc.plane_integral(vol, phi, th, s)
+        elif planted == "transpose":
+            got = orc.plane_integral(volT, phi, th, s)
+        else:  # one factor delta dropped from the term (value*delta)*delta
+            got = orc.plane_integral(vol, phi, th, s) * n
+        want = orc.ellipsoid_plane(c, ax, R, rho, nr, s)
+        worst = max(worst, abs(got - want))
+    assert worst > tol

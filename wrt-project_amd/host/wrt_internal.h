@@ -1,0 +1,56 @@
+/*
+ * wrt_internal.h -- private structures of the C host layer.
+ */
+#ifndef WRT_INTERNAL_H
+#define WRT_INTERNAL_H
+
+#include <hip/hip_runtime_api.h>
+#include <stddef.h>
+
+#include "../../include/wrt.h"
+#include "../csrc/wrt_dev.h"
+
+
+struct wrt_ctx {
+  int device;
+  hipStream_t stream;
+  hipEvent_t ev0, ev1, ev2, ev3;
+  int cu_count;
+  /* batch graph cache */
+  void* batch;
+};
+
+struct wrt_job {
+  wrt_ctx* ctx;
+  wrt_job_desc d;
+  /* owned host copies of the grids */
+  double *phi, *theta, *theta_w, *shift, *z;
+  size_t in_count, out_count;
+  /* device buffers */
+  void* d_in;      /* input, job precision/layout */
+  void* d_out;     /* output: double (F64) or float (F32/BF16) */
+  double* d_tabs;  /* all small tables in one allocation */
+  double* d_shift;
+  double* d_z;
+  double* d_dirs;  /* radon: [ndir][9]; ray: [nphi][4] */
+  double* d_ttrig; /* [ntheta][2] cos, sin */
+  double* d_ptrig; /* [nphi][2] cos, sin */
+  double* d_wgt;   /* [ntheta] */
+  double* d_fac;   /* [2*nshift] */
+  double* d_spl;   /* packed spline table */
+  void* d_work;    /* fast-path work lists / partials */
+  size_t work_bytes;
+  void* fast;      /* fast-path plan (wrt_fast.h) */
+  int loaded;
+  size_t bytes;
+};
+
+/* host-side tables (C library sin/cos, the reference's bits) */
+void wrt_host_radon_dirs(int n, const double* phi, int nphi, const double* theta, int ntheta,
+                         double* dirs);
+void wrt_host_ray_trig(const double* phi, int nphi, double* trig);
+
+int wrt_hip(hipError_t e);
+int wrt_check_grid(const double* g, int count, int min_count);
+
+#endif

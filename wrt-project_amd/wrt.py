@@ -1,0 +1,315 @@
+"""ctypes binding of lib/libwrt.so (the C-ABI declared in include/wrt.h).
+
+Plumbing for tests and bench.py: numpy arrays in, numpy arrays out.  The binding loads the
+in-tree library and raises WrtError if it is missing -- there is no fallback path.
+
+Array layouts follow the reference's files: volumes vol[x][y][z]; radon [s][phi][theta];
+rays [z][s][phi]; adjoint output [z][x][y].
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libwrt.so")
+
+WRT_OK, WRT_E_ARG, WRT_E_NODEV, WRT_E_HIP, WRT_E_NOMEM, WRT_E_ADMIT, WRT_E_IO, WRT_E_STATE = (
+    0, -1, -2, -3, -4, -5, -6, -7)
+PREC = {"f64": 0, "f32": 1, "bf16": 2}
+OP = {"radon": 0, "ray": 1, "reduction": 2, "adjoint": 3, "radon_T": 4, "ray_T": 5}
+
+
+class WrtError(RuntimeError):
+    def __init__(self, code, where):
+        self.code = code
+        super().__init__(f"{where}: {strerror(code)} ({code})")
+
+
+class Stats(C.Structure):
+    _fields_ = [("seconds", C.c_double), ("kernel_seconds", C.c_double),
+                ("logical_samples", C.c_double), ("alg_bytes", C.c_double),
+                ("outputs", C.c_longlong)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class JobDesc(C.Structure):
+    _fields_ = [("op", C.c_int), ("prec", C.c_int), ("n", C.c_int),
+                ("phi", C.c_void_p), ("nphi", C.c_int),
+                ("theta", C.c_void_p), ("theta_w", C.c_void_p), ("ntheta", C.c_int),
+                ("shift", C.c_void_p), ("nshift", C.c_int),
+                ("z", C.c_void_p), ("nz", C.c_int)]
+
+
+_lib = None
+
+# every symbol declared in include/wrt.h
+ABI_SYMBOLS = [
+    "wrt_init", "wrt_free", "wrt_strerror", "wrt_mem_info", "wrt_gl_nodes",
+    "wrt_radon_grid_init", "wrt_ray_grid_init", "wrt_zslice_grid", "wrt_radon_grid",
+    "wrt_ray_grid", "wrt_radon_reduction", "wrt_radon_adjoint", "wrt_radon_grid_T",
+    "wrt_ray_grid_T", "wrt_job_bytes", "wrt_job_input_count", "wrt_job_output_count",
+    "wrt_job_create", "wrt_job_load", "wrt_job_load_synthetic", "wrt_job_run",
+    "wrt_job_fetch", "wrt_job_destroy", "wrt_batch_run", "wrt_phantom",
+    "wrt_phantom_params", "wrt_gaussian",
+]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise WrtError(WRT_E_NODEV, f"libwrt.so not built at {LIB_PATH}")
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        I = C.c_int
+        D = C.c_double
+        L.wrt_strerror.restype = C.c_char_p
+        L.wrt_strerror.argtypes = [I]
+        L.wrt_init.argtypes = [C.POINTER(P), I]
+        L.wrt_free.argtypes = [P]
+        L.wrt_free.restype = None
+        L.wrt_mem_info.argtypes = [P, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
+        L.wrt_gl_nodes.argtypes = [P, I, P, P]
+        L.wrt_radon_grid_init.argtypes = [P, I, I, I, P, P, P, P]
+        L.wrt_ray_grid_init.argtypes = [I, I, P, P]
+        L.wrt_zslice_grid.argtypes = [I, P]
+        L.wrt_radon_grid.argtypes = [P, P, I, P, I, P, I, P, I, I, P, P]
+        L.wrt_ray_grid.argtypes = [P, P, I, P, I, P, I, P, I, I, P, P]
+        L.wrt_radon_reduction.argtypes = [P, P, I, P, I, P, I, P, I, P, P]
+        L.wrt_radon_adjoint.argtypes = [P, P, P, I, P, P, I, P, I, I, I, P, P]
+        L.wrt_radon_grid_T.argtypes = [P, P, I, P, I, P, I, P, I, I, P, P]
+        L.wrt_ray_grid_T.argtypes = [P, P, I, P, I, P, I, P, I, I, P, P]
+        L.wrt_job_bytes.argtypes = [P]
+        L.wrt_job_bytes.restype = C.c_size_t
+        L.wrt_job_input_count.argtypes = [P]
+        L.wrt_job_input_count.restype = C.c_size_t
+        L.wrt_job_output_count.argtypes = [P]
+        L.wrt_job_output_count.restype = C.c_size_t
+        L.wrt_job_create.argtypes = [P, P, C.POINTER(P)]
+        L.wrt_job_load.argtypes = [P, P]
+        L.wrt_job_load_synthetic.argtypes = [P, C.c_ulonglong, I, D]
+        L.wrt_job_run.argtypes = [P, P]
+        L.wrt_job_fetch.argtypes = [P, P]
+        L.wrt_job_destroy.argtypes = [P]
+        L.wrt_job_destroy.restype = None
+        L.wrt_batch_run.argtypes = [P, P, I, P]
+        L.wrt_phantom.argtypes = [I, I, I, C.c_ulonglong, D, P]
+        L.wrt_phantom_params.argtypes = [I, C.c_ulonglong, P]
+        L.wrt_gaussian.argtypes = [C.c_size_t, C.c_ulonglong, P]
+        _lib = L
+    return _lib
+
+
+def strerror(code):
+    try:
+        return lib().wrt_strerror(code).decode()
+    except Exception:  # library missing
+        return f"error {code}"
+
+
+def _chk(rc, where):
+    if rc != WRT_OK:
+        raise WrtError(rc, where)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+# ------------------------------------------------------------------ host-only helpers ---
+def phantom(n, cell_centred=True, nell=10, seed=0, sigma=0.0):
+    out = np.empty((n, n, n))
+    _chk(lib().wrt_phantom(n, int(bool(cell_centred)), nell, seed, sigma, _p(out)), "wrt_phantom")
+    return out
+
+
+def phantom_params(nell, seed=0):
+    out = np.empty((nell, 16))
+    _chk(lib().wrt_phantom_params(nell, seed, _p(out)), "wrt_phantom_params")
+    return out
+
+
+def gaussian(shape, seed=0):
+    out = np.empty(shape)
+    _chk(lib().wrt_gaussian(out.size, seed, _p(out)), "wrt_gaussian")
+    return out
+
+
+def ray_grid_init(nshift, nphi):
+    s, p = np.empty(nshift), np.empty(nphi)
+    _chk(lib().wrt_ray_grid_init(nshift, nphi, _p(s), _p(p)), "wrt_ray_grid_init")
+    return s, p
+
+
+def zslice_grid(nz):
+    z = np.empty(nz)
+    _chk(lib().wrt_zslice_grid(nz, _p(z)), "wrt_zslice_grid")
+    return z
+
+
+# ---------------------------------------------------------------------------- context ---
+class Context:
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        _chk(lib().wrt_init(C.byref(h), device), "wrt_init")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().wrt_free(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def mem_info(self):
+        f, t = C.c_size_t(), C.c_size_t()
+        _chk(lib().wrt_mem_info(self.h, C.byref(f), C.byref(t)), "wrt_mem_info")
+        return f.value, t.value
+
+    def gl_nodes(self, n):
+        x, w = np.empty(n), np.empty(n)
+        _chk(lib().wrt_gl_nodes(self.h, n, _p(x), _p(w)), "wrt_gl_nodes")
+        return x, w
+
+    def radon_grid_init(self, nphi, ntheta, nshift):
+        phi, th, w, s = np.empty(nphi), np.empty(ntheta), np.empty(ntheta), np.empty(nshift)
+        _chk(lib().wrt_radon_grid_init(self.h, nphi, ntheta, nshift, _p(phi), _p(th), _p(w),
+                                       _p(s)), "wrt_radon_grid_init")
+        return phi, th, w, s
+
+    def radon_grid(self, vol, phi, theta, shift, prec="f64"):
+        vol, phi, theta, shift = map(_f64, (vol, phi, theta, shift))
+        n = vol.shape[0]
+        out = np.empty((len(shift), len(phi), len(theta)))
+        st = Stats()
+        _chk(lib().wrt_radon_grid(self.h, _p(vol), n, _p(phi), len(phi), _p(theta), len(theta),
+                                  _p(shift), len(shift), PREC[prec], _p(out), C.byref(st)),
+             "wrt_radon_grid")
+        return out, st.as_dict()
+
+    def ray_grid(self, vol, z, shift, phi, prec="f64"):
+        vol, z, shift, phi = map(_f64, (vol, z, shift, phi))
+        n = vol.shape[0]
+        out = np.empty((len(z), len(shift), len(phi)))
+        st = Stats()
+        _chk(lib().wrt_ray_grid(self.h, _p(vol), n, _p(z), len(z), _p(shift), len(shift),
+                                _p(phi), len(phi), PREC[prec], _p(out), C.byref(st)),
+             "wrt_ray_grid")
+        return out, st.as_dict()
+
+    def radon_reduction(self, rays, shift, phi, theta):
+        rays, shift, phi, theta = map(_f64, (rays, shift, phi, theta))
+        out = np.empty((len(shift), len(phi), len(theta)))
+        st = Stats()
+        _chk(lib().wrt_radon_reduction(self.h, _p(rays), rays.shape[0], _p(shift), len(shift),
+                                       _p(phi), len(phi), _p(theta), len(theta), _p(out),
+                                       C.byref(st)), "wrt_radon_reduction")
+        return out, st.as_dict()
+
+    def radon_adjoint(self, radon, phi, theta, theta_w, shift, n, prec="f64"):
+        radon, phi, theta, theta_w, shift = map(_f64, (radon, phi, theta, theta_w, shift))
+        out = np.empty((n, n, n))
+        st = Stats()
+        _chk(lib().wrt_radon_adjoint(self.h, _p(radon), _p(phi), len(phi), _p(theta),
+                                     _p(theta_w), len(theta), _p(shift), len(shift), n,
+                                     PREC[prec], _p(out), C.byref(st)), "wrt_radon_adjoint")
+        return out, st.as_dict()
+
+    def radon_grid_T(self, g, n, phi, theta, shift, prec="f64"):
+        g, phi, theta, shift = map(_f64, (g, phi, theta, shift))
+        out = np.empty((n, n, n))
+        st = Stats()
+        _chk(lib().wrt_radon_grid_T(self.h, _p(g), n, _p(phi), len(phi), _p(theta), len(theta),
+                                    _p(shift), len(shift), PREC[prec], _p(out), C.byref(st)),
+             "wrt_radon_grid_T")
+        return out, st.as_dict()
+
+    def ray_grid_T(self, g, n, z, shift, phi, prec="f64"):
+        g, z, shift, phi = map(_f64, (g, z, shift, phi))
+        out = np.empty((n, n, n))
+        st = Stats()
+        _chk(lib().wrt_ray_grid_T(self.h, _p(g), n, _p(z), len(z), _p(shift), len(shift),
+                                  _p(phi), len(phi), PREC[prec], _p(out), C.byref(st)),
+             "wrt_ray_grid_T")
+        return out, st.as_dict()
+
+    def job(self, op, prec="f64", n=0, phi=None, theta=None, theta_w=None, shift=None, z=None,
+            nz=None):
+        return Job(self, op, prec, n, phi, theta, theta_w, shift, z, nz)
+
+    def batch_run(self, jobs):
+        arr = (C.c_void_p * len(jobs))(*[j.h for j in jobs])
+        st = Stats()
+        _chk(lib().wrt_batch_run(self.h, arr, len(jobs), C.byref(st)), "wrt_batch_run")
+        return st.as_dict()
+
+
+def make_desc(op, prec="f64", n=0, phi=None, theta=None, theta_w=None, shift=None, z=None,
+              nz=None):
+    keep = [None if a is None else _f64(a) for a in (phi, theta, theta_w, shift, z)]
+    phi, theta, theta_w, shift, z = keep
+    d = JobDesc(OP[op], PREC[prec], n, _p(phi), 0 if phi is None else len(phi), _p(theta),
+                _p(theta_w), 0 if theta is None else len(theta), _p(shift),
+                0 if shift is None else len(shift), _p(z),
+                nz if nz is not None else (0 if z is None else len(z)))
+    return d, keep
+
+
+class Job:
+    def __init__(self, ctx, op, prec, n, phi, theta, theta_w, shift, z, nz):
+        self.ctx = ctx
+        self.desc, self._keep = make_desc(op, prec, n, phi, theta, theta_w, shift, z, nz)
+        h = C.c_void_p()
+        _chk(lib().wrt_job_create(ctx.h, C.byref(self.desc), C.byref(h)), "wrt_job_create")
+        self.h = h
+        self.nin = lib().wrt_job_input_count(C.byref(self.desc))
+        self.nout = lib().wrt_job_output_count(C.byref(self.desc))
+
+    @property
+    def bytes(self):
+        return lib().wrt_job_bytes(C.byref(self.desc))
+
+    def load(self, a):
+        a = _f64(a)
+        assert a.size == self.nin, (a.size, self.nin)
+        _chk(lib().wrt_job_load(self.h, _p(a)), "wrt_job_load")
+
+    def load_synthetic(self, seed, nell=10, sigma=0.0):
+        _chk(lib().wrt_job_load_synthetic(self.h, seed, nell, sigma), "wrt_job_load_synthetic")
+
+    def run(self):
+        st = Stats()
+        _chk(lib().wrt_job_run(self.h, C.byref(st)), "wrt_job_run")
+        return st.as_dict()
+
+    def fetch(self):
+        out = np.empty(self.nout)
+        _chk(lib().wrt_job_fetch(self.h, _p(out)), "wrt_job_fetch")
+        return out
+
+    def close(self):
+        if self.h:
+            lib().wrt_job_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def job_bytes(op, prec="f64", **kw):
+    d, keep = make_desc(op, prec, **kw)
+    return lib().wrt_job_bytes(C.byref(d))
