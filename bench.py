@@ -1,0 +1,163 @@
+"""Benchmark of the plane-integral hot path (radon_grid) on one MI355X.
+
+Workload (the metric's configuration): a 256^3 seeded ellipsoid phantom with N(0, 0.01^2)
+texture, FP32 bricks, Gauss-uniform plane grid 256 phi x 128 Gauss-Legendre theta x 257
+shifts -- 8,421,376 planes, 16 n^2 = 1,048,576 lattice samples each (8.83e12 plane samples
+per step).  A step is one pass of the FP32 form over that grid with the volume resident in
+HBM.  Printed: one JSON line with plane samples/s, the dominant kernel's roofline
+(algorithmic bytes from the exact in-cube sample count), the FP32 form's relative L2 and
+max error against the plain FP64 form, the plain FP64 form's rate on a subset of the same
+grid, and a CPU baseline (the oracle on the host cores, a bounded sample of planes).
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "wrt-project_amd"))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, dtype=np.float64).ravel()
+    b = np.asarray(b, dtype=np.float64).ravel()
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def cpu_baseline(vol, phi, th, s, seconds=15.0):
+    """The oracle (plain C, -O2, OpenMP) on the host cores, on seeded planes of the grid."""
+    import orc
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    rng = np.random.default_rng(1234)
+    n = vol.shape[0]
+    m = max(threads, 8)
+    while True:
+        ip, it, js = rng.integers(len(phi), size=m), rng.integers(len(th), size=m), rng.integers(len(s), size=m)
+        t0 = time.perf_counter()
+        orc.radon_planes(vol, phi[ip], th[it], s[js])
+        dt = time.perf_counter() - t0
+        if dt >= seconds / 4 or m >= 1 << 20:
+            break
+        m = int(m * min(16.0, max(2.0, seconds / 4 / max(dt, 1e-3))))
+    rate = m * 16.0 * n * n / dt
+    return {"value": rate, "unit": "plane samples/s", "cores": threads, "kind": "port",
+            "sample": f"{m} seeded uniform planes of the {n}^3 grid, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--nphi", type=int, default=256)
+    ap.add_argument("--ntheta", type=int, default=128)
+    ap.add_argument("--nshift", type=int, default=257)
+    ap.add_argument("--prec", default="f32")
+    ap.add_argument("--plain-phi-stride", type=int, default=16,
+                    help="plain FP64 reference on every k-th phi (rate and relL2)")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(every box here has one GPU)", file=sys.stderr)
+        sys.exit(2)
+
+    import wrt
+
+    ctx = wrt.Context(0)
+    n = args.n
+    log(f"bench: phantom {n}^3")
+    vol = wrt.phantom(n, True, 10, 0, 0.01)
+    phi, th, w, s = ctx.radon_grid_init(args.nphi, args.ntheta, args.nshift)
+    nplanes = len(phi) * len(th) * len(s)
+    logical = 16.0 * n * n * nplanes
+
+    job = ctx.job("radon", args.prec, n=n, phi=phi, theta=th, shift=s)
+    job.load(vol)
+    for i in range(args.warmup):
+        st = job.run()
+        log(f"bench: warmup {i} {st['seconds']:.3f} s")
+    ks = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        st = job.run()
+        ks.append(st["kernel_seconds"])
+    wall = time.perf_counter() - t0
+    log(f"bench: {args.steps} steps {wall:.3f} s, kernel avg {np.mean(ks):.4f} s")
+    fast = job.fetch().reshape(len(s), len(phi), len(th))
+    job.close()
+
+    # plain FP64 form on a phi subset: its rate, and the FP32 form's error against it
+    sub = phi[::args.plain_phi_stride]
+    pj = ctx.job("radon", "f64", n=n, phi=sub, theta=th, shift=s)
+    pj.load(vol)
+    pst = pj.run()
+    plain = pj.fetch().reshape(len(s), len(sub), len(th))
+    pj.close()
+    plain_rate = 16.0 * n * n * plain.size / pst["kernel_seconds"]
+    fs = fast[:, ::args.plain_phi_stride, :]
+    err_l2 = rel_l2(fs, plain)
+    err_max = float(np.max(np.abs(fs - plain)) / np.max(np.abs(plain)))
+    log(f"bench: plain f64 subset {pst['kernel_seconds']:.3f} s, relL2 {err_l2:.3e}")
+
+    incube = ctx.radon_count_incube(n, phi, th, s)
+    eb = 2 if args.prec == "bf16" else 4
+    alg_bytes = incube * eb + nplanes * 4 + n ** 3 * eb
+    kavg = float(np.mean(ks))
+    achieved = alg_bytes / kavg
+    ms = wall / args.steps * 1e3
+    value = logical / (wall / args.steps)
+
+    cpu = None
+    if not args.no_cpu:
+        log("bench: cpu baseline")
+        cpu = cpu_baseline(vol, phi, th, s)
+    ctx.close()
+
+    out = {
+        "metric": "plane samples/s at 256^3 FP32 (relL2<=1e-5 vs FP64) vs plain; p99 served latency",
+        "value": value,
+        "unit": "plane samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.prec,
+        "data": "synthetic (seeded 10-ellipsoid phantom + N(0,0.01^2) texture)",
+        "config": {"workload": f"radon_grid {n}^3 {args.prec} bricks, {len(phi)} phi x {len(th)} "
+                               f"GL theta x {len(s)} shifts", "n": n, "planes": nplanes,
+                   "logical_samples": logical, "incube_samples": incube,
+                   "index_arithmetic": "binary64, reference rounding"},
+        "rel_l2_vs_f64": err_l2,
+        "max_err_vs_f64": err_max,
+        "plain_f64": {"value": plain_rate, "unit": "plane samples/s",
+                      "subset": f"every {args.plain_phi_stride}th phi ({plain.size} planes)"},
+        "speedup_vs_plain": value / plain_rate,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK, "unit": "B/s",
+                     "frac": achieved / HBM_PEAK, "traffic": None,
+                     "kernel": "k_radon_fast (interior + boundary launches)",
+                     "kernel_seconds": kavg},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

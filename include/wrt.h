@@ -132,6 +132,13 @@ int wrt_ray_grid_T(wrt_ctx* ctx, const double* g, int n, const double* z, int nz
                    const double* shift, int nshift, const double* phi, int nphi, wrt_prec prec,
                    double* out, wrt_stats* st);
 
+/* Work accounting (no reference counterpart): the number of lattice samples of the whole
+ * plane grid that pass zero_interp's cube test (radon_grid/src/interp.c:65-66), i.e. that
+ * read a voxel; computed exactly on the GPU with the reference's arithmetic. */
+int wrt_radon_count_incube(wrt_ctx* ctx, int n, const double* phi, int nphi,
+                           const double* theta, int ntheta, const double* shift, int nshift,
+                           double* count);
+
 /* ---------------------------------------------- device-resident jobs (batch/serve) --- */
 /* A job owns its device input, output and tables; load/run/fetch may repeat.  The grids
  * passed at creation are copied.  Unused grid fields may be NULL/0 for an op:

@@ -699,3 +699,35 @@ double orc_ellipsoid_line(const double c[3], const double axes[3], const double 
   if (disc <= 0) return 0;
   return rho * 2 * sqrt(disc) / a;
 }
+
+/* ------------------------------------------------------------------------------------ */
+/* arithmetic property used by the fast kernels                                          */
+/* ------------------------------------------------------------------------------------ */
+
+/* Number of u in [0, 2] (random, and within a few ulps of every multiple k*d) for which the
+ * corrected reciprocal q = fma(fma(-y, d, u), inv, y), y = u*inv, inv = RN(1/d), differs
+ * from the IEEE quotient u/d, for the cell width d = RN(2/n) of zero_interp.  u ranges over
+ * the values x + 1 can take for a double x >= -1: zero or at least 2^-53 (subnormal u, where
+ * the reciprocal form loses its guarantee, cannot occur). */
+long long orc_div_rn_mismatches(int n, long long count, unsigned long long seed) {
+  const double d = 2.0 / n, inv = 1.0 / d;
+  unsigned long long s = seed ? seed : 1;
+  long long bad = 0;
+  for (long long i = 0; i < count; ++i) {
+    s ^= s << 13; s ^= s >> 7; s ^= s << 17;
+    double u = (double)(s >> 11) * (2.0 / 9007199254740992.0);
+    if (u != 0.0 && u < 0x1p-53) continue;
+    const double y = u * inv, r = fma(-y, d, u), q = fma(r, inv, y);
+    bad += q != u / d;
+  }
+  for (int k = 0; k <= n; ++k) {
+    double u = k * d;
+    for (int t = 0; t < 8; ++t) u = nextafter(u, -1.0);
+    for (int t = 0; t < 17; ++t, u = nextafter(u, 3.0)) {
+      if (u < 0 || u > 2.0 || (u != 0.0 && u < 0x1p-53)) continue;
+      const double y = u * inv, r = fma(-y, d, u), q = fma(r, inv, y);
+      bad += q != u / d;
+    }
+  }
+  return bad;
+}

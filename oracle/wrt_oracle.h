@@ -88,6 +88,9 @@ double orc_ellipsoid_plane(const double c[3], const double axes[3], const double
 double orc_ellipsoid_line(const double c[3], const double axes[3], const double rot[9],
                           double rho, const double p0[3], const double dir[3]);
 
+/* ---- arithmetic property of the fast kernels' division ---- */
+long long orc_div_rn_mismatches(int n, long long count, unsigned long long seed);
+
 #ifdef __cplusplus
 }
 #endif

@@ -38,8 +38,14 @@ def test_radon_grid_init(ctx):
     assert np.array_equal(s, orc.shift_grid(129))
 
 
+def _uniq(a):
+    """distinct values in order of first appearance"""
+    _, idx = np.unique(a, return_index=True)
+    return a[np.sort(idx)]
+
+
 def _unique_grid(phi, theta, shift):
-    up, ut, us = np.unique(phi), np.unique(theta), np.unique(shift)
+    up, ut, us = _uniq(phi), _uniq(theta), _uniq(shift)
     if len(up) * len(ut) * len(us) != len(phi):
         return None
     return up, ut, us
@@ -94,7 +100,7 @@ def test_ray_f64_golden_bitwise(ctx):
         n = int(f[k + "_n"])
         z, s, phi = f[k + "_in"]
         vol = volume_formula(n)
-        uz, us, up = np.unique(z), np.unique(s), np.unique(phi)
+        uz, us, up = _uniq(z), _uniq(s), _uniq(phi)
         if len(uz) * len(us) * len(up) == len(z):
             out, _ = ctx.ray_grid(vol, uz, us, up)
             got = out.ravel()  # golden meshgrid order (z, s, phi) == product order
@@ -185,6 +191,13 @@ def test_transposes_f64_dot_and_oracle(ctx):
     ATg, _ = ctx.ray_grid_T(g, n, z, s, phi)
     assert abs(np.sum(Af * g) - np.sum(f * ATg)) <= 1e-12 * np.linalg.norm(Af) * np.linalg.norm(g)
     assert orc.rel_l2(ATg, orc.ray_grid_T(g, n, z, s, phi)) <= 1e-13
+
+
+def test_incube_count_matches_oracle(ctx):
+    for n, (nphi, nth, ns) in ((16, (8, 4, 9)), (33, (6, 3, 7)), (64, (4, 4, 5))):
+        phi, th, w, s = ctx.radon_grid_init(nphi, nth, ns)
+        want = sum(orc.plane_incube(n, p, t, v) for v in s for p in phi for t in th)
+        assert ctx.radon_count_incube(n, phi, th, s) == want
 
 
 def test_invalid_arguments(ctx):

@@ -280,3 +280,15 @@ def test_checks_catch_planted_mistakes(planted):
         want = orc.ellipsoid_plane(c, ax, R, rho, nr, s)
         worst = max(worst, abs(got - want))
     assert worst > tol
+
+
+def test_corrected_reciprocal_division_is_ieee():
+    # the fast plane kernel replaces (x+1)/(2/n) by a corrected reciprocal when n is not a
+    # power of two; it must decide every cell index exactly as the IEEE quotient does
+    import ctypes as C
+    f = orc.L().orc_div_rn_mismatches
+    f.restype = C.c_longlong
+    f.argtypes = [C.c_int, C.c_longlong, C.c_ulonglong]
+    for n in list(range(3, 1025)):
+        if n & (n - 1):
+            assert f(n, 2000, n) == 0, n

@@ -1,0 +1,277 @@
+// k_fast.hip -- fast form of the plane integrals (radon_grid, FP32 or BF16 values).
+//
+// Work unit: one workgroup per (non-empty 32^3 brick, group of directions).  The brick is
+// staged once into LDS as FP32; every plane (shift, direction) that cuts the brick is a
+// "section".  A wavefront walks a section in 8-row bands: lane (r, k) = (lane>>3, lane&7)
+// handles lattice row i1 = band + r and columns i2 = c0 + k, c0 stepping by 8 across the
+// band's conservative column interval.  Per sample the cell index is decided exactly as
+// radon_grid/src/integral.c:47-51 + interp.c:70-72 decide it in binary64
+//   x = ((s*n) + (a*i1)) + (b*i2),  ix = (int)((x + 1) / (2/n)),  index n -> n-1,
+// (the per-column products b*i2 come from a per-wave LDS table, the division is an exact
+// power-of-two scaling or a corrected reciprocal), so only the value rounding (FP32/BF16)
+// and the summation order differ from the plain form.  Samples whose cell lies in the
+// brick add its FP32 value to a per-band partial, folded into a Kahan-compensated FP32
+// lane sum; a section's lane sums are reduced over the wavefront in FP64 and added to the
+// plane's FP64 accumulator (one atomic per section and brick).
+#include "wrt_common.hpp"
+#include "wrt_fast.h"
+
+#include <limits.h>
+
+#define FB WRT_BRICK
+#define FR_WAVES 8
+#define FR_COLS 136
+// skewed LDS pitches: bank = (5*lx + ly + lz) mod 32, so a small patch of a plane spreads
+// over the banks for every orientation except a few measure-zero ones
+#define FR_PY 33
+#define FR_PX (33 * 32 + 5)
+
+static_assert(FB == 32, "the band walk assumes 32^3 bricks");
+
+// RN(u / d) for d = RN(2/n) when n is not a power of two: reciprocal estimate plus one
+// exact-residual correction (Markstein); equal to the IEEE quotient for these operands
+// (checked on the host against the C division for every n <= 1024 in the tests).
+__device__ __forceinline__ double div_rn(double u, double d, double inv) {
+  const double y = u * inv;
+  const double r = fma(-y, d, u);
+  return fma(r, inv, y);
+}
+
+__device__ __forceinline__ float bf16_to_f32(unsigned short v) {
+  return __uint_as_float(((unsigned)v) << 16);
+}
+
+template <bool BND, bool BF16, bool POW2>
+__global__ __launch_bounds__(FR_WAVES * 64) void k_radon_fast(wrt_radon_fast_args a, double h,
+                                                              double dc, double inv_dc) {
+  __shared__ float sb[FB * FR_PX];
+  __shared__ double scol[FR_WAVES][3][FR_COLS];
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int brick = a.tasks[2 * blockIdx.x], grp = a.tasks[2 * blockIdx.x + 1];
+  const int nb = a.nb, n = a.n;
+  const int bxi = brick / (nb * nb), byi = (brick / nb) % nb, bzi = brick % nb;
+  const int b0x = bxi * FB, b0y = byi * FB, b0z = bzi * FB;
+
+  // stage the brick into the skewed LDS layout
+  {
+    const size_t base = (size_t)brick * (FB * FB * FB);
+    for (int i = threadIdx.x; i < FB * FB * FB; i += blockDim.x) {
+      const int lx = i >> 10, ly = (i >> 5) & 31, lz = i & 31;
+      float v;
+      if (BF16) v = bf16_to_f32(((const unsigned short*)a.bricks)[base + i]);
+      else v = ((const float*)a.bricks)[base + i];
+      sb[lx * FR_PX + ly * FR_PY + lz] = v;
+    }
+  }
+  __syncthreads();
+
+  const double delta = (1.0 / (n));
+  const double lo[3] = {-1.0 + b0x * dc, -1.0 + b0y * dc, -1.0 + b0z * dc};
+  const double hi[3] = {-1.0 + (b0x + FB) * dc, -1.0 + (b0y + FB) * dc, -1.0 + (b0z + FB) * dc};
+  const double fb0x = (double)b0x, fb0y = (double)b0y, fb0z = (double)b0z;
+  const int L0 = -2 * n, L1 = 2 * n - 1;  // lattice index range of i1 and i2
+  const int r_lane = lane >> 3, k_lane = lane & 7;
+  double* colx = &scol[wv][0][0];
+  double* coly = &scol[wv][1][0];
+  double* colz = &scol[wv][2][0];
+
+  const int d_begin = grp * a.dgroup;
+  const int d_end = min(d_begin + a.dgroup, a.ndir);
+  for (int d = d_begin + wv; d < d_end; d += FR_WAVES) {
+    const double* D = a.dirs + 9 * (size_t)d;
+    const double nrm[3] = {D[0], D[1], D[2]};
+    const double av[3] = {D[3], D[4], D[5]};
+    const double bv[3] = {D[6], D[7], D[8]};
+    // projections of the brick box on a, b and the normal (box corners extremise them)
+    double pa0 = 0, pa1 = 0, pb0 = 0, pb1 = 0, pn0 = 0, pn1 = 0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      pa0 += fmin(lo[c] * av[c], hi[c] * av[c]);
+      pa1 += fmax(lo[c] * av[c], hi[c] * av[c]);
+      pb0 += fmin(lo[c] * bv[c], hi[c] * bv[c]);
+      pb1 += fmax(lo[c] * bv[c], hi[c] * bv[c]);
+      pn0 += fmin(lo[c] * nrm[c], hi[c] * nrm[c]);
+      pn1 += fmax(lo[c] * nrm[c], hi[c] * nrm[c]);
+    }
+    const double aa = av[0] * av[0] + av[1] * av[1] + av[2] * av[2];
+    const double bb = bv[0] * bv[0] + bv[1] * bv[1] + bv[2] * bv[2];
+    const int R0 = max(L0, (int)floor(pa0 / aa) - 2), R1 = min(L1, (int)ceil(pa1 / aa) + 2);
+    const int C0 = max(L0, (int)floor(pb0 / bb) - 2), C1 = min(L1, (int)ceil(pb1 / bb) + 2);
+    if (R0 > R1 || C0 > C1 || C1 - C0 + 8 > FR_COLS) continue;  // (C1-C0 <= 117 by geometry)
+    // shifts with s in [pn0 - eps, pn1 + eps] (sorted shift grid, binary search)
+    const double eps = 4.0 * delta;
+    int j0, j1;
+    {
+      int l = 0, r = a.nshift;
+      while (l < r) { const int m = (l + r) >> 1; if (a.shift[m] < pn0 - eps) l = m + 1; else r = m; }
+      j0 = l;
+      l = 0; r = a.nshift;
+      while (l < r) { const int m = (l + r) >> 1; if (a.shift[m] <= pn1 + eps) l = m + 1; else r = m; }
+      j1 = l - 1;
+    }
+    if (j0 > j1) continue;
+    // per-column products b*i2, rounded as the reference rounds delta*v2*i2
+    __builtin_amdgcn_wave_barrier();
+    for (int k = lane; k < FR_COLS; k += 64) {
+      const double c = (double)(C0 + k);
+      colx[k] = bv[0] * c;
+      coly[k] = bv[1] * c;
+      colz[k] = bv[2] * c;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+    const int nbands = (R1 - R0 + 8) >> 3;
+    for (int j = j0; j <= j1; ++j) {
+      const double s = a.shift[j];
+      const double S[3] = {s * nrm[0], s * nrm[1], s * nrm[2]};
+      // lane k < nbands: conservative column interval of band k (rows R0+8k .. +7)
+      int blo = INT_MAX, bhi = INT_MIN;
+      if (lane < nbands) {
+        const int r0 = R0 + 8 * lane, r1 = min(r0 + 7, R1);
+        double tlo = -1e300, thi = 1e300;
+        bool empty = false;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const double w0 = S[c] + av[c] * r0, w1 = S[c] + av[c] * r1;
+          const double wmin = fmin(w0, w1), wmax = fmax(w0, w1);
+          const double tol = 1e-9 * dc;
+          if (fabs(bv[c]) * (4.0 * n) < tol) {
+            if (wmax < lo[c] - tol || wmin > hi[c] + tol) empty = true;
+          } else if (bv[c] > 0) {
+            tlo = fmax(tlo, (lo[c] - tol - wmax) / bv[c]);
+            thi = fmin(thi, (hi[c] + tol - wmin) / bv[c]);
+          } else {
+            tlo = fmax(tlo, (hi[c] + tol - wmin) / bv[c]);
+            thi = fmin(thi, (lo[c] - tol - wmax) / bv[c]);
+          }
+        }
+        if (!empty && tlo <= thi) {
+          tlo = fmax(tlo, (double)C0 - 1.0);
+          thi = fmin(thi, (double)C1 + 1.0);
+          blo = max(C0, (int)floor(tlo) - 1);
+          bhi = min(C1, (int)ceil(thi) + 1);
+        }
+      }
+      float ks = 0.f, kc = 0.f;  // Kahan-compensated lane sum of the section
+      for (int k = 0; k < nbands; ++k) {
+        const int clo = __shfl(blo, k, 64), chi = __shfl(bhi, k, 64);
+        if (clo > chi) continue;
+        const int i1 = R0 + 8 * k + r_lane;
+        const bool rowok = i1 <= R1;
+        const double f1 = (double)i1;
+        const double Rx = S[0] + av[0] * f1, Ry = S[1] + av[1] * f1, Rz = S[2] + av[2] * f1;
+        float bs = 0.f;
+        for (int c0 = clo; c0 <= chi; c0 += 8) {
+          const int c = c0 + k_lane;
+          const int ci = c - C0;
+          const double x = Rx + colx[ci], y = Ry + coly[ci], z = Rz + colz[ci];
+          bool ok = rowok && (c <= L1);
+          if (BND) ok = ok && (fabs(x) <= 1.0) && (fabs(y) <= 1.0) && (fabs(z) <= 1.0);
+          const double ux = x + 1.0, uy = y + 1.0, uz = z + 1.0;
+          double qx, qy, qz;
+          if (POW2) {
+            qx = ux * h; qy = uy * h; qz = uz * h;
+          } else {
+            qx = div_rn(ux, dc, inv_dc); qy = div_rn(uy, dc, inv_dc); qz = div_rn(uz, dc, inv_dc);
+          }
+          qx -= fb0x; qy -= fb0y; qz -= fb0z;
+          ok = ok && (qx >= 0.0) && (qy >= 0.0) && (qz >= 0.0);
+          int ix = (int)qx, iy = (int)qy, iz = (int)qz;
+          if (BND) {
+            if (ix + b0x == n) ix -= 1;
+            if (iy + b0y == n) iy -= 1;
+            if (iz + b0z == n) iz -= 1;
+          }
+          ok = ok && ((unsigned)ix < (unsigned)FB) && ((unsigned)iy < (unsigned)FB) &&
+               ((unsigned)iz < (unsigned)FB);
+          const int addr = ok ? (ix * FR_PX + iy * FR_PY + iz) : 0;
+          const float v = sb[addr];
+          bs += ok ? v : 0.f;
+        }
+        const float y = bs - kc;
+        const float t = ks + y;
+        kc = (t - ks) - y;
+        ks = t;
+      }
+      double tot = (double)ks - (double)kc;
+      tot = wrt_wave_sum(tot);
+      if (lane == 0 && tot != 0.0) unsafeAtomicAdd(a.acc + (size_t)j * a.ndir + d, tot * delta * delta);
+    }
+  }
+}
+
+template <bool BND, bool BF16>
+static int launch_pow2(const wrt_radon_fast_args* a, bool pow2, hipStream_t st) {
+  const double n = a->n;
+  const double dc = 2.0 / a->n;
+  const double inv = 1.0 / dc;
+  const double h = n / 2.0;
+  if (pow2)
+    k_radon_fast<BND, BF16, true><<<a->ntasks, FR_WAVES * 64, 0, st>>>(*a, h, dc, inv);
+  else
+    k_radon_fast<BND, BF16, false><<<a->ntasks, FR_WAVES * 64, 0, st>>>(*a, h, dc, inv);
+  WRT_LAUNCH_CHECK();
+  return WRT_OK;
+}
+
+extern "C" int wrtk_radon_fast(const wrt_radon_fast_args* a, int boundary, hipStream_t st) {
+  if (a->ntasks <= 0) return WRT_OK;
+  const bool pow2 = (a->n & (a->n - 1)) == 0;
+  if (boundary) return a->bf16 ? launch_pow2<true, true>(a, pow2, st) : launch_pow2<true, false>(a, pow2, st);
+  return a->bf16 ? launch_pow2<false, true>(a, pow2, st) : launch_pow2<false, false>(a, pow2, st);
+}
+
+// ------------------------------------------------------------------- brick loader ---
+__device__ __forceinline__ unsigned short f32_to_bf16_rne(float f) {
+  unsigned u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (unsigned short)(u >> 16);  // inf / nan
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+
+__global__ __launch_bounds__(256) void k_brickify(const double* __restrict__ vol, int n, int nb,
+                                                  int bf16, void* __restrict__ bricks,
+                                                  int* __restrict__ flags) {
+  const size_t total = (size_t)nb * nb * nb * (FB * FB * FB);
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const size_t brick = i / (FB * FB * FB);
+    const int li = (int)(i % (FB * FB * FB));
+    const int bx = (int)(brick / ((size_t)nb * nb)), by = (int)((brick / nb) % nb), bz = (int)(brick % nb);
+    const int x = bx * FB + (li >> 10), y = by * FB + ((li >> 5) & 31), z = bz * FB + (li & 31);
+    double v = 0.0;
+    if (x < n && y < n && z < n) v = vol[((size_t)x * n + y) * n + z];
+    const float f = (float)v;
+    if (bf16) ((unsigned short*)bricks)[i] = f32_to_bf16_rne(f);
+    else ((float*)bricks)[i] = f;
+    if (v != 0.0) flags[brick] = 1;
+  }
+}
+
+extern "C" int wrtk_brickify(const double* vol, int n, int nb, int bf16, void* bricks, int* flags,
+                             hipStream_t st) {
+  const size_t total = (size_t)nb * nb * nb * (FB * FB * FB);
+  size_t blocks = (total + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  k_brickify<<<(unsigned)blocks, 256, 0, st>>>(vol, n, nb, bf16, bricks, flags);
+  WRT_LAUNCH_CHECK();
+  return WRT_OK;
+}
+
+__global__ void k_acc_to_f32(const double* __restrict__ acc, float* __restrict__ out, size_t count) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+       i += (size_t)gridDim.x * blockDim.x)
+    out[i] = (float)acc[i];
+}
+
+extern "C" int wrtk_acc_to_f32(const double* acc, float* out, size_t count, hipStream_t st) {
+  size_t blocks = (count + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  if (!blocks) return WRT_OK;
+  k_acc_to_f32<<<(unsigned)blocks, 256, 0, st>>>(acc, out, count);
+  WRT_LAUNCH_CHECK();
+  return WRT_OK;
+}

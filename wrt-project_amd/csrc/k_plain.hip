@@ -54,6 +54,7 @@ extern "C" int wrtk_radon_plain(const double* vol, int n, const double* dirs, in
   const long long nplanes = (long long)ndir * nshift;
   if (nplanes <= 0) return WRT_OK;
   const long long blocks = (nplanes + 3) / 4;
+  if (blocks * 256 >= (1LL << 32)) return WRT_E_ARG;  // a dispatch holds < 2^32 work-items
   k_radon_plain<<<dim3((unsigned)blocks), dim3(256), 0, st>>>(vol, n, dirs, ndir, shift,
                                                                nplanes, out);
   WRT_LAUNCH_CHECK();
@@ -441,6 +442,7 @@ extern "C" int wrtk_radon_T_plain(const double* g, int n, const double* dirs, in
                                   const double* shift, int nshift, double* out, hipStream_t st) {
   const long long nplanes = (long long)ndir * nshift;
   if (nplanes <= 0) return WRT_OK;
+  if (((nplanes + 3) / 4) * 256 >= (1LL << 32)) return WRT_E_ARG;
   k_radon_T_plain<<<dim3((unsigned)((nplanes + 3) / 4)), dim3(256), 0, st>>>(
       g, n, dirs, ndir, shift, nplanes, out);
   WRT_LAUNCH_CHECK();
@@ -536,6 +538,80 @@ extern "C" int wrtk_f64_to_f32(const double* src, float* dst, size_t count, hipS
 }
 extern "C" int wrtk_f32_to_f64(const float* src, double* dst, size_t count, hipStream_t st) {
   k_f32_to_f64<<<grid_for(count), 256, 0, st>>>(src, dst, count);
+  WRT_LAUNCH_CHECK();
+  return WRT_OK;
+}
+
+// -------------------------------------------------------------- in-cube sample count ---
+// Number of lattice samples of each plane that pass zero_interp's cube test, with the
+// reference's arithmetic.  Along a lattice row every coordinate x(i2) = RN(R + RN(b*i2)) is
+// monotone in i2, so each row's in-cube set is an interval; its ends are found by bisection
+// on the exact predicate.  One thread per (plane, row).
+__device__ __forceinline__ bool in_axis(double R, double b, int i2) {
+  const double x = R + b * (double)i2;
+  return !(fabs(x) > 1.0);
+}
+
+__global__ __launch_bounds__(256) void k_radon_incube(int n, const double* __restrict__ dirs,
+                                                      int ndir, const double* __restrict__ shift,
+                                                      long long nplanes,
+                                                      unsigned long long* __restrict__ total) {
+  const int rows = 4 * n;
+  const long long work = nplanes * rows;
+  unsigned long long cnt = 0;
+  // grid-stride: a dispatch holds fewer than 2^32 work-items
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < work;
+       t += (long long)gridDim.x * blockDim.x) {
+    const long long plane = t / rows;
+    const int i1 = (int)(t % rows) - 2 * n;
+    const int d = (int)(plane % ndir), is = (int)(plane / ndir);
+    const double* D = dirs + 9 * (size_t)d;
+    const double s = shift[is];
+    int lo = -2 * n, hi = 2 * n - 1;
+    for (int c = 0; c < 3 && lo <= hi; ++c) {
+      const double R = s * D[c] + D[3 + c] * (double)i1;
+      const double b = D[6 + c];
+      if (b == 0.0) {
+        if (!in_axis(R, b, 0)) lo = hi + 1;
+        continue;
+      }
+      // x(i2) is non-decreasing for b > 0, non-increasing for b < 0; "x <= 1" and "x >= -1"
+      // are each monotone: bisect for the first and last index satisfying both
+      auto ok_hi = [&](int i) { const double x = R + b * (double)i; return !(x > 1.0); };
+      auto ok_lo = [&](int i) { const double x = R + b * (double)i; return !(x < -1.0); };
+      int a0, a1;
+      if (b > 0) {
+        int l = lo, r = hi + 1;
+        while (l < r) { const int m = l + ((r - l) >> 1); if (ok_lo(m)) r = m; else l = m + 1; }
+        a0 = l;
+        l = lo - 1; r = hi;
+        while (l < r) { const int m = l + ((r - l + 1) >> 1); if (ok_hi(m)) l = m; else r = m - 1; }
+        a1 = l;
+      } else {
+        int l = lo, r = hi + 1;
+        while (l < r) { const int m = l + ((r - l) >> 1); if (ok_hi(m)) r = m; else l = m + 1; }
+        a0 = l;
+        l = lo - 1; r = hi;
+        while (l < r) { const int m = l + ((r - l + 1) >> 1); if (ok_lo(m)) l = m; else r = m - 1; }
+        a1 = l;
+      }
+      lo = max(lo, a0);
+      hi = min(hi, a1);
+    }
+    if (hi >= lo) cnt += (unsigned long long)(hi - lo + 1);
+  }
+  for (int off = 32; off > 0; off >>= 1) cnt += __shfl_down(cnt, off, 64);
+  if ((threadIdx.x & 63) == 0 && cnt) atomicAdd(total, cnt);
+}
+
+extern "C" int wrtk_radon_incube(int n, const double* dirs, int ndir, const double* shift,
+                                 int nshift, unsigned long long* total, hipStream_t st) {
+  const long long nplanes = (long long)ndir * nshift;
+  const long long threads = nplanes * 4 * n;
+  if (threads <= 0) return WRT_OK;
+  long long blocks = (threads + 255) / 256;
+  if (blocks > (1LL << 20)) blocks = 1LL << 20;
+  k_radon_incube<<<dim3((unsigned)blocks), 256, 0, st>>>(n, dirs, ndir, shift, nplanes, total);
   WRT_LAUNCH_CHECK();
   return WRT_OK;
 }

@@ -57,6 +57,10 @@ int wrtk_radon_T_plain(const double* g, int n, const double* dirs, int ndir,
 int wrtk_ray_T_plain(const double* g, int n, const double* z, int nz, const double* shift,
                      int nshift, const double* trig, int nphi, double* out, hipStream_t st);
 
+/* number of lattice samples passing zero_interp's cube test, summed over all planes */
+int wrtk_radon_incube(int n, const double* dirs, int ndir, const double* shift, int nshift,
+                      unsigned long long* total, hipStream_t st);
+
 /* utilities */
 int wrtk_fill_f64(double* p, double v, size_t count, hipStream_t st);
 int wrtk_f64_to_f32(const double* src, float* dst, size_t count, hipStream_t st);

@@ -582,3 +582,31 @@ int wrt_ray_grid_T(wrt_ctx* c, const double* g, int n, const double* z, int nz,
   wrt_job_desc d = {WRT_OP_RAY_T, prec, n, phi, nphi, NULL, NULL, 0, shift, nshift, z, nz};
   return run_once(c, &d, g, out, st);
 }
+
+int wrt_radon_count_incube(wrt_ctx* c, int n, const double* phi, int nphi, const double* theta,
+                           int ntheta, const double* shift, int nshift, double* count) {
+  if (!c || !count || n < 1 || wrt_check_grid(phi, nphi, 1) || wrt_check_grid(theta, ntheta, 1) ||
+      wrt_check_grid(shift, nshift, 1))
+    return WRT_E_ARG;
+  HIPCHK(hipSetDevice(c->device));
+  const size_t ndir = (size_t)nphi * ntheta;
+  double* h = (double*)malloc(sizeof(double) * (9 * ndir + nshift));
+  if (!h) return WRT_E_NOMEM;
+  wrt_host_radon_dirs(n, phi, nphi, theta, ntheta, h);
+  memcpy(h + 9 * ndir, shift, sizeof(double) * nshift);
+  double* d = NULL;
+  unsigned long long* tot = NULL;
+  int rc = wrt_hip(hipMalloc((void**)&d, sizeof(double) * (9 * ndir + nshift)));
+  if (rc == WRT_OK) rc = wrt_hip(hipMalloc((void**)&tot, sizeof(unsigned long long)));
+  if (rc == WRT_OK) rc = wrt_hip(hipMemcpyAsync(d, h, sizeof(double) * (9 * ndir + nshift), hipMemcpyHostToDevice, c->stream));
+  if (rc == WRT_OK) rc = wrt_hip(hipMemsetAsync(tot, 0, sizeof(unsigned long long), c->stream));
+  if (rc == WRT_OK) rc = wrtk_radon_incube(n, d, (int)ndir, d + 9 * ndir, nshift, tot, c->stream);
+  unsigned long long v = 0;
+  if (rc == WRT_OK) rc = wrt_hip(hipMemcpyAsync(&v, tot, sizeof(v), hipMemcpyDeviceToHost, c->stream));
+  if (rc == WRT_OK) rc = wrt_hip(hipStreamSynchronize(c->stream));
+  *count = (double)v;
+  hipFree(d);
+  hipFree(tot);
+  free(h);
+  return rc;
+}

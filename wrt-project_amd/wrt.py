@@ -53,7 +53,7 @@ ABI_SYMBOLS = [
     "wrt_ray_grid_T", "wrt_job_bytes", "wrt_job_input_count", "wrt_job_output_count",
     "wrt_job_create", "wrt_job_load", "wrt_job_load_synthetic", "wrt_job_run",
     "wrt_job_fetch", "wrt_job_destroy", "wrt_batch_run", "wrt_phantom",
-    "wrt_phantom_params", "wrt_gaussian",
+    "wrt_phantom_params", "wrt_gaussian", "wrt_radon_count_incube",
 ]
 
 
@@ -99,6 +99,7 @@ def lib():
         L.wrt_phantom.argtypes = [I, I, I, C.c_ulonglong, D, P]
         L.wrt_phantom_params.argtypes = [I, C.c_ulonglong, P]
         L.wrt_gaussian.argtypes = [C.c_size_t, C.c_ulonglong, P]
+        L.wrt_radon_count_incube.argtypes = [P, I, P, I, P, I, P, I, P]
         _lib = L
     return _lib
 
@@ -243,6 +244,14 @@ class Context:
                                   _p(phi), len(phi), PREC[prec], _p(out), C.byref(st)),
              "wrt_ray_grid_T")
         return out, st.as_dict()
+
+    def radon_count_incube(self, n, phi, theta, shift):
+        phi, theta, shift = map(_f64, (phi, theta, shift))
+        out = C.c_double()
+        _chk(lib().wrt_radon_count_incube(self.h, n, _p(phi), len(phi), _p(theta), len(theta),
+                                          _p(shift), len(shift), C.byref(out)),
+             "wrt_radon_count_incube")
+        return out.value
 
     def job(self, op, prec="f64", n=0, phi=None, theta=None, theta_w=None, shift=None, z=None,
             nz=None):
