@@ -7,7 +7,7 @@
 // band's conservative column interval.  Per sample the cell index is decided exactly as
 // radon_grid/src/integral.c:47-51 + interp.c:70-72 decide it in binary64
 //   x = ((s*n) + (a*i1)) + (b*i2),  ix = (int)((x + 1) / (2/n)),  index n -> n-1,
-// (the per-column products b*i2 come from a per-wave LDS table, the division is an exact
+// (the per-column products b*i2 are formed in registers, the division is an exact
 // power-of-two scaling or a corrected reciprocal), so only the value rounding (FP32/BF16)
 // and the summation order differ from the plain form.  Samples whose cell lies in the
 // brick add its FP32 value to a per-band partial, folded into a Kahan-compensated FP32
@@ -19,8 +19,7 @@
 #include <limits.h>
 
 #define FB WRT_BRICK
-#define FR_WAVES 8
-#define FR_COLS 136
+#define FR_WAVES 16
 // skewed LDS pitches: bank = (5*lx + ly + lz) mod 32, so a small patch of a plane spreads
 // over the banks for every orientation except a few measure-zero ones
 #define FR_PY 33
@@ -37,6 +36,35 @@ __device__ __forceinline__ double div_rn(double u, double d, double inv) {
   return fma(r, inv, y);
 }
 
+// a*b + c on the 24-bit multiplier (full rate); a, b < 2^24.  Written out because the
+// compiler turns small known-range products into the quarter-rate 32-bit multiply.
+__device__ __forceinline__ unsigned mad_u24(unsigned a, unsigned b, unsigned c) {
+  unsigned r;
+  asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+  return r;
+}
+
+// Wavefront sum of a double with DPP lane moves (no LDS round trips): Hillis-Steele scan
+// within 16-lane rows (row_shr 1, 2, 4, 8, zero fill), then row broadcasts 15 and 31; the
+// total ends in lane 63 and is returned to every lane.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_move(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWMASK, 0xf, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWMASK, 0xf, true);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double wave_sum_dpp(double v) {
+  v += dpp_move<0x111, 0xf>(v);  // row_shr:1
+  v += dpp_move<0x112, 0xf>(v);  // row_shr:2
+  v += dpp_move<0x114, 0xf>(v);  // row_shr:4
+  v += dpp_move<0x118, 0xf>(v);  // row_shr:8
+  v += dpp_move<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+  v += dpp_move<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
+  return __hiloint2double(hi, lo);
+}
+
 __device__ __forceinline__ float bf16_to_f32(unsigned short v) {
   return __uint_as_float(((unsigned)v) << 16);
 }
@@ -44,8 +72,10 @@ __device__ __forceinline__ float bf16_to_f32(unsigned short v) {
 template <bool BND, bool BF16, bool POW2>
 __global__ __launch_bounds__(FR_WAVES * 64) void k_radon_fast(wrt_radon_fast_args a, double h,
                                                               double dc, double inv_dc) {
-  __shared__ float sb[FB * FR_PX];
-  __shared__ double scol[FR_WAVES][3][FR_COLS];
+  // the brick, plus one zero slot that rejected samples read (branch-free accumulation)
+  __shared__ float sb[FB * FR_PX + 1];
+  // per wave: first column of every row of the current section
+  __shared__ int rowlo[FR_WAVES][136];
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int brick = a.tasks[2 * blockIdx.x], grp = a.tasks[2 * blockIdx.x + 1];
@@ -63,8 +93,10 @@ __global__ __launch_bounds__(FR_WAVES * 64) void k_radon_fast(wrt_radon_fast_arg
       else v = ((const float*)a.bricks)[base + i];
       sb[lx * FR_PX + ly * FR_PY + lz] = v;
     }
+    if (threadIdx.x == 0) sb[FB * FR_PX] = 0.f;
   }
   __syncthreads();
+  const unsigned zero_off = 4u * (FB * FR_PX);
 
   const double delta = (1.0 / (n));
   const double lo[3] = {-1.0 + b0x * dc, -1.0 + b0y * dc, -1.0 + b0z * dc};
@@ -72,9 +104,6 @@ __global__ __launch_bounds__(FR_WAVES * 64) void k_radon_fast(wrt_radon_fast_arg
   const double fb0x = (double)b0x, fb0y = (double)b0y, fb0z = (double)b0z;
   const int L0 = -2 * n, L1 = 2 * n - 1;  // lattice index range of i1 and i2
   const int r_lane = lane >> 3, k_lane = lane & 7;
-  double* colx = &scol[wv][0][0];
-  double* coly = &scol[wv][1][0];
-  double* colz = &scol[wv][2][0];
 
   const int d_begin = grp * a.dgroup;
   const int d_end = min(d_begin + a.dgroup, a.ndir);
@@ -98,7 +127,7 @@ __global__ __launch_bounds__(FR_WAVES * 64) void k_radon_fast(wrt_radon_fast_arg
     const double bb = bv[0] * bv[0] + bv[1] * bv[1] + bv[2] * bv[2];
     const int R0 = max(L0, (int)floor(pa0 / aa) - 2), R1 = min(L1, (int)ceil(pa1 / aa) + 2);
     const int C0 = max(L0, (int)floor(pb0 / bb) - 2), C1 = min(L1, (int)ceil(pb1 / bb) + 2);
-    if (R0 > R1 || C0 > C1 || C1 - C0 + 8 > FR_COLS) continue;  // (C1-C0 <= 117 by geometry)
+    if (R0 > R1 || C0 > C1) continue;
     // shifts with s in [pn0 - eps, pn1 + eps] (sorted shift grid, binary search)
     const double eps = 4.0 * delta;
     int j0, j1;
@@ -111,93 +140,111 @@ __global__ __launch_bounds__(FR_WAVES * 64) void k_radon_fast(wrt_radon_fast_arg
       j1 = l - 1;
     }
     if (j0 > j1) continue;
-    // per-column products b*i2, rounded as the reference rounds delta*v2*i2
-    __builtin_amdgcn_wave_barrier();
-    for (int k = lane; k < FR_COLS; k += 64) {
-      const double c = (double)(C0 + k);
-      colx[k] = bv[0] * c;
-      coly[k] = bv[1] * c;
-      colz[k] = bv[2] * c;
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-
-    const int nbands = (R1 - R0 + 8) >> 3;
+    const int nrows = R1 - R0 + 1;  // <= 122 by geometry (brick diameter + margins)
+    // reciprocals of the column steps (for the conservative row intervals)
+    double ib[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) ib[c] = (fabs(bv[c]) * (4.0 * n) < 1e-9 * dc) ? 0.0 : 1.0 / bv[c];
+    const int nbands = (nrows + 7) >> 3;
     for (int j = j0; j <= j1; ++j) {
       const double s = a.shift[j];
       const double S[3] = {s * nrm[0], s * nrm[1], s * nrm[2]};
-      // lane k < nbands: conservative column interval of band k (rows R0+8k .. +7)
-      int blo = INT_MAX, bhi = INT_MIN;
-      if (lane < nbands) {
-        const int r0 = R0 + 8 * lane, r1 = min(r0 + 7, R1);
-        double tlo = -1e300, thi = 1e300;
-        bool empty = false;
+      // conservative column interval of every row (rows R0 + lane and R0 + 64 + lane): first
+      // column into the wave's row table, 8-column step count reduced to a per-band maximum
+      int bsteps[2];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const double w0 = S[c] + av[c] * r0, w1 = S[c] + av[c] * r1;
-          const double wmin = fmin(w0, w1), wmax = fmax(w0, w1);
-          const double tol = 1e-9 * dc;
-          if (fabs(bv[c]) * (4.0 * n) < tol) {
-            if (wmax < lo[c] - tol || wmin > hi[c] + tol) empty = true;
-          } else if (bv[c] > 0) {
-            tlo = fmax(tlo, (lo[c] - tol - wmax) / bv[c]);
-            thi = fmin(thi, (hi[c] + tol - wmin) / bv[c]);
-          } else {
-            tlo = fmax(tlo, (hi[c] + tol - wmin) / bv[c]);
-            thi = fmin(thi, (lo[c] - tol - wmax) / bv[c]);
+      for (int half = 0; half < 2; ++half) {
+        const int t = lane + 64 * half;
+        int rl = C0, nst = 0;
+        if (t < nrows) {
+          const double f1 = (double)(R0 + t);
+          double tlo = -1e300, thi = 1e300;
+          bool empty = false;
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            const double w = S[c] + av[c] * f1;
+            const double tol = 1e-9 * dc;
+            if (ib[c] == 0.0) {
+              if (w < lo[c] - tol || w > hi[c] + tol) empty = true;
+            } else if (bv[c] > 0) {
+              tlo = fmax(tlo, (lo[c] - tol - w) * ib[c]);
+              thi = fmin(thi, (hi[c] + tol - w) * ib[c]);
+            } else {
+              tlo = fmax(tlo, (hi[c] + tol - w) * ib[c]);
+              thi = fmin(thi, (lo[c] - tol - w) * ib[c]);
+            }
+          }
+          if (!empty && tlo <= thi + 2.0) {
+            tlo = fmax(tlo, (double)C0 - 1.0);
+            thi = fmin(thi, (double)C1 + 1.0);
+            rl = max(C0, (int)floor(tlo) - 1);
+            const int rh = min(C1, (int)ceil(thi) + 1);
+            nst = rh >= rl ? ((rh - rl + 8) >> 3) : 0;
           }
         }
-        if (!empty && tlo <= thi) {
-          tlo = fmax(tlo, (double)C0 - 1.0);
-          thi = fmin(thi, (double)C1 + 1.0);
-          blo = max(C0, (int)floor(tlo) - 1);
-          bhi = min(C1, (int)ceil(thi) + 1);
-        }
+        rowlo[wv][t] = rl;
+        // max over the 8 consecutive rows of a band (lanes 8b .. 8b+7)
+        nst = max(nst, __shfl_xor(nst, 1, 64));
+        nst = max(nst, __shfl_xor(nst, 2, 64));
+        nst = max(nst, __shfl_xor(nst, 4, 64));
+        bsteps[half] = nst;
       }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       float ks = 0.f, kc = 0.f;  // Kahan-compensated lane sum of the section
+      int rlo_next = rowlo[wv][r_lane];
       for (int k = 0; k < nbands; ++k) {
-        const int clo = __shfl(blo, k, 64), chi = __shfl(bhi, k, 64);
-        if (clo > chi) continue;
-        const int i1 = R0 + 8 * k + r_lane;
-        const bool rowok = i1 <= R1;
+        const int t = 8 * k + r_lane;  // this lane's row within the section
+        const int rlo = rlo_next;
+        if (k + 1 < nbands) rlo_next = rowlo[wv][t + 8];
+        const int steps = (k < 8) ? __builtin_amdgcn_readlane(bsteps[0], 8 * k)
+                                  : __builtin_amdgcn_readlane(bsteps[1], 8 * (k - 8));
+        if (steps == 0) continue;
+        const int i1 = R0 + t;
+        const bool rowok = t < nrows;
         const double f1 = (double)i1;
         const double Rx = S[0] + av[0] * f1, Ry = S[1] + av[1] * f1, Rz = S[2] + av[2] * f1;
         float bs = 0.f;
-        for (int c0 = clo; c0 <= chi; c0 += 8) {
-          const int c = c0 + k_lane;
-          const int ci = c - C0;
-          const double x = Rx + colx[ci], y = Ry + coly[ci], z = Rz + colz[ci];
+        int c = rlo + k_lane;
+        for (int m = 0; m < steps; ++m, c += 8) {
+          // per-column products b*i2, rounded as the reference rounds delta*v2*i2
+          const double f2 = (double)c;
+          const double x = Rx + bv[0] * f2, y = Ry + bv[1] * f2, z = Rz + bv[2] * f2;
           bool ok = rowok && (c <= L1);
           if (BND) ok = ok && (fabs(x) <= 1.0) && (fabs(y) <= 1.0) && (fabs(z) <= 1.0);
-          const double ux = x + 1.0, uy = y + 1.0, uz = z + 1.0;
           double qx, qy, qz;
           if (POW2) {
-            qx = ux * h; qy = uy * h; qz = uz * h;
+            // RN(x + 1) * h == RN(x*h + h) for h = n/2 a power of two: scaling by 2^k
+            // commutes with rounding, so one fma gives the reference's quotient exactly
+            qx = fma(x, h, h); qy = fma(y, h, h); qz = fma(z, h, h);
           } else {
-            qx = div_rn(ux, dc, inv_dc); qy = div_rn(uy, dc, inv_dc); qz = div_rn(uz, dc, inv_dc);
+            qx = div_rn(x + 1.0, dc, inv_dc);
+            qy = div_rn(y + 1.0, dc, inv_dc);
+            qz = div_rn(z + 1.0, dc, inv_dc);
           }
-          qx -= fb0x; qy -= fb0y; qz -= fb0z;
-          ok = ok && (qx >= 0.0) && (qy >= 0.0) && (qz >= 0.0);
+          // truncation as in zero_interp; for an interior brick a negative quotient (x < -1)
+          // truncates to 0 <= b0 - 32 and a quotient above n lands past the brick, so the
+          // brick test below also is the cube test; bricks on the faces test |x| <= 1 above
           int ix = (int)qx, iy = (int)qy, iz = (int)qz;
           if (BND) {
-            if (ix + b0x == n) ix -= 1;
-            if (iy + b0y == n) iy -= 1;
-            if (iz + b0z == n) iz -= 1;
+            if (ix == n) ix = n - 1;
+            if (iy == n) iy = n - 1;
+            if (iz == n) iz = n - 1;
           }
-          ok = ok && ((unsigned)ix < (unsigned)FB) && ((unsigned)iy < (unsigned)FB) &&
-               ((unsigned)iz < (unsigned)FB);
-          const int addr = ok ? (ix * FR_PX + iy * FR_PY + iz) : 0;
-          const float v = sb[addr];
-          bs += ok ? v : 0.f;
+          const int lx = ix - b0x, ly = iy - b0y, lz = iz - b0z;
+          ok = ok && ((unsigned)lx < (unsigned)FB) && ((unsigned)ly < (unsigned)FB) &&
+               ((unsigned)lz < (unsigned)FB);
+          const unsigned off = mad_u24((unsigned)lx, 4u * FR_PX,
+                                       mad_u24((unsigned)ly, 4u * FR_PY, (unsigned)lz << 2));
+          bs += *(const float*)((const char*)sb + (ok ? off : zero_off));
         }
         const float y = bs - kc;
-        const float t = ks + y;
-        kc = (t - ks) - y;
-        ks = t;
+        const float tt = ks + y;
+        kc = (tt - ks) - y;
+        ks = tt;
       }
-      double tot = (double)ks - (double)kc;
-      tot = wrt_wave_sum(tot);
+      const double tot = wave_sum_dpp((double)ks - (double)kc);
       if (lane == 0 && tot != 0.0) unsafeAtomicAdd(a.acc + (size_t)j * a.ndir + d, tot * delta * delta);
     }
   }
