@@ -102,7 +102,11 @@ __global__ __launch_bounds__(FR_WAVES * 64) void k_radon_fast(wrt_radon_fast_arg
   const double lo[3] = {-1.0 + b0x * dc, -1.0 + b0y * dc, -1.0 + b0z * dc};
   const double hi[3] = {-1.0 + (b0x + FB) * dc, -1.0 + (b0y + FB) * dc, -1.0 + (b0z + FB) * dc};
   const double fb0x = (double)b0x, fb0y = (double)b0y, fb0z = (double)b0z;
-  const int L0 = -2 * n, L1 = 2 * n - 1;  // lattice index range of i1 and i2
+  const int L0 = -2 * n, L1 = 2 * n - 1;  // lattice index range of i1 and i2 (bounds of R0..R1, C0..C1)
+  // axes along which the brick touches a face of the cube (only those need the cube test;
+  // along the others x < -1 truncates outside the brick and x > 1 lands past it)
+  const bool fx = b0x == 0 || b0x + FB >= n, fy = b0y == 0 || b0y + FB >= n,
+             fz = b0z == 0 || b0z + FB >= n;
   const int r_lane = lane >> 3, k_lane = lane & 7;
 
   const int d_begin = grp * a.dgroup;
@@ -160,6 +164,21 @@ __global__ __launch_bounds__(FR_WAVES * 64) void k_radon_fast(wrt_radon_fast_arg
           const double f1 = (double)(R0 + t);
           double tlo = -1e300, thi = 1e300;
           bool empty = false;
+          // the row's points p(t) = w + b t within the support sphere |p|^2 <= rho2 (every
+          // sample outside it reads a zero cell, so clipping the row there changes no sum)
+          {
+            const double wx = S[0] + av[0] * f1, wy = S[1] + av[1] * f1, wz = S[2] + av[2] * f1;
+            const double qb = wx * bv[0] + wy * bv[1] + wz * bv[2];
+            const double qc = wx * wx + wy * wy + wz * wz - a.rho2;
+            const double disc = qb * qb - bb * qc;
+            if (disc < 0.0) {
+              empty = true;
+            } else {
+              const double sq = sqrt(disc);
+              tlo = (-qb - sq) / bb;
+              thi = (-qb + sq) / bb;
+            }
+          }
 #pragma unroll
           for (int c = 0; c < 3; ++c) {
             const double w = S[c] + av[c] * f1;
@@ -175,10 +194,12 @@ __global__ __launch_bounds__(FR_WAVES * 64) void k_radon_fast(wrt_radon_fast_arg
             }
           }
           if (!empty && tlo <= thi + 2.0) {
+            // the continuous bounds are accurate to ~1e-12 columns (and the slab tolerance
+            // widens them), so the integer columns within 1e-6 of them are the candidates
             tlo = fmax(tlo, (double)C0 - 1.0);
             thi = fmin(thi, (double)C1 + 1.0);
-            rl = max(C0, (int)floor(tlo) - 1);
-            const int rh = min(C1, (int)ceil(thi) + 1);
+            rl = max(C0, (int)ceil(tlo - 1e-6));
+            const int rh = min(C1, (int)floor(thi + 1e-6));
             nst = rh >= rl ? ((rh - rl + 8) >> 3) : 0;
           }
         }
@@ -202,7 +223,9 @@ __global__ __launch_bounds__(FR_WAVES * 64) void k_radon_fast(wrt_radon_fast_arg
                                   : __builtin_amdgcn_readlane(bsteps[1], 8 * (k - 8));
         if (steps == 0) continue;
         const int i1 = R0 + t;
-        const bool rowok = t < nrows;
+        // Rows past R1 and columns past the lattice end need no test: a plane's cube section
+        // lies within sqrt(3) < 2 of the plane centre, inside the [-2n, 2n) lattice, so only
+        // the brick test below can reject a sample.
         const double f1 = (double)i1;
         const double Rx = S[0] + av[0] * f1, Ry = S[1] + av[1] * f1, Rz = S[2] + av[2] * f1;
         float bs = 0.f;
@@ -211,8 +234,12 @@ __global__ __launch_bounds__(FR_WAVES * 64) void k_radon_fast(wrt_radon_fast_arg
           // per-column products b*i2, rounded as the reference rounds delta*v2*i2
           const double f2 = (double)c;
           const double x = Rx + bv[0] * f2, y = Ry + bv[1] * f2, z = Rz + bv[2] * f2;
-          bool ok = rowok && (c <= L1);
-          if (BND) ok = ok && (fabs(x) <= 1.0) && (fabs(y) <= 1.0) && (fabs(z) <= 1.0);
+          bool ok = true;
+          if (BND) {  // cube test on the axes along which the brick touches a face
+            if (fx) ok = ok && (fabs(x) <= 1.0);
+            if (fy) ok = ok && (fabs(y) <= 1.0);
+            if (fz) ok = ok && (fabs(z) <= 1.0);
+          }
           double qx, qy, qz;
           if (POW2) {
             // RN(x + 1) * h == RN(x*h + h) for h = n/2 a power of two: scaling by 2^k
@@ -228,13 +255,13 @@ __global__ __launch_bounds__(FR_WAVES * 64) void k_radon_fast(wrt_radon_fast_arg
           // brick test below also is the cube test; bricks on the faces test |x| <= 1 above
           int ix = (int)qx, iy = (int)qy, iz = (int)qz;
           if (BND) {
-            if (ix == n) ix = n - 1;
-            if (iy == n) iy = n - 1;
-            if (iz == n) iz = n - 1;
+            if (fx && ix == n) ix = n - 1;
+            if (fy && iy == n) iy = n - 1;
+            if (fz && iz == n) iz = n - 1;
           }
           const int lx = ix - b0x, ly = iy - b0y, lz = iz - b0z;
-          ok = ok && ((unsigned)lx < (unsigned)FB) && ((unsigned)ly < (unsigned)FB) &&
-               ((unsigned)lz < (unsigned)FB);
+          // all three offsets in [0, 32): a negative offset sets high bits of the OR
+          ok = ok && (((unsigned)lx | (unsigned)ly | (unsigned)lz) < (unsigned)FB);
           const unsigned off = mad_u24((unsigned)lx, 4u * FR_PX,
                                        mad_u24((unsigned)ly, 4u * FR_PY, (unsigned)lz << 2));
           bs += *(const float*)((const char*)sb + (ok ? off : zero_off));
@@ -281,8 +308,10 @@ __device__ __forceinline__ unsigned short f32_to_bf16_rne(float f) {
 
 __global__ __launch_bounds__(256) void k_brickify(const double* __restrict__ vol, int n, int nb,
                                                   int bf16, void* __restrict__ bricks,
-                                                  int* __restrict__ flags) {
+                                                  int* __restrict__ flags,
+                                                  unsigned long long* __restrict__ r2max) {
   const size_t total = (size_t)nb * nb * nb * (FB * FB * FB);
+  double r2 = 0.0;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
        i += (size_t)gridDim.x * blockDim.x) {
     const size_t brick = i / (FB * FB * FB);
@@ -294,16 +323,28 @@ __global__ __launch_bounds__(256) void k_brickify(const double* __restrict__ vol
     const float f = (float)v;
     if (bf16) ((unsigned short*)bricks)[i] = f32_to_bf16_rne(f);
     else ((float*)bricks)[i] = f;
-    if (v != 0.0) flags[brick] = 1;
+    if (v != 0.0) {
+      flags[brick] = 1;
+      const double cx = -1.0 + (2.0 * x + 1.0) / n, cy = -1.0 + (2.0 * y + 1.0) / n,
+                   cz = -1.0 + (2.0 * z + 1.0) / n;
+      r2 = fmax(r2, cx * cx + cy * cy + cz * cz);
+    }
   }
+  // non-negative doubles order like their bit patterns
+  unsigned long long b = (unsigned long long)__double_as_longlong(r2);
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_down(b, off, 64);
+    b = o > b ? o : b;
+  }
+  if ((threadIdx.x & 63) == 0 && b) atomicMax(r2max, b);
 }
 
 extern "C" int wrtk_brickify(const double* vol, int n, int nb, int bf16, void* bricks, int* flags,
-                             hipStream_t st) {
+                             unsigned long long* r2max, hipStream_t st) {
   const size_t total = (size_t)nb * nb * nb * (FB * FB * FB);
   size_t blocks = (total + 255) / 256;
   if (blocks > 65536) blocks = 65536;
-  k_brickify<<<(unsigned)blocks, 256, 0, st>>>(vol, n, nb, bf16, bricks, flags);
+  k_brickify<<<(unsigned)blocks, 256, 0, st>>>(vol, n, nb, bf16, bricks, flags, r2max);
   WRT_LAUNCH_CHECK();
   return WRT_OK;
 }

@@ -30,11 +30,16 @@ typedef struct {
   const double* shift;
   int nshift;
   double* acc;        /* [nshift][ndir] FP64 accumulators, zeroed by caller   */
+  double rho2;        /* squared support radius: every sample farther from the
+                         origin reads a zero cell (cell-centre radius of the
+                         farthest non-zero cell plus half a cell diagonal)     */
 } wrt_radon_fast_args;
 
-/* FP64 vol[x][y][z] -> bricks; flags[brick] = 1 if any value is non-zero */
+/* FP64 vol[x][y][z] -> bricks; flags[brick] = 1 if any value is non-zero;
+ * r2max = largest squared cell-centre radius of a non-zero cell (as the bit pattern of a
+ * non-negative double, so unsigned max orders it), zeroed by the caller */
 int wrtk_brickify(const double* vol, int n, int nb, int bf16, void* bricks, int* flags,
-                  hipStream_t st);
+                  unsigned long long* r2max, hipStream_t st);
 /* boundary = 1: the tasks' bricks touch the cube's faces (cube test and edge clamp) */
 int wrtk_radon_fast(const wrt_radon_fast_args* a, int boundary, hipStream_t st);
 /* out[i] = (float)acc[i] */

@@ -8,6 +8,7 @@
  * FP32 on the device, which is the output of the form.
  */
 #include <stdlib.h>
+#include <math.h>
 #include <string.h>
 
 #include "../csrc/wrt_fast.h"
@@ -22,7 +23,8 @@ typedef struct {
   int nb;
   size_t nbricks;
   void* bricks;
-  int* flags;
+  int* flags;       /* [nbricks] non-zero flags, then the support radius (8 bytes) */
+  double rho2;      /* squared support radius of the loaded volume */
   int* tasks;       /* device, interior then boundary */
   int n_interior, n_boundary, dgroup;
   double* acc;      /* [nshift][ndir] */
@@ -72,7 +74,7 @@ int wrt_fast_create(wrt_job* j) {
   const size_t nout = (size_t)d->nshift * ndir;
   const size_t ngroups = (ndir + 7) / 8;
   if (hipMalloc(&f->bricks, f->nbricks * WRT_BRICK * WRT_BRICK * WRT_BRICK * eb) != hipSuccess ||
-      hipMalloc((void**)&f->flags, f->nbricks * sizeof(int)) != hipSuccess ||
+      hipMalloc((void**)&f->flags, f->nbricks * sizeof(int) + 16) != hipSuccess ||
       hipMalloc((void**)&f->tasks, 2 * f->nbricks * ngroups * sizeof(int)) != hipSuccess ||
       hipMalloc((void**)&f->acc, nout * sizeof(double)) != hipSuccess ||
       hipMalloc((void**)&f->out, nout * sizeof(float)) != hipSuccess ||
@@ -89,23 +91,34 @@ int wrt_fast_load(wrt_job* j, const double* in) {
   hipStream_t st = j->ctx->stream;
   const size_t n3 = (size_t)d->n * d->n * d->n;
   HIPCHK(hipMemcpyAsync(f->staging, in, n3 * sizeof(double), hipMemcpyHostToDevice, st));
-  HIPCHK(hipMemsetAsync(f->flags, 0, f->nbricks * sizeof(int), st));
-  int rc = wrtk_brickify(f->staging, d->n, f->nb, d->prec == WRT_BF16, f->bricks, f->flags, st);
+  /* flags, then (8-byte aligned) the squared support radius as ordered bits */
+  const size_t r2off = (f->nbricks * sizeof(int) + 7) & ~(size_t)7;
+  unsigned long long* d_r2 = (unsigned long long*)((char*)f->flags + r2off);
+  HIPCHK(hipMemsetAsync(f->flags, 0, r2off + 8, st));
+  int rc = wrtk_brickify(f->staging, d->n, f->nb, d->prec == WRT_BF16, f->bricks, f->flags, d_r2, st);
   if (rc != WRT_OK) return rc;
-  int* flags = (int*)malloc(f->nbricks * sizeof(int));
+  int* flags = (int*)malloc(r2off + 8);
   if (!flags) return WRT_E_NOMEM;
-  if (hipMemcpyAsync(flags, f->flags, f->nbricks * sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+  if (hipMemcpyAsync(flags, f->flags, r2off + 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
       hipStreamSynchronize(st) != hipSuccess) {
     free(flags);
     return WRT_E_HIP;
   }
+  {
+    /* a sample farther than (farthest non-zero cell centre + half a cell diagonal) lies in a
+     * zero cell; the small relative margin covers the rounding of the sample coordinates */
+    double r2;
+    memcpy(&r2, (char*)flags + r2off, sizeof(double));
+    const double rho = sqrt(r2) + sqrt(3.0) / d->n;
+    f->rho2 = rho * rho * (1.0 + 1e-12) + 1e-12;
+  }
   const int ndir = d->nphi * d->ntheta;
   int nonempty = 0;
   for (size_t b = 0; b < f->nbricks; ++b) nonempty += flags[b] != 0;
-  /* enough tasks to fill 256 CUs several times, at least 8 directions (one per wave) */
+  /* enough tasks to fill 256 CUs several times, at least 16 directions (one per wave) */
   long long want = (long long)ndir * (nonempty ? nonempty : 1) / 2048;
-  int dg = (int)(want < 8 ? 8 : (want > 512 ? 512 : want));
-  dg = (dg + 7) & ~7;
+  int dg = (int)(want < 16 ? 16 : (want > 512 ? 512 : want));
+  dg = (dg + 15) & ~15;
   const int ngroups = (ndir + dg - 1) / dg;
   f->dgroup = dg;
   int* t = (int*)malloc(2 * (f->nbricks * ngroups + 1) * sizeof(int));
@@ -157,6 +170,7 @@ int wrt_fast_enqueue(wrt_job* j) {
   a.shift = j->d_shift;
   a.nshift = d->nshift;
   a.acc = f->acc;
+  a.rho2 = f->rho2;
   a.tasks = f->tasks;
   a.ntasks = f->n_interior;
   rc = wrtk_radon_fast(&a, 0, st);
