@@ -150,9 +150,10 @@ def main():
         "plain_f64": {"value": plain_rate, "unit": "plane samples/s",
                       "subset": f"every {args.plain_phi_stride}th phi ({plain.size} planes)"},
         "speedup_vs_plain": value / plain_rate,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK, "unit": "B/s",
-                     "frac": achieved / HBM_PEAK, "traffic": None,
-                     "kernel": "k_radon_fast (interior + boundary launches)",
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+                     "alg_bytes": alg_bytes,
+                     "kernel": "k_radon_fast (interior + face-brick launches)",
                      "kernel_seconds": kavg},
         "cpu_baseline": cpu,
     }

@@ -172,8 +172,15 @@ int wrt_job_load(wrt_job* job, const double* in);
  * centred) and RAY (node centred), Gaussian data for the other ops */
 int wrt_job_load_synthetic(wrt_job* job, unsigned long long seed, int n_ellipsoids,
                            double texture_sigma);
-/* enqueue on the context stream and wait; fills st when non-NULL */
+/* enqueue on the job's own stream and wait; fills st when non-NULL */
 int wrt_job_run(wrt_job* job, wrt_stats* st);
+/* asynchronous form: wrt_job_launch enqueues the job's device work on its own stream and
+ * returns; launches of different jobs overlap on the device.  wrt_job_poll returns 1 while
+ * the work runs and WRT_OK (stats filled) once it is done; wrt_job_wait blocks.  A launched
+ * job accepts no load/launch/fetch until polled done or waited for (WRT_E_STATE). */
+int wrt_job_launch(wrt_job* job);
+int wrt_job_poll(wrt_job* job, wrt_stats* st);
+int wrt_job_wait(wrt_job* job, wrt_stats* st);
 int wrt_job_fetch(wrt_job* job, double* out);
 void wrt_job_destroy(wrt_job* job);
 

@@ -71,6 +71,12 @@ def test_job_sizes():
     assert wrt.job_bytes("radon", "f64", n=8, phi=phi, theta=th, shift=s) > 8 ** 3 * 8
     assert wrt.job_bytes("radon", "f64", n=0, phi=phi, theta=th, shift=s) == 0
     assert wrt.job_bytes("reduction", "f32", phi=phi, theta=th, shift=s, nz=8) == 0
+    # the FP32 adjoint needs a uniform shift grid (it locates intervals arithmetically)
+    w = np.ones(3)
+    assert wrt.job_bytes("adjoint", "f32", n=8, phi=phi, theta=th, theta_w=w, shift=s) > 0
+    s2 = s.copy()
+    s2[2] = 0.1
+    assert wrt.job_bytes("adjoint", "f32", n=8, phi=phi, theta=th, theta_w=w, shift=s2) == 0
 
 
 def test_init_without_device_reports_nodev():

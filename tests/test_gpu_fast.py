@@ -74,3 +74,54 @@ def test_fast_job_repeatable(ctx):
     b = j.fetch()
     j.close()
     assert orc.rel_l2(a, b) <= 1e-12
+
+
+def _adjoint_case(ctx, n, nphi, nth, ns, seed):
+    vol = wrt.phantom(n, True, 10, seed, 0.01)
+    phi, th, w, s = ctx.radon_grid_init(nphi, nth, ns)
+    radon, _ = ctx.radon_grid(vol, phi, th, s, prec="f64")
+    return radon, phi, th, w, s
+
+
+@pytest.mark.parametrize("n", [32, 48, 64])
+def test_fast_adjoint_f32_matches_plain(ctx, n):
+    radon, phi, th, w, s = _adjoint_case(ctx, n, 32, 16, n + 1, seed=n)
+    ref, _ = ctx.radon_adjoint(radon, phi, th, w, s, n, prec="f64")
+    got, st = ctx.radon_adjoint(radon, phi, th, w, s, n, prec="f32")
+    assert orc.rel_l2(got, ref) <= 1e-5, orc.rel_l2(got, ref)
+    assert np.max(np.abs(got - ref)) <= 1e-5 * np.max(np.abs(ref))
+    assert st["logical_samples"] == float(len(phi) * len(th) * n ** 3)
+
+
+def test_fast_adjoint_f32_oracle_points(ctx):
+    # noisy rows (every interval's cubic matters) against the oracle at a voxel sample
+    n = 24
+    phi, th, w, s = ctx.radon_grid_init(12, 8, 49)
+    radon = wrt.gaussian((len(s), len(phi), len(th)), seed=11)
+    got, _ = ctx.radon_adjoint(radon, phi, th, w, s, n, prec="f32")
+    want = orc.radon_adjoint(radon, phi, th, w, s, n)
+    assert orc.rel_l2(got, want) <= 1e-5, orc.rel_l2(got, want)
+
+
+def test_fast_adjoint_job_reuses_rows(ctx):
+    radon, phi, th, w, s = _adjoint_case(ctx, 32, 16, 8, 33, seed=9)
+    j = ctx.job("adjoint", "f32", n=32, phi=phi, theta=th, theta_w=w, shift=s)
+    j.load(radon)
+    j.run()
+    a = j.fetch()
+    j.load(2.0 * radon)
+    j.run()
+    b = j.fetch()
+    j.close()
+    assert orc.rel_l2(b, 2.0 * a) <= 1e-6
+
+
+@pytest.mark.parametrize("n,nphi,nth", [(128, 32, 16), (64, 128, 64)])
+def test_fast_adjoint_f32_cut_decisions(ctx, n, nphi, nth):
+    # rows that do not vanish at s = +-1: the |s| <= 1 cut must follow the FP64 decision
+    phi, th, w, s = ctx.radon_grid_init(nphi, nth, n + 1)
+    radon = wrt.gaussian((len(s), nphi, nth), seed=1)
+    ref, _ = ctx.radon_adjoint(radon, phi, th, w, s, n, prec="f64")
+    got, _ = ctx.radon_adjoint(radon, phi, th, w, s, n, prec="f32")
+    assert orc.rel_l2(got, ref) <= 1e-5, orc.rel_l2(got, ref)
+    assert np.max(np.abs(got - ref)) <= 1e-4 * np.max(np.abs(ref)), np.max(np.abs(got - ref))

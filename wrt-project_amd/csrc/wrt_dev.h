@@ -57,6 +57,14 @@ int wrtk_radon_T_plain(const double* g, int n, const double* dirs, int ndir,
 int wrtk_ray_T_plain(const double* g, int n, const double* z, int nz, const double* shift,
                      int nshift, const double* trig, int nphi, double* out, hipStream_t st);
 
+/* FP32 backprojection: per-interval FP32 cubics from the FP64 packed spline table, then one
+ * thread per voxel over float4 directions (n.x, n.y, n.z, weight); uniform shift grid.
+ * trig [ndir][4] = sin(theta) cos(phi) sin(phi) cos(theta) in FP64 (cut decisions) */
+int wrtk_adjoint_cubics(const double* x, int ns, const double* tab, void* cub, long long rows,
+                        hipStream_t st);
+int wrtk_adjoint_f32(const void* dirs, const void* trig, int ndir, const void* cub, int ns,
+                     double s0, double ds, int n, float* out, hipStream_t st);
+
 /* number of lattice samples passing zero_interp's cube test, summed over all planes */
 int wrtk_radon_incube(int n, const double* dirs, int ndir, const double* shift, int nshift,
                       unsigned long long* total, hipStream_t st);

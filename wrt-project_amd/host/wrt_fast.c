@@ -6,6 +6,11 @@
  * task list of the brick kernel (interior bricks and bricks touching the cube's faces are
  * separate launches).  The kernel accumulates each plane in FP64; the result is rounded to
  * FP32 on the device, which is the output of the form.
+ *
+ * ADJOINT (FP32): the Radon rows are uploaded in FP64 and splined in FP64 as in the plain
+ * form; each interval's cubic is then rounded to FP32 (y, b, c, d), directions become float4
+ * (unit normal, weight), and one thread per voxel sums the directions with Kahan compensation.
+ * The interval of s is found arithmetically, so the shift grid must be uniform.
  */
 #include <stdlib.h>
 #include <math.h>
@@ -32,14 +37,140 @@ typedef struct {
   double* staging;  /* n^3 FP64 input */
 } radon_fast;
 
+/* ADJOINT in FP32: per-interval FP32 cubics of the FP64 spline, float4 direction table */
+typedef struct {
+  double* staging; /* FP64 Radon input [s][phi][theta] */
+  void* cub;       /* [ndir][nshift-1] float4 (y, b, c, d) */
+  void* dirs;      /* [ndir] float4 (n.x, n.y, n.z, (2pi/nphi) w_theta) */
+  double* trig;    /* [ndir][4] sin t, cos p, sin p, cos t: exact cut decisions near |s| = 1 */
+  float* out;      /* n^3 [z][x][y] */
+  double s0, ds;
+} adjoint_fast;
+
+/* the FP32 adjoint evaluates the spline interval arithmetically: the shift grid must be
+ * uniform (every grid the reference programs build is) */
+static int uniform_grid(const double* s, int n, double* s0, double* ds) {
+  if (n < 3) return 0;
+  const double d = (s[n - 1] - s[0]) / (n - 1);
+  for (int i = 0; i < n; ++i)
+    if (fabs(s[i] - (s[0] + i * d)) > 1e-12 * (1.0 + fabs(s[i]))) return 0;
+  *s0 = s[0];
+  *ds = d;
+  return d > 0;
+}
+
 int wrt_fast_supported(const wrt_job_desc* d) {
-  return d && d->op == WRT_OP_RADON && (d->prec == WRT_F32 || d->prec == WRT_BF16);
+  if (!d) return 0;
+  if (d->op == WRT_OP_RADON) return d->prec == WRT_F32 || d->prec == WRT_BF16;
+  if (d->op == WRT_OP_ADJOINT) {
+    double s0, ds;
+    return d->prec == WRT_F32 && d->shift && uniform_grid(d->shift, d->nshift, &s0, &ds);
+  }
+  return 0;
+}
+
+static size_t adjoint_fast_bytes(const wrt_job_desc* d) {
+  const size_t ndir = (size_t)d->nphi * d->ntheta;
+  return (size_t)d->nshift * ndir * 8 + ndir * (size_t)(d->nshift - 1) * 16 + ndir * 48 +
+         (size_t)d->n * d->n * d->n * 4;
+}
+
+static void adjoint_fast_destroy(wrt_job* j) {
+  adjoint_fast* f = (adjoint_fast*)j->fast;
+  hipFree(f->staging);
+  hipFree(f->cub);
+  hipFree(f->dirs);
+  hipFree(f->trig);
+  hipFree(f->out);
+  free(f);
+  j->fast = NULL;
+}
+
+static int adjoint_fast_create(wrt_job* j) {
+  const wrt_job_desc* d = &j->d;
+  adjoint_fast* f = (adjoint_fast*)calloc(1, sizeof(adjoint_fast));
+  if (!f) return WRT_E_NOMEM;
+  j->fast = f;
+  uniform_grid(d->shift, d->nshift, &f->s0, &f->ds);
+  const size_t ndir = (size_t)d->nphi * d->ntheta;
+  if (hipMalloc((void**)&f->staging, (size_t)d->nshift * ndir * 8) != hipSuccess ||
+      hipMalloc(&f->cub, ndir * (size_t)(d->nshift - 1) * 16) != hipSuccess ||
+      hipMalloc(&f->dirs, ndir * 16) != hipSuccess ||
+      hipMalloc((void**)&f->trig, ndir * 32) != hipSuccess ||
+      hipMalloc((void**)&f->out, (size_t)d->n * d->n * d->n * 4) != hipSuccess)
+    return WRT_E_NOMEM;
+  /* unit normals (sin t cos p, sin t sin p, cos t) and weights (2pi/nphi) w_t, host libm */
+  float* h = (float*)malloc(ndir * 16);
+  double* tr = (double*)malloc(ndir * 32);
+  if (!h || !tr) {
+    free(h);
+    free(tr);
+    return WRT_E_NOMEM;
+  }
+  const double dphi = 2 * M_PI / d->nphi;
+  for (int ip = 0; ip < d->nphi; ++ip)
+    for (int it = 0; it < d->ntheta; ++it) {
+      const size_t k = (size_t)ip * d->ntheta + it;
+      double* t = tr + 4 * k;
+      t[0] = sin(d->theta[it]);
+      t[1] = cos(d->phi[ip]);
+      t[2] = sin(d->phi[ip]);
+      t[3] = cos(d->theta[it]);
+      float* v = h + 4 * k;
+      v[0] = (float)(t[0] * t[1]);
+      v[1] = (float)(t[0] * t[2]);
+      v[2] = (float)t[3];
+      v[3] = (float)(dphi * d->theta_w[it]);
+    }
+  int rc = WRT_OK;
+  if (hipMemcpyAsync(f->dirs, h, ndir * 16, hipMemcpyHostToDevice, j->st) != hipSuccess ||
+      hipMemcpyAsync(f->trig, tr, ndir * 32, hipMemcpyHostToDevice, j->st) != hipSuccess ||
+      hipStreamSynchronize(j->st) != hipSuccess)
+    rc = WRT_E_HIP;
+  free(h);
+  free(tr);
+  return rc;
+}
+
+static int adjoint_fast_load(wrt_job* j, const double* in) {
+  adjoint_fast* f = (adjoint_fast*)j->fast;
+  const wrt_job_desc* d = &j->d;
+  const size_t ndir = (size_t)d->nphi * d->ntheta;
+  HIPCHK(hipMemcpyAsync(f->staging, in, (size_t)d->nshift * ndir * 8, hipMemcpyHostToDevice, j->st));
+  int rc = wrtk_spline_rows(j->d_shift, d->nshift, j->d_fac, f->staging, j->d_spl, 1, (int)ndir, 0,
+                            (long long)ndir, j->st);
+  if (rc == WRT_OK) rc = wrtk_adjoint_cubics(j->d_shift, d->nshift, j->d_spl, f->cub, (long long)ndir, j->st);
+  if (rc == WRT_OK) HIPCHK(hipStreamSynchronize(j->st));
+  return rc;
+}
+
+static int adjoint_fast_enqueue(wrt_job* j) {
+  adjoint_fast* f = (adjoint_fast*)j->fast;
+  const wrt_job_desc* d = &j->d;
+  HIPCHK(hipEventRecord(j->e1, j->st));
+  const int rc = wrtk_adjoint_f32(f->dirs, f->trig, d->nphi * d->ntheta, f->cub, d->nshift, f->s0, f->ds,
+                                  d->n, f->out, j->st);
+  if (rc != WRT_OK) return rc;
+  HIPCHK(hipEventRecord(j->e3, j->st));
+  return WRT_OK;
+}
+
+static int fetch_f32(wrt_job* j, const float* src, size_t count, double* out) {
+  float* h = (float*)malloc(count * sizeof(float));
+  if (!h) return WRT_E_NOMEM;
+  int rc = WRT_OK;
+  if (hipMemcpyAsync(h, src, count * sizeof(float), hipMemcpyDeviceToHost, j->st) != hipSuccess ||
+      hipStreamSynchronize(j->st) != hipSuccess)
+    rc = WRT_E_HIP;
+  if (rc == WRT_OK)
+    for (size_t i = 0; i < count; ++i) out[i] = (double)h[i];
+  free(h);
+  return rc;
 }
 
 static int nb_of(int n) { return (n + WRT_BRICK - 1) / WRT_BRICK; }
 
-size_t wrt_fast_bytes(const wrt_job_desc* d) {
-  if (!wrt_fast_supported(d)) return 0;
+static size_t radon_fast_bytes(const wrt_job_desc* d) {
   const size_t nb = (size_t)nb_of(d->n);
   const size_t nbr = nb * nb * nb;
   const size_t ndir = (size_t)d->nphi * d->ntheta;
@@ -49,7 +180,7 @@ size_t wrt_fast_bytes(const wrt_job_desc* d) {
          (size_t)d->nshift * ndir * 12 + (size_t)d->n * d->n * d->n * 8;
 }
 
-void wrt_fast_destroy(wrt_job* j) {
+static void radon_fast_destroy(wrt_job* j) {
   radon_fast* f = (radon_fast*)j->fast;
   if (!f) return;
   hipFree(f->bricks);
@@ -62,7 +193,7 @@ void wrt_fast_destroy(wrt_job* j) {
   j->fast = NULL;
 }
 
-int wrt_fast_create(wrt_job* j) {
+static int radon_fast_create(wrt_job* j) {
   const wrt_job_desc* d = &j->d;
   radon_fast* f = (radon_fast*)calloc(1, sizeof(radon_fast));
   if (!f) return WRT_E_NOMEM;
@@ -79,16 +210,16 @@ int wrt_fast_create(wrt_job* j) {
       hipMalloc((void**)&f->acc, nout * sizeof(double)) != hipSuccess ||
       hipMalloc((void**)&f->out, nout * sizeof(float)) != hipSuccess ||
       hipMalloc((void**)&f->staging, (size_t)d->n * d->n * d->n * sizeof(double)) != hipSuccess) {
-    wrt_fast_destroy(j);
+    radon_fast_destroy(j);
     return WRT_E_NOMEM;
   }
   return WRT_OK;
 }
 
-int wrt_fast_load(wrt_job* j, const double* in) {
+static int radon_fast_load(wrt_job* j, const double* in) {
   radon_fast* f = (radon_fast*)j->fast;
   const wrt_job_desc* d = &j->d;
-  hipStream_t st = j->ctx->stream;
+  hipStream_t st = j->st;
   const size_t n3 = (size_t)d->n * d->n * d->n;
   HIPCHK(hipMemcpyAsync(f->staging, in, n3 * sizeof(double), hipMemcpyHostToDevice, st));
   /* flags, then (8-byte aligned) the squared support radius as ordered bits */
@@ -148,17 +279,16 @@ int wrt_fast_load(wrt_job* j, const double* in) {
   return rc;
 }
 
-/* enqueue between ev0 and ev2 (ev1 = start of the brick kernels) */
-int wrt_fast_enqueue(wrt_job* j) {
+/* enqueue on the job's stream; e1 / e3 bracket the brick kernels */
+static int radon_fast_enqueue(wrt_job* j) {
   radon_fast* f = (radon_fast*)j->fast;
   const wrt_job_desc* d = &j->d;
-  wrt_ctx* c = j->ctx;
-  hipStream_t st = c->stream;
+  hipStream_t st = j->st;
   const size_t ndir = (size_t)d->nphi * d->ntheta;
   const size_t nout = (size_t)d->nshift * ndir;
   int rc = wrtk_fill_f64(f->acc, 0.0, nout, st);
   if (rc != WRT_OK) return rc;
-  HIPCHK(hipEventRecord(c->ev1, st));
+  HIPCHK(hipEventRecord(j->e1, st));
   wrt_radon_fast_args a;
   a.bricks = f->bricks;
   a.n = d->n;
@@ -179,43 +309,42 @@ int wrt_fast_enqueue(wrt_job* j) {
   a.ntasks = f->n_boundary;
   rc = wrtk_radon_fast(&a, 1, st);
   if (rc != WRT_OK) return rc;
-  HIPCHK(hipEventRecord(c->ev3, st));
+  HIPCHK(hipEventRecord(j->e3, st));
   return wrtk_acc_to_f32(f->acc, f->out, nout, st);
 }
 
-int wrt_fast_run(wrt_job* j, wrt_stats* s) {
-  wrt_ctx* c = j->ctx;
-  HIPCHK(hipEventRecord(c->ev0, c->stream));
-  int rc = wrt_fast_enqueue(j);
-  if (rc != WRT_OK) return rc;
-  HIPCHK(hipEventRecord(c->ev2, c->stream));
-  HIPCHK(hipEventSynchronize(c->ev2));
-  if (s) {
-    float ms_all = 0, ms_k = 0;
-    HIPCHK(hipEventElapsedTime(&ms_all, c->ev0, c->ev2));
-    HIPCHK(hipEventElapsedTime(&ms_k, c->ev1, c->ev3));
-    const wrt_job_desc* d = &j->d;
-    s->seconds = ms_all * 1e-3;
-    s->kernel_seconds = ms_k * 1e-3;
-    s->logical_samples = 16.0 * d->n * (double)d->n * (double)d->nshift * d->nphi * d->ntheta;
-    s->alg_bytes = 0;
-    s->outputs = (long long)d->nshift * d->nphi * d->ntheta;
-  }
-  return WRT_OK;
+/* dispatch by operation; each job owns one fast state */
+size_t wrt_fast_bytes(const wrt_job_desc* d) {
+  if (!wrt_fast_supported(d)) return 0;
+  return d->op == WRT_OP_RADON ? radon_fast_bytes(d) : adjoint_fast_bytes(d);
+}
+
+int wrt_fast_create(wrt_job* j) {
+  if (j->d.op == WRT_OP_RADON) return radon_fast_create(j);
+  const int rc = adjoint_fast_create(j);
+  if (rc != WRT_OK && j->fast) adjoint_fast_destroy(j);
+  return rc;
+}
+
+void wrt_fast_destroy(wrt_job* j) {
+  if (!j->fast) return;
+  if (j->d.op == WRT_OP_RADON)
+    radon_fast_destroy(j);
+  else
+    adjoint_fast_destroy(j);
+}
+
+int wrt_fast_load(wrt_job* j, const double* in) {
+  return j->d.op == WRT_OP_RADON ? radon_fast_load(j, in) : adjoint_fast_load(j, in);
+}
+
+int wrt_fast_enqueue(wrt_job* j) {
+  return j->d.op == WRT_OP_RADON ? radon_fast_enqueue(j) : adjoint_fast_enqueue(j);
 }
 
 int wrt_fast_fetch(wrt_job* j, double* out) {
-  radon_fast* f = (radon_fast*)j->fast;
   const wrt_job_desc* d = &j->d;
-  const size_t nout = (size_t)d->nshift * d->nphi * d->ntheta;
-  float* h = (float*)malloc(nout * sizeof(float));
-  if (!h) return WRT_E_NOMEM;
-  int rc = WRT_OK;
-  if (hipMemcpyAsync(h, f->out, nout * sizeof(float), hipMemcpyDeviceToHost, j->ctx->stream) != hipSuccess ||
-      hipStreamSynchronize(j->ctx->stream) != hipSuccess)
-    rc = WRT_E_HIP;
-  if (rc == WRT_OK)
-    for (size_t i = 0; i < nout; ++i) out[i] = (double)h[i];
-  free(h);
-  return rc;
+  if (d->op == WRT_OP_RADON)
+    return fetch_f32(j, ((radon_fast*)j->fast)->out, (size_t)d->nshift * d->nphi * d->ntheta, out);
+  return fetch_f32(j, ((adjoint_fast*)j->fast)->out, (size_t)d->n * d->n * d->n, out);
 }

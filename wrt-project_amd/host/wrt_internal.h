@@ -23,6 +23,11 @@ struct wrt_ctx {
 struct wrt_job {
   wrt_ctx* ctx;
   wrt_job_desc d;
+  /* own stream (default priority) and timing events: e0 start, e1 start of the dominant
+   * kernel, e3 its end, e2 end of the job's device work */
+  hipStream_t st;
+  hipEvent_t e0, e1, e2, e3;
+  int pending; /* launched, not yet waited for */
   /* owned host copies of the grids */
   double *phi, *theta, *theta_w, *shift, *z;
   size_t in_count, out_count;

@@ -53,7 +53,8 @@ ABI_SYMBOLS = [
     "wrt_ray_grid_T", "wrt_job_bytes", "wrt_job_input_count", "wrt_job_output_count",
     "wrt_job_create", "wrt_job_load", "wrt_job_load_synthetic", "wrt_job_run",
     "wrt_job_fetch", "wrt_job_destroy", "wrt_batch_run", "wrt_phantom",
-    "wrt_phantom_params", "wrt_gaussian", "wrt_radon_count_incube",
+    "wrt_phantom_params", "wrt_gaussian", "wrt_radon_count_incube", "wrt_job_launch",
+    "wrt_job_poll", "wrt_job_wait",
 ]
 
 
@@ -92,6 +93,9 @@ def lib():
         L.wrt_job_load.argtypes = [P, P]
         L.wrt_job_load_synthetic.argtypes = [P, C.c_ulonglong, I, D]
         L.wrt_job_run.argtypes = [P, P]
+        L.wrt_job_launch.argtypes = [P]
+        L.wrt_job_poll.argtypes = [P, P]
+        L.wrt_job_wait.argtypes = [P, P]
         L.wrt_job_fetch.argtypes = [P, P]
         L.wrt_job_destroy.argtypes = [P]
         L.wrt_job_destroy.restype = None
@@ -300,6 +304,22 @@ class Job:
     def run(self):
         st = Stats()
         _chk(lib().wrt_job_run(self.h, C.byref(st)), "wrt_job_run")
+        return st.as_dict()
+
+    def launch(self):
+        _chk(lib().wrt_job_launch(self.h), "wrt_job_launch")
+
+    def poll(self):
+        st = Stats()
+        rc = lib().wrt_job_poll(self.h, C.byref(st))
+        if rc == 1:
+            return None
+        _chk(rc, "wrt_job_poll")
+        return st.as_dict()
+
+    def wait(self):
+        st = Stats()
+        _chk(lib().wrt_job_wait(self.h, C.byref(st)), "wrt_job_wait")
         return st.as_dict()
 
     def fetch(self):
