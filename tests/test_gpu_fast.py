@@ -125,3 +125,21 @@ def test_fast_adjoint_f32_cut_decisions(ctx, n, nphi, nth):
     got, _ = ctx.radon_adjoint(radon, phi, th, w, s, n, prec="f32")
     assert orc.rel_l2(got, ref) <= 1e-5, orc.rel_l2(got, ref)
     assert np.max(np.abs(got - ref)) <= 1e-4 * np.max(np.abs(ref)), np.max(np.abs(got - ref))
+
+
+@pytest.mark.parametrize("n", [32, 40, 64])
+def test_fast_f32_structured_directions_on_faces(ctx, n):
+    # Non-zero values up to the cube's faces, axis-aligned and diagonal directions, and
+    # shifts on cell faces and on the cube's faces: the index and cube decisions that hinge
+    # on the last bit must follow the plain form's (a flipped decision is an O(1) error).
+    rng = np.random.default_rng(n)
+    vol = rng.standard_normal((n, n, n))
+    phi = np.array([0.0, np.pi / 4, np.pi / 2, np.pi, 1.5 * np.pi, 0.3])
+    _, th_gl, _, _ = ctx.radon_grid_init(4, 6, 3)
+    th = np.concatenate([th_gl, [np.pi / 2, np.pi / 4, 0.0]])
+    s = np.unique(np.concatenate([-1.0 + np.arange(0, n + 1, 3) * (2.0 / n), [-1.0, 1.0, 0.0]]))
+    ref, _ = ctx.radon_grid(vol, phi, th, s, prec="f64")
+    got, _ = ctx.radon_grid(vol, phi, th, s, prec="f32")
+    scale = np.max(np.abs(ref))
+    assert orc.rel_l2(got, ref) <= 1e-5, orc.rel_l2(got, ref)
+    assert np.max(np.abs(got - ref)) <= 1e-5 * scale, np.max(np.abs(got - ref)) / scale

@@ -53,6 +53,25 @@ __device__ __forceinline__ double dpp_move(double v) {
   const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWMASK, 0xf, true);
   return __hiloint2double(hi, lo);
 }
+// min / max over the 16 lanes of each DPP row (row_ror 1, 2, 4, 8): every lane of a row ends
+// with its row's extremum
+template <int CTRL>
+__device__ __forceinline__ int dpp_ror(int v) {
+  return __builtin_amdgcn_update_dpp(v, v, CTRL, 0xf, 0xf, false);
+}
+__device__ __forceinline__ int row_min16(int v) {
+  v = min(v, dpp_ror<0x121>(v));
+  v = min(v, dpp_ror<0x122>(v));
+  v = min(v, dpp_ror<0x124>(v));
+  return min(v, dpp_ror<0x128>(v));
+}
+__device__ __forceinline__ int row_max16(int v) {
+  v = max(v, dpp_ror<0x121>(v));
+  v = max(v, dpp_ror<0x122>(v));
+  v = max(v, dpp_ror<0x124>(v));
+  return max(v, dpp_ror<0x128>(v));
+}
+
 __device__ __forceinline__ double wave_sum_dpp(double v) {
   v += dpp_move<0x111, 0xf>(v);  // row_shr:1
   v += dpp_move<0x112, 0xf>(v);  // row_shr:2
@@ -69,6 +88,55 @@ __device__ __forceinline__ float bf16_to_f32(unsigned short v) {
   return __uint_as_float(((unsigned)v) << 16);
 }
 
+// Exact cube test of one axis on a row: x(c) = w + b*c in binary64 (the reference's
+// rounding) is monotone in c, so {c : x >= -1} and {c : x <= 1} are a prefix or a suffix of
+// the row.  Narrows [rl, rh] to the columns passing the tests that lo / hi ask for.  The
+// continuous bounds already lie within ~1e-6 columns of the answer, so the first probe
+// usually settles it; a bisection covers rows almost parallel to the face.
+__device__ __forceinline__ void cube_ends(double w, double b, bool lo, bool hi, int& rl,
+                                          int& rh) {
+  if (rl > rh) return;
+  for (int side = 0; side < 2; ++side) {
+    if (side == 0 ? !lo : !hi) continue;
+    // P(c): the test holds at column c; suffix-type when x moves into the cube as c grows
+    const bool suffix = side == 0 ? b > 0.0 : b < 0.0;
+    auto P = [&](int c) {
+      const double x = w + b * (double)c;
+      return side == 0 ? x >= -1.0 : x <= 1.0;
+    };
+    if (b == 0.0) {
+      if (!P(rl)) rh = rl - 1;
+    } else if (suffix) {
+      if (!P(rl)) {
+        if (!P(rh)) {
+          rh = rl - 1;
+        } else {
+          int l = rl, h = rh;  // P(l) false, P(h) true
+          while (h - l > 1) {
+            const int m = (l + h) >> 1;
+            if (P(m)) h = m; else l = m;
+          }
+          rl = h;
+        }
+      }
+    } else {
+      if (!P(rh)) {
+        if (!P(rl)) {
+          rh = rl - 1;
+        } else {
+          int l = rl, h = rh;  // P(l) true, P(h) false
+          while (h - l > 1) {
+            const int m = (l + h) >> 1;
+            if (P(m)) l = m; else h = m;
+          }
+          rh = l;
+        }
+      }
+    }
+    if (rl > rh) return;
+  }
+}
+
 template <bool BND, bool BF16, bool POW2>
 __global__ __launch_bounds__(FR_WAVES * 64) void k_radon_fast(wrt_radon_fast_args a, double h,
                                                               double dc, double inv_dc) {
@@ -76,6 +144,7 @@ __global__ __launch_bounds__(FR_WAVES * 64) void k_radon_fast(wrt_radon_fast_arg
   __shared__ float sb[FB * FR_PX + 1];
   // per wave: first column of every row of the current section
   __shared__ int rowlo[FR_WAVES][136];
+  __shared__ int next_dir;
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int brick = a.tasks[2 * blockIdx.x], grp = a.tasks[2 * blockIdx.x + 1];
@@ -93,7 +162,10 @@ __global__ __launch_bounds__(FR_WAVES * 64) void k_radon_fast(wrt_radon_fast_arg
       else v = ((const float*)a.bricks)[base + i];
       sb[lx * FR_PX + ly * FR_PY + lz] = v;
     }
-    if (threadIdx.x == 0) sb[FB * FR_PX] = 0.f;
+    if (threadIdx.x == 0) {
+      sb[FB * FR_PX] = 0.f;
+      next_dir = a.tasks[2 * blockIdx.x + 1] * a.dgroup;
+    }
   }
   __syncthreads();
   const unsigned zero_off = 4u * (FB * FR_PX);
@@ -103,15 +175,33 @@ __global__ __launch_bounds__(FR_WAVES * 64) void k_radon_fast(wrt_radon_fast_arg
   const double hi[3] = {-1.0 + (b0x + FB) * dc, -1.0 + (b0y + FB) * dc, -1.0 + (b0z + FB) * dc};
   const double fb0x = (double)b0x, fb0y = (double)b0y, fb0z = (double)b0z;
   const int L0 = -2 * n, L1 = 2 * n - 1;  // lattice index range of i1 and i2 (bounds of R0..R1, C0..C1)
-  // axes along which the brick touches a face of the cube (only those need the cube test;
-  // along the others x < -1 truncates outside the brick and x > 1 lands past it)
-  const bool fx = b0x == 0 || b0x + FB >= n, fy = b0y == 0 || b0y + FB >= n,
-             fz = b0z == 0 || b0z + FB >= n;
+  // Faces of the cube the brick touches.  Only along those axes can a sample whose cell is
+  // in the brick fail the reference's cube test |x| <= 1 (elsewhere x < -1 truncates to cell
+  // 0, outside the brick, and x > 1 lands past it).  Face bricks apply the cube test exactly
+  // at the ends of every row (x is monotone along a row) and mask the lanes past a row's
+  // end; x == 1 exactly gives index n, which the reference maps to n - 1.
+  const bool flo[3] = {b0x == 0, b0y == 0, b0z == 0};
+  const bool fhi[3] = {b0x + FB >= n, b0y + FB >= n, b0z + FB >= n};
+  const int cxh = fhi[0] ? n - 1 : INT_MAX, cyh = fhi[1] ? n - 1 : INT_MAX,
+            czh = fhi[2] ? n - 1 : INT_MAX;
   const int r_lane = lane >> 3, k_lane = lane & 7;
+  // the brick's farthest corner lies within the support sphere
+  bool inside;
+  {
+    double f2c = 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) f2c += fmax(lo[c] * lo[c], hi[c] * hi[c]);
+    inside = f2c * (1.0 + 1e-12) < a.rho2;
+  }
 
-  const int d_begin = grp * a.dgroup;
-  const int d_end = min(d_begin + a.dgroup, a.ndir);
-  for (int d = d_begin + wv; d < d_end; d += FR_WAVES) {
+  const int d_end = min((grp + 1) * a.dgroup, a.ndir);
+  // waves take the group's directions one at a time from a shared counter, so that they
+  // finish together however unequal the directions' sections are
+  for (;;) {
+    int d = 0;
+    if (lane == 0) d = atomicAdd(&next_dir, 1);
+    d = __builtin_amdgcn_readfirstlane(d);
+    if (d >= d_end) break;
     const double* D = a.dirs + 9 * (size_t)d;
     const double nrm[3] = {D[0], D[1], D[2]};
     const double av[3] = {D[3], D[4], D[5]};
@@ -144,29 +234,75 @@ __global__ __launch_bounds__(FR_WAVES * 64) void k_radon_fast(wrt_radon_fast_arg
       j1 = l - 1;
     }
     if (j0 > j1) continue;
-    const int nrows = R1 - R0 + 1;  // <= 122 by geometry (brick diameter + margins)
     // reciprocals of the column steps (for the conservative row intervals)
     double ib[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) ib[c] = (fabs(bv[c]) * (4.0 * n) < 1e-9 * dc) ? 0.0 : 1.0 / bv[c];
-    const int nbands = (nrows + 7) >> 3;
+    const double ibb = 1.0 / bb, iaa = 1.0 / aa;
+    // this lane's box edge (lanes 0..11): axis ek, the other two coordinates at lo/hi corners
+    const int ek = lane >> 2, ek1 = ek == 2 ? 0 : ek + 1, ek2 = ek == 0 ? 2 : ek - 1;
+    const double e_n = ek == 0 ? nrm[0] : (ek == 1 ? nrm[1] : nrm[2]);
+    const double e_n1 = ek1 == 0 ? nrm[0] : (ek1 == 1 ? nrm[1] : nrm[2]);
+    const double e_n2 = ek2 == 0 ? nrm[0] : (ek2 == 1 ? nrm[1] : nrm[2]);
+    const double e_a = ek == 0 ? av[0] : (ek == 1 ? av[1] : av[2]);
+    const double e_a1 = ek1 == 0 ? av[0] : (ek1 == 1 ? av[1] : av[2]);
+    const double e_a2 = ek2 == 0 ? av[0] : (ek2 == 1 ? av[1] : av[2]);
+    const double e_lo = ek == 0 ? lo[0] : (ek == 1 ? lo[1] : lo[2]);
+    const double e_hi = ek == 0 ? hi[0] : (ek == 1 ? hi[1] : hi[2]);
+    const double e_p1 = (lane & 1) ? (ek1 == 0 ? hi[0] : (ek1 == 1 ? hi[1] : hi[2]))
+                                   : (ek1 == 0 ? lo[0] : (ek1 == 1 ? lo[1] : lo[2]));
+    const double e_p2 = (lane & 2) ? (ek2 == 0 ? hi[0] : (ek2 == 1 ? hi[1] : hi[2]))
+                                   : (ek2 == 0 ? lo[0] : (ek2 == 1 ? lo[1] : lo[2]));
+    const bool e_ok = lane < 12 && e_n != 0.0;
+    const double e_in = e_ok ? 1.0 / e_n : 0.0;
+    const double e_rest = e_n1 * e_p1 + e_n2 * e_p2, e_arest = e_a1 * e_p1 + e_a2 * e_p2;
+    const double ia = 1.0 / sqrt(aa);
     for (int j = j0; j <= j1; ++j) {
       const double s = a.shift[j];
+      // the plane meets the support sphere in a circle of radius sqrt(rho2 - s^2) about the
+      // foot point s*n, where row 0 passes: rows farther than that (+1) are empty
+      const double rc2 = a.rho2 - s * s;
+      if (rc2 < 0.0) continue;
+      const double rr = sqrt(rc2) * ia + 1.0;
       const double S[3] = {s * nrm[0], s * nrm[1], s * nrm[2]};
-      // conservative column interval of every row (rows R0 + lane and R0 + 64 + lane): first
+      // rows of this section: the section polygon's vertices lie on the box edges; the row
+      // of a point p of the plane is a.p / |a|^2 (a is orthogonal to n and b), +-2 rows margin
+      int r_lo = INT_MAX, r_hi = INT_MIN;
+      if (e_ok) {
+        const double tol = 1e-9 * dc;
+        double pk = (s - e_rest) * e_in;
+        if (pk >= e_lo - tol && pk <= e_hi + tol) {
+          pk = fmin(fmax(pk, e_lo), e_hi);
+          const double r = (e_a * pk + e_arest) * iaa;
+          r_lo = (int)fmax(floor(r) - 2.0, (double)L0);
+          r_hi = (int)fmin(ceil(r) + 2.0, (double)L1);
+        }
+      }
+      r_lo = __builtin_amdgcn_readlane(row_min16(r_lo), 0);
+      r_hi = __builtin_amdgcn_readlane(row_max16(r_hi), 0);
+      const int SR0 = max(max(R0, r_lo), (int)floor(-rr)), SR1 = min(min(R1, r_hi), (int)ceil(rr));
+      if (SR0 > SR1) continue;
+      const int nrows = SR1 - SR0 + 1;  // <= 122 by geometry (brick diameter + margins)
+      const int nbands = (nrows + 7) >> 3;
+      // conservative column interval of every row (rows SR0 + lane and SR0 + 64 + lane): first
       // column into the wave's row table, 8-column step count reduced to a per-band maximum
       int bsteps[2];
 #pragma unroll
       for (int half = 0; half < 2; ++half) {
+        if (64 * half >= nrows) {  // uniform: the section has at most 64 rows
+          bsteps[half] = 0;
+          continue;
+        }
         const int t = lane + 64 * half;
-        int rl = C0, nst = 0;
+        int rl = C0, rh = C0 - 1, nst = 0;
         if (t < nrows) {
-          const double f1 = (double)(R0 + t);
+          const double f1 = (double)(SR0 + t);
           double tlo = -1e300, thi = 1e300;
           bool empty = false;
           // the row's points p(t) = w + b t within the support sphere |p|^2 <= rho2 (every
-          // sample outside it reads a zero cell, so clipping the row there changes no sum)
-          {
+          // sample outside it reads a zero cell, so clipping the row there changes no sum);
+          // nothing to clip when the whole brick lies inside the sphere
+          if (!inside) {
             const double wx = S[0] + av[0] * f1, wy = S[1] + av[1] * f1, wz = S[2] + av[2] * f1;
             const double qb = wx * bv[0] + wy * bv[1] + wz * bv[2];
             const double qc = wx * wx + wy * wy + wz * wz - a.rho2;
@@ -175,21 +311,22 @@ __global__ __launch_bounds__(FR_WAVES * 64) void k_radon_fast(wrt_radon_fast_arg
               empty = true;
             } else {
               const double sq = sqrt(disc);
-              tlo = (-qb - sq) / bb;
-              thi = (-qb + sq) / bb;
+              tlo = (-qb - sq) * ibb;
+              thi = (-qb + sq) * ibb;
             }
           }
 #pragma unroll
           for (int c = 0; c < 3; ++c) {
             const double w = S[c] + av[c] * f1;
             const double tol = 1e-9 * dc;
+            const double hc = fmin(hi[c], 1.0);  // the cube's face (n not a multiple of 32)
             if (ib[c] == 0.0) {
-              if (w < lo[c] - tol || w > hi[c] + tol) empty = true;
+              if (w < lo[c] - tol || w > hc + tol) empty = true;
             } else if (bv[c] > 0) {
               tlo = fmax(tlo, (lo[c] - tol - w) * ib[c]);
-              thi = fmin(thi, (hi[c] + tol - w) * ib[c]);
+              thi = fmin(thi, (hc + tol - w) * ib[c]);
             } else {
-              tlo = fmax(tlo, (hi[c] + tol - w) * ib[c]);
+              tlo = fmax(tlo, (hc + tol - w) * ib[c]);
               thi = fmin(thi, (lo[c] - tol - w) * ib[c]);
             }
           }
@@ -199,17 +336,25 @@ __global__ __launch_bounds__(FR_WAVES * 64) void k_radon_fast(wrt_radon_fast_arg
             tlo = fmax(tlo, (double)C0 - 1.0);
             thi = fmin(thi, (double)C1 + 1.0);
             rl = max(C0, (int)ceil(tlo - 1e-6));
-            const int rh = min(C1, (int)floor(thi + 1e-6));
+            rh = min(C1, (int)floor(thi + 1e-6));
+            if (BND) {
+#pragma unroll
+              for (int c = 0; c < 3; ++c)
+                if (flo[c] || fhi[c]) cube_ends(S[c] + av[c] * f1, bv[c], flo[c], fhi[c], rl, rh);
+            }
             nst = rh >= rl ? ((rh - rl + 8) >> 3) : 0;
           }
         }
-        rowlo[wv][t] = rl;
+        // first and last column, 16 bits each (|i2| <= 2n < 2^15)
+        rowlo[wv][t] = (int)(((unsigned)rl & 0xffffu) | ((unsigned)rh << 16));
         // max over the 8 consecutive rows of a band (lanes 8b .. 8b+7)
         nst = max(nst, __shfl_xor(nst, 1, 64));
         nst = max(nst, __shfl_xor(nst, 2, 64));
         nst = max(nst, __shfl_xor(nst, 4, 64));
         bsteps[half] = nst;
       }
+      if (__ballot((bsteps[0] | bsteps[1]) != 0) == 0)
+        continue;  // no row of the section meets the brick and the sphere
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -217,29 +362,23 @@ __global__ __launch_bounds__(FR_WAVES * 64) void k_radon_fast(wrt_radon_fast_arg
       int rlo_next = rowlo[wv][r_lane];
       for (int k = 0; k < nbands; ++k) {
         const int t = 8 * k + r_lane;  // this lane's row within the section
-        const int rlo = rlo_next;
+        const int rlo = (rlo_next << 16) >> 16;
+        const double frh = (double)(rlo_next >> 16);  // the row's last column
         if (k + 1 < nbands) rlo_next = rowlo[wv][t + 8];
         const int steps = (k < 8) ? __builtin_amdgcn_readlane(bsteps[0], 8 * k)
                                   : __builtin_amdgcn_readlane(bsteps[1], 8 * (k - 8));
         if (steps == 0) continue;
-        const int i1 = R0 + t;
+        const int i1 = SR0 + t;
         // Rows past R1 and columns past the lattice end need no test: a plane's cube section
         // lies within sqrt(3) < 2 of the plane centre, inside the [-2n, 2n) lattice, so only
         // the brick test below can reject a sample.
         const double f1 = (double)i1;
         const double Rx = S[0] + av[0] * f1, Ry = S[1] + av[1] * f1, Rz = S[2] + av[2] * f1;
         float bs = 0.f;
-        int c = rlo + k_lane;
-        for (int m = 0; m < steps; ++m, c += 8) {
+        double f2 = (double)(rlo + k_lane);  // column i2, stepped exactly in binary64
+        for (int m = 0; m < steps; ++m, f2 += 8.0) {
           // per-column products b*i2, rounded as the reference rounds delta*v2*i2
-          const double f2 = (double)c;
           const double x = Rx + bv[0] * f2, y = Ry + bv[1] * f2, z = Rz + bv[2] * f2;
-          bool ok = true;
-          if (BND) {  // cube test on the axes along which the brick touches a face
-            if (fx) ok = ok && (fabs(x) <= 1.0);
-            if (fy) ok = ok && (fabs(y) <= 1.0);
-            if (fz) ok = ok && (fabs(z) <= 1.0);
-          }
           double qx, qy, qz;
           if (POW2) {
             // RN(x + 1) * h == RN(x*h + h) for h = n/2 a power of two: scaling by 2^k
@@ -252,16 +391,18 @@ __global__ __launch_bounds__(FR_WAVES * 64) void k_radon_fast(wrt_radon_fast_arg
           }
           // truncation as in zero_interp; for an interior brick a negative quotient (x < -1)
           // truncates to 0 <= b0 - 32 and a quotient above n lands past the brick, so the
-          // brick test below also is the cube test; bricks on the faces test |x| <= 1 above
+          // brick test below also is the cube test.  Face bricks: the row ends hold the cube
+          // test, and x == 1 (index n) maps to n - 1.
           int ix = (int)qx, iy = (int)qy, iz = (int)qz;
           if (BND) {
-            if (fx && ix == n) ix = n - 1;
-            if (fy && iy == n) iy = n - 1;
-            if (fz && iz == n) iz = n - 1;
+            ix = min(ix, cxh);
+            iy = min(iy, cyh);
+            iz = min(iz, czh);
           }
           const int lx = ix - b0x, ly = iy - b0y, lz = iz - b0z;
           // all three offsets in [0, 32): a negative offset sets high bits of the OR
-          ok = ok && (((unsigned)lx | (unsigned)ly | (unsigned)lz) < (unsigned)FB);
+          bool ok = ((unsigned)lx | (unsigned)ly | (unsigned)lz) < (unsigned)FB;
+          if (BND) ok = ok && f2 <= frh;
           const unsigned off = mad_u24((unsigned)lx, 4u * FR_PX,
                                        mad_u24((unsigned)ly, 4u * FR_PY, (unsigned)lz << 2));
           bs += *(const float*)((const char*)sb + (ok ? off : zero_off));
@@ -323,8 +464,10 @@ __global__ __launch_bounds__(256) void k_brickify(const double* __restrict__ vol
     const float f = (float)v;
     if (bf16) ((unsigned short*)bricks)[i] = f32_to_bf16_rne(f);
     else ((float*)bricks)[i] = f;
+    // one count per wavefront: the 64 lanes of a wave always share a brick
+    const unsigned long long nz = __ballot(v != 0.0);
+    if ((threadIdx.x & 63) == 0 && nz) atomicAdd(&flags[brick], (int)__popcll(nz));
     if (v != 0.0) {
-      flags[brick] = 1;
       const double cx = -1.0 + (2.0 * x + 1.0) / n, cy = -1.0 + (2.0 * y + 1.0) / n,
                    cz = -1.0 + (2.0 * z + 1.0) / n;
       r2 = fmax(r2, cx * cx + cy * cy + cz * cz);

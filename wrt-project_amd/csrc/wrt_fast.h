@@ -35,7 +35,7 @@ typedef struct {
                          farthest non-zero cell plus half a cell diagonal)     */
 } wrt_radon_fast_args;
 
-/* FP64 vol[x][y][z] -> bricks; flags[brick] = 1 if any value is non-zero;
+/* FP64 vol[x][y][z] -> bricks; flags[brick] += its number of non-zero values (flags zeroed by the caller);
  * r2max = largest squared cell-centre radius of a non-zero cell (as the bit pattern of a
  * non-negative double, so unsigned max orders it), zeroed by the caller */
 int wrtk_brickify(const double* vol, int n, int nb, int bf16, void* bricks, int* flags,

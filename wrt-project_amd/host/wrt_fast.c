@@ -28,7 +28,7 @@ typedef struct {
   int nb;
   size_t nbricks;
   void* bricks;
-  int* flags;       /* [nbricks] non-zero flags, then the support radius (8 bytes) */
+  int* flags;       /* [nbricks] non-zero voxel counts, then the support radius (8 bytes) */
   double rho2;      /* squared support radius of the loaded volume */
   int* tasks;       /* device, interior then boundary */
   int n_interior, n_boundary, dgroup;
@@ -61,7 +61,8 @@ static int uniform_grid(const double* s, int n, double* s0, double* ds) {
 
 int wrt_fast_supported(const wrt_job_desc* d) {
   if (!d) return 0;
-  if (d->op == WRT_OP_RADON) return d->prec == WRT_F32 || d->prec == WRT_BF16;
+  /* the brick kernel packs a row's first and last lattice column (|i2| <= 2n) in 16 bits */
+  if (d->op == WRT_OP_RADON) return (d->prec == WRT_F32 || d->prec == WRT_BF16) && d->n <= 4096;
   if (d->op == WRT_OP_ADJOINT) {
     double s0, ds;
     return d->prec == WRT_F32 && d->shift && uniform_grid(d->shift, d->nshift, &s0, &ds);
@@ -254,21 +255,36 @@ static int radon_fast_load(wrt_job* j, const double* in) {
   f->dgroup = dg;
   int* t = (int*)malloc(2 * (f->nbricks * ngroups + 1) * sizeof(int));
   if (!t) { free(flags); return WRT_E_NOMEM; }
+  /* bricks by falling non-zero count (the work of a task grows with it), so that the
+   * longest tasks start first and the launch ends on short ones */
+  int* order = (int*)malloc(f->nbricks * sizeof(int));
+  if (!order) { free(flags); free(t); return WRT_E_NOMEM; }
+  int nne = 0;
+  for (size_t b = 0; b < f->nbricks; ++b)
+    if (flags[b]) order[nne++] = (int)b;
+  for (int i = 1; i < nne; ++i) { /* insertion sort: at most a few thousand bricks */
+    const int v = order[i];
+    int q = i - 1;
+    while (q >= 0 && flags[order[q]] < flags[v]) { order[q + 1] = order[q]; --q; }
+    order[q + 1] = v;
+  }
   int k = 0;
   for (int pass = 0; pass < 2; ++pass) {
-    for (int g = 0; g < ngroups; ++g)
-      for (size_t b = 0; b < f->nbricks; ++b) {
-        if (!flags[b]) continue;
-        const int bx = (int)(b / ((size_t)f->nb * f->nb)), by = (int)((b / f->nb) % f->nb), bz = (int)(b % f->nb);
-        const int bnd = bx == 0 || by == 0 || bz == 0 || (bx + 1) * WRT_BRICK >= d->n ||
-                        (by + 1) * WRT_BRICK >= d->n || (bz + 1) * WRT_BRICK >= d->n;
-        if (bnd != pass) continue;
-        t[2 * k] = (int)b;
+    for (int i = 0; i < nne; ++i) {
+      const int b = order[i];
+      const int bx = b / (f->nb * f->nb), by = (b / f->nb) % f->nb, bz = b % f->nb;
+      const int bnd = bx == 0 || by == 0 || bz == 0 || (bx + 1) * WRT_BRICK >= d->n ||
+                      (by + 1) * WRT_BRICK >= d->n || (bz + 1) * WRT_BRICK >= d->n;
+      if (bnd != pass) continue;
+      for (int g = 0; g < ngroups; ++g) {
+        t[2 * k] = b;
         t[2 * k + 1] = g;
         ++k;
       }
+    }
     if (pass == 0) f->n_interior = k;
   }
+  free(order);
   f->n_boundary = k - f->n_interior;
   free(flags);
   rc = WRT_OK;
