@@ -9,6 +9,12 @@ HBM.  Printed: one JSON line with plane samples/s, the dominant kernel's rooflin
 max error against the plain FP64 form, the plain FP64 form's rate on a subset of the same
 grid, and a CPU baseline (the oracle on the host cores, a bounded sample of planes).
 
+The "serve" object is the served form (C request loop): p50/p99 latency of a seeded
+Poisson stream (8 requests/s, 64 requests, mix 50% plane integrals FP32 / 30% backprojection
+FP32 / 20% reduction FP64, edges {64,96,128,192,256}), and the throughput of the first 32
+requests of that stream arriving at once, served (overlapping, resident data) and in the
+plain per-request FP64 form (one at a time).
+
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
 import argparse
@@ -70,6 +76,10 @@ def main():
     ap.add_argument("--plain-phi-stride", type=int, default=16,
                     help="plain FP64 reference on every k-th phi (rate and relL2)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-qps", type=float, default=8.0)
+    ap.add_argument("--serve-requests", type=int, default=64)
+    ap.add_argument("--serve-burst", type=int, default=32)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -126,6 +136,25 @@ def main():
     if not args.no_cpu:
         log("bench: cpu baseline")
         cpu = cpu_baseline(vol, phi, th, s)
+    serve = None
+    if not args.no_serve:
+        log("bench: serve stream")
+        cfg = wrt.serve_cfg(qps=args.serve_qps, nreq=args.serve_requests, seed=1)
+        rep, _, _ = ctx.serve(cfg)
+        bcfg = wrt.serve_cfg(qps=0.0, nreq=args.serve_burst, seed=1)
+        brep, _, _ = ctx.serve(bcfg)
+        log("bench: plain per-request FP64 stream")
+        pcfg = wrt.serve_cfg(qps=0.0, nreq=args.serve_burst, seed=1, plain=True)
+        prep, _, _ = ctx.serve(pcfg)
+        serve = {"qps": args.serve_qps, "requests": rep["completed"], "p50_ms": rep["p50_ms"],
+                 "p99_ms": rep["p99_ms"], "mean_ms": rep["mean_ms"],
+                 "mix": "50% radon f32, 30% adjoint f32, 20% reduction f64; edges 64-256",
+                 "burst_requests": brep["completed"],
+                 "served_samples_per_s": brep["samples_per_s"],
+                 "plain_f64_samples_per_s": prep["samples_per_s"],
+                 "speedup_vs_plain": brep["samples_per_s"] / prep["samples_per_s"]}
+        log(f"bench: serve p50 {rep['p50_ms']:.1f} ms p99 {rep['p99_ms']:.1f} ms, burst "
+            f"{serve['speedup_vs_plain']:.2f}x plain")
     ctx.close()
 
     out = {
@@ -156,6 +185,7 @@ def main():
                      "kernel": "k_radon_fast (interior + face-brick launches)",
                      "kernel_seconds": kavg},
         "cpu_baseline": cpu,
+        "serve": serve,
     }
     print(json.dumps(out), flush=True)
 

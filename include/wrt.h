@@ -189,6 +189,66 @@ void wrt_job_destroy(wrt_job* job);
  * output equals its wrt_job_run output within the documented tolerance. */
 int wrt_batch_run(wrt_ctx* ctx, wrt_job* const* jobs, int njobs, wrt_stats* st);
 
+/* ------------------------------------------------------------------- serve loop --- */
+/* A request stream served from one context (no reference counterpart: the reference runs
+ * one program per input file).  Requests arrive as a seeded Poisson process; each names an
+ * operation, a cube edge e and one of `ndatasets` seeded datasets of that (op, e), on the
+ * grids (e/2) phi x (e/4) GL theta x (e+1) shifts (REDUCTION: rays on e z-slices with the
+ * same phi and shift grids).  RADON and ADJOINT run in FP32, REDUCTION in FP64.
+ * Admission: a request's dataset becomes resident (job created and loaded) when its device
+ * bytes fit the HBM budget, evicting idle resident datasets least recently used first; a
+ * resident dataset serves later requests without reloading.  Up to max_inflight requests
+ * run at once, each on its dataset's own stream, so requests that arrive while others are
+ * on the GPU start as soon as a slot and memory are free and overlap with them.
+ * Latency = completion - arrival, on the host's monotonic clock. */
+typedef struct wrt_serve_cfg {
+  double qps;          /* mean arrival rate (1/s); <= 0: every request arrives at t = 0 */
+  int nreq;            /* number of requests */
+  unsigned long long seed;
+  double mix[3];       /* relative weights of RADON, ADJOINT, REDUCTION */
+  const int* edges;    /* cube edges drawn uniformly (each even, >= 8) */
+  int nedges;
+  int ndatasets;       /* distinct datasets per (op, edge), >= 1 */
+  size_t hbm_budget;   /* device bytes for resident datasets; 0: 90% of the free HBM */
+  int max_inflight;    /* concurrent requests on the device, >= 1 */
+  int plain;           /* 1: every request runs the plain FP64 form, one at a time (the
+                          per-request baseline the served form is compared with) */
+} wrt_serve_cfg;
+
+typedef struct wrt_serve_req {
+  wrt_op op;
+  wrt_prec prec;
+  int edge;
+  int dataset;
+  unsigned long long data_seed; /* seed of the dataset's synthetic input */
+  double arrival_s;             /* arrival time after the start of the stream */
+} wrt_serve_req;
+
+typedef struct wrt_serve_report {
+  int completed;
+  double wall_s;                 /* start of the stream to the last completion */
+  double p50_ms, p99_ms, mean_ms, max_ms;
+  double samples, samples_per_s; /* logical samples of all requests, per wall second */
+  int loads, evictions, peak_inflight;
+  size_t peak_bytes;             /* largest device bytes held by resident datasets */
+} wrt_serve_report;
+
+/* request i of the stream cfg describes (deterministic in cfg) */
+int wrt_serve_describe(const wrt_serve_cfg* cfg, int i, wrt_serve_req* req);
+/* the grids of an edge: phi, theta, theta_w hold e/2, e/4, e/4 values, shift e+1, z e */
+int wrt_serve_grids(wrt_ctx* ctx, int edge, double* phi, double* theta, double* theta_w,
+                    double* shift, double* z);
+/* a request's job description on grids from wrt_serve_grids (pointers kept, not copied) */
+int wrt_serve_desc(const wrt_serve_req* req, const double* phi, const double* theta,
+                   const double* theta_w, const double* shift, const double* z,
+                   wrt_job_desc* d);
+/* the input of a request's dataset: wrt_job_input_count(desc) doubles */
+int wrt_serve_input(const wrt_serve_req* req, double* in);
+/* Serve the stream.  latency_ms (nreq entries) and outputs (nreq pointers, NULL entries
+ * skipped; each non-NULL one receives wrt_job_output_count doubles) are optional. */
+int wrt_serve_run(wrt_ctx* ctx, const wrt_serve_cfg* cfg, wrt_serve_report* rep,
+                  double* latency_ms, double* const* outputs);
+
 /* ----------------------------------------------------------- synthetic inputs --- */
 /* Seeded ellipsoid phantom on [-1,1]^3 (no reference counterpart): nell ellipsoids with
  * centres uniform in |c| < 0.5, semi-axes U(0.05,0.4), uniform random rotations and

@@ -1,0 +1,60 @@
+"""GPU tests of the serve loop: every request's result is the same in the mix as alone
+(FP32 forms: relative L2 <= 1e-6, the FP64 atomics' summation order being the only
+difference; FP64 reduction: bitwise), admission under a small HBM budget evicts and still
+serves every request, and the latency report is consistent."""
+import numpy as np
+import pytest
+
+import orc
+import wrt
+
+pytestmark = pytest.mark.gpu
+
+
+def _alone(ctx, req):
+    j = ctx.serve_job(req)
+    j.run()
+    out = j.fetch()
+    j.close()
+    return out
+
+
+def test_serve_results_alone_equal_mix(ctx):
+    cfg = wrt.serve_cfg(qps=400.0, nreq=30, seed=11, edges=(32, 48, 64), ndatasets=2,
+                        max_inflight=4)
+    rep, lat, outs = ctx.serve(cfg, keep=range(30))
+    assert rep["completed"] == 30 and np.all(lat > 0)
+    assert rep["peak_inflight"] >= 2  # requests overlapped on the device
+    for i, got in outs.items():
+        req = wrt.serve_describe(cfg, i)
+        want = _alone(ctx, req)
+        if req.op == wrt.OP["reduction"]:
+            assert np.array_equal(got, want), i
+        else:
+            assert orc.rel_l2(got, want) <= 1e-6, (i, orc.rel_l2(got, want))
+
+
+def test_serve_admission_evicts_under_budget(ctx):
+    cfg = wrt.serve_cfg(qps=0.0, nreq=24, seed=5, edges=(48, 64), ndatasets=3, max_inflight=2)
+    sizes = []
+    for i in range(cfg.nreq):
+        r = wrt.serve_describe(cfg, i)
+        phi, th, w, s, z = ctx.serve_grids(r.edge)
+        kw = dict(phi=phi, theta=th, shift=s)
+        name = {v: k for k, v in wrt.OP.items()}[r.op]
+        if name == "reduction":
+            kw["z"] = z
+        else:
+            kw["n"] = r.edge
+        if name == "adjoint":
+            kw["theta_w"] = w
+        prec = {v: k for k, v in wrt.PREC.items()}[r.prec]
+        sizes.append(wrt.job_bytes(name, prec, **kw))
+    budget = 2 * max(sizes) + 1
+    cfg = wrt.serve_cfg(qps=0.0, nreq=24, seed=5, edges=(48, 64), ndatasets=3, max_inflight=2,
+                        hbm_budget=budget)
+    rep, lat, _ = ctx.serve(cfg)
+    assert rep["completed"] == 24
+    assert rep["evictions"] > 0 and rep["peak_bytes"] <= budget
+    assert 0 < rep["p50_ms"] <= rep["p99_ms"] <= rep["max_ms"]
+    assert abs(rep["p99_ms"] - np.sort(lat)[int(np.ceil(0.99 * 24)) - 1]) < 1e-9

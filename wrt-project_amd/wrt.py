@@ -43,6 +43,31 @@ class JobDesc(C.Structure):
                 ("z", C.c_void_p), ("nz", C.c_int)]
 
 
+class ServeCfg(C.Structure):
+    _fields_ = [("qps", C.c_double), ("nreq", C.c_int), ("seed", C.c_ulonglong),
+                ("mix", C.c_double * 3), ("edges", C.POINTER(C.c_int)), ("nedges", C.c_int),
+                ("ndatasets", C.c_int), ("hbm_budget", C.c_size_t), ("max_inflight", C.c_int),
+                ("plain", C.c_int)]
+
+
+class ServeReq(C.Structure):
+    _fields_ = [("op", C.c_int), ("prec", C.c_int), ("edge", C.c_int), ("dataset", C.c_int),
+                ("data_seed", C.c_ulonglong), ("arrival_s", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ServeReport(C.Structure):
+    _fields_ = [("completed", C.c_int), ("wall_s", C.c_double), ("p50_ms", C.c_double),
+                ("p99_ms", C.c_double), ("mean_ms", C.c_double), ("max_ms", C.c_double),
+                ("samples", C.c_double), ("samples_per_s", C.c_double), ("loads", C.c_int),
+                ("evictions", C.c_int), ("peak_inflight", C.c_int), ("peak_bytes", C.c_size_t)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 _lib = None
 
 # every symbol declared in include/wrt.h
@@ -54,7 +79,8 @@ ABI_SYMBOLS = [
     "wrt_job_create", "wrt_job_load", "wrt_job_load_synthetic", "wrt_job_run",
     "wrt_job_fetch", "wrt_job_destroy", "wrt_batch_run", "wrt_phantom",
     "wrt_phantom_params", "wrt_gaussian", "wrt_radon_count_incube", "wrt_job_launch",
-    "wrt_job_poll", "wrt_job_wait",
+    "wrt_job_poll", "wrt_job_wait", "wrt_serve_describe", "wrt_serve_grids", "wrt_serve_desc",
+    "wrt_serve_input", "wrt_serve_run",
 ]
 
 
@@ -104,6 +130,11 @@ def lib():
         L.wrt_phantom_params.argtypes = [I, C.c_ulonglong, P]
         L.wrt_gaussian.argtypes = [C.c_size_t, C.c_ulonglong, P]
         L.wrt_radon_count_incube.argtypes = [P, I, P, I, P, I, P, I, P]
+        L.wrt_serve_describe.argtypes = [P, I, P]
+        L.wrt_serve_grids.argtypes = [P, I, P, P, P, P, P]
+        L.wrt_serve_desc.argtypes = [P, P, P, P, P, P, P]
+        L.wrt_serve_input.argtypes = [P, P]
+        L.wrt_serve_run.argtypes = [P, P, P, P, P]
         _lib = L
     return _lib
 
@@ -257,6 +288,47 @@ class Context:
              "wrt_radon_count_incube")
         return out.value
 
+    def serve_grids(self, edge):
+        """phi, theta, theta_w, shift, z of a serve request of this cube edge"""
+        phi, th, w = np.empty(edge // 2), np.empty(edge // 4), np.empty(edge // 4)
+        s, z = np.empty(edge + 1), np.empty(edge)
+        _chk(lib().wrt_serve_grids(self.h, edge, _p(phi), _p(th), _p(w), _p(s), _p(z)),
+             "wrt_serve_grids")
+        return phi, th, w, s, z
+
+    def serve_job(self, req):
+        """A job for a serve request, on its grids, loaded with its dataset's input."""
+        phi, th, w, s, z = self.serve_grids(req.edge)
+        name = {v: k for k, v in OP.items()}[req.op]
+        prec = {v: k for k, v in PREC.items()}[req.prec]
+        kw = dict(phi=phi, theta=th, shift=s)
+        if name == "reduction":
+            kw["z"] = z
+        else:
+            kw["n"] = req.edge
+        if name == "adjoint":
+            kw["theta_w"] = w
+        j = self.job(name, prec, **kw)
+        j.load(serve_input(req))
+        return j
+
+    def serve(self, cfg, keep=()):
+        """Run the stream; returns (report dict, latencies in ms, {request: output})."""
+        L = lib()
+        lat = np.empty(cfg.nreq)
+        outs = {}
+        ptrs = (C.c_void_p * cfg.nreq)()
+        for i in keep:
+            r = serve_describe(cfg, i)
+            e = r.edge
+            cnt = {OP["radon"]: (e + 1) * (e // 2) * (e // 4), OP["adjoint"]: e ** 3,
+                   OP["reduction"]: (e + 1) * (e // 2) * (e // 4)}[r.op]
+            outs[i] = np.empty(cnt)
+            ptrs[i] = outs[i].ctypes.data
+        rep = ServeReport()
+        _chk(L.wrt_serve_run(self.h, C.byref(cfg), C.byref(rep), _p(lat), ptrs), "wrt_serve_run")
+        return rep.as_dict(), lat, outs
+
     def job(self, op, prec="f64", n=0, phi=None, theta=None, theta_w=None, shift=None, z=None,
             nz=None):
         return Job(self, op, prec, n, phi, theta, theta_w, shift, z, nz)
@@ -342,3 +414,30 @@ class Job:
 def job_bytes(op, prec="f64", **kw):
     d, keep = make_desc(op, prec, **kw)
     return lib().wrt_job_bytes(C.byref(d))
+
+
+# ------------------------------------------------------------------------ serve loop ---
+def serve_cfg(qps=8.0, nreq=64, seed=1, mix=(0.5, 0.3, 0.2), edges=(64, 96, 128, 192, 256),
+              ndatasets=2, hbm_budget=0, max_inflight=4, plain=False):
+    """A wrt_serve_cfg and the array it points to (keep both alive while in use)."""
+    e = (C.c_int * len(edges))(*edges)
+    cfg = ServeCfg(qps, nreq, seed, (C.c_double * 3)(*mix), e, len(edges), ndatasets,
+                   hbm_budget, max_inflight, int(bool(plain)))
+    cfg._edges = e
+    return cfg
+
+
+def serve_describe(cfg, i):
+    r = ServeReq()
+    _chk(lib().wrt_serve_describe(C.byref(cfg), i, C.byref(r)), "wrt_serve_describe")
+    return r
+
+
+def serve_input(req):
+    """The request's dataset input as the flat array wrt_job_load takes."""
+    e = req.edge
+    count = {OP["radon"]: e ** 3, OP["adjoint"]: (e + 1) * (e // 2) * (e // 4),
+             OP["reduction"]: e * (e + 1) * (e // 2)}[req.op]
+    out = np.empty(count)
+    _chk(lib().wrt_serve_input(C.byref(req), _p(out)), "wrt_serve_input")
+    return out
