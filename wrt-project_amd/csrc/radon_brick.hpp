@@ -1,0 +1,451 @@
+// radon_brick.hpp -- the brick task of the fast plane integrals (radon_grid, FP32 or BF16
+// values), shared by the per-job launches (k_fast.hip) and the persistent batch launch
+// (k_batch.hip).
+//
+// Work unit: one workgroup per (non-empty 32^3 brick, group of directions).  The brick is
+// staged once into LDS as FP32; every plane (shift, direction) that cuts the brick is a
+// "section".  A wavefront walks a section in 8-row bands: lane (r, k) = (lane>>3, lane&7)
+// handles lattice row i1 = band + r and columns i2 = c0 + k, c0 stepping by 8 across the
+// band's conservative column interval.  Per sample the cell index is decided exactly as
+// radon_grid/src/integral.c:47-51 + interp.c:70-72 decide it in binary64
+//   x = ((s*n) + (a*i1)) + (b*i2),  ix = (int)((x + 1) / (2/n)),  index n -> n-1,
+// (the per-column products b*i2 are formed in registers, the division is an exact
+// power-of-two scaling or a corrected reciprocal), so only the value rounding (FP32/BF16)
+// and the summation order differ from the plain form.  Samples whose cell lies in the
+// brick add its FP32 value to a per-band partial, folded into a Kahan-compensated FP32
+// lane sum; a section's lane sums are reduced over the wavefront in FP64 and added to the
+// plane's FP64 accumulator (one atomic per section and brick).
+#ifndef WRT_RADON_BRICK_HPP
+#define WRT_RADON_BRICK_HPP
+#include "wrt_common.hpp"
+#include "wrt_fast.h"
+
+#include <limits.h>
+
+#define FB WRT_BRICK
+#define FR_WAVES 16
+// skewed LDS pitches: bank = (5*lx + ly + lz) mod 32, so a small patch of a plane spreads
+// over the banks for every orientation except a few measure-zero ones
+#define FR_PY 33
+#define FR_PX (33 * 32 + 5)
+
+static_assert(FB == 32, "the band walk assumes 32^3 bricks");
+
+// RN(u / d) for d = RN(2/n) when n is not a power of two: reciprocal estimate plus one
+// exact-residual correction (Markstein); equal to the IEEE quotient for these operands
+// (checked on the host against the C division for every n <= 1024 in the tests).
+__device__ __forceinline__ double div_rn(double u, double d, double inv) {
+  const double y = u * inv;
+  const double r = fma(-y, d, u);
+  return fma(r, inv, y);
+}
+
+// a*b + c on the 24-bit multiplier (full rate); a, b < 2^24, b wave-uniform (held in an
+// SGPR, which keeps loop-invariant constants out of the VGPR budget).  Written out because
+// the compiler turns small known-range products into the quarter-rate 32-bit multiply.
+__device__ __forceinline__ unsigned mad_u24(unsigned a, unsigned b, unsigned c) {
+  unsigned r;
+  asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "s"(b), "v"(c));
+  return r;
+}
+
+// Wavefront sum of a double with DPP lane moves (no LDS round trips): Hillis-Steele scan
+// within 16-lane rows (row_shr 1, 2, 4, 8, zero fill), then row broadcasts 15 and 31; the
+// total ends in lane 63 and is returned to every lane.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_move(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWMASK, 0xf, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWMASK, 0xf, true);
+  return __hiloint2double(hi, lo);
+}
+// min / max over the 16 lanes of each DPP row (row_ror 1, 2, 4, 8): every lane of a row ends
+// with its row's extremum
+template <int CTRL>
+__device__ __forceinline__ int dpp_ror(int v) {
+  return __builtin_amdgcn_update_dpp(v, v, CTRL, 0xf, 0xf, false);
+}
+__device__ __forceinline__ int row_min16(int v) {
+  v = min(v, dpp_ror<0x121>(v));
+  v = min(v, dpp_ror<0x122>(v));
+  v = min(v, dpp_ror<0x124>(v));
+  return min(v, dpp_ror<0x128>(v));
+}
+__device__ __forceinline__ int row_max16(int v) {
+  v = max(v, dpp_ror<0x121>(v));
+  v = max(v, dpp_ror<0x122>(v));
+  v = max(v, dpp_ror<0x124>(v));
+  return max(v, dpp_ror<0x128>(v));
+}
+
+__device__ __forceinline__ double wave_sum_dpp(double v) {
+  v += dpp_move<0x111, 0xf>(v);  // row_shr:1
+  v += dpp_move<0x112, 0xf>(v);  // row_shr:2
+  v += dpp_move<0x114, 0xf>(v);  // row_shr:4
+  v += dpp_move<0x118, 0xf>(v);  // row_shr:8
+  v += dpp_move<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+  v += dpp_move<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ float bf16_to_f32(unsigned short v) {
+  return __uint_as_float(((unsigned)v) << 16);
+}
+
+// Exact cube test of one axis on a row: x(c) = w + b*c in binary64 (the reference's
+// rounding) is monotone in c, so {c : x >= -1} and {c : x <= 1} are a prefix or a suffix of
+// the row.  Narrows [rl, rh] to the columns passing the tests that lo / hi ask for.  The
+// continuous bounds already lie within ~1e-6 columns of the answer, so the first probe
+// usually settles it; a bisection covers rows almost parallel to the face.
+__device__ __forceinline__ void cube_ends(double w, double b, bool lo, bool hi, int& rl,
+                                          int& rh) {
+  if (rl > rh) return;
+  for (int side = 0; side < 2; ++side) {
+    if (side == 0 ? !lo : !hi) continue;
+    // P(c): the test holds at column c; suffix-type when x moves into the cube as c grows
+    const bool suffix = side == 0 ? b > 0.0 : b < 0.0;
+    auto P = [&](int c) {
+      const double x = w + b * (double)c;
+      return side == 0 ? x >= -1.0 : x <= 1.0;
+    };
+    if (b == 0.0) {
+      if (!P(rl)) rh = rl - 1;
+    } else if (suffix) {
+      if (!P(rl)) {
+        if (!P(rh)) {
+          rh = rl - 1;
+        } else {
+          int l = rl, h = rh;  // P(l) false, P(h) true
+          while (h - l > 1) {
+            const int m = (l + h) >> 1;
+            if (P(m)) h = m; else l = m;
+          }
+          rl = h;
+        }
+      }
+    } else {
+      if (!P(rh)) {
+        if (!P(rl)) {
+          rh = rl - 1;
+        } else {
+          int l = rl, h = rh;  // P(l) true, P(h) false
+          while (h - l > 1) {
+            const int m = (l + h) >> 1;
+            if (P(m)) l = m; else h = m;
+          }
+          rh = l;
+        }
+      }
+    }
+    if (rl > rh) return;
+  }
+}
+
+// The LDS a brick task works in: the brick in the skewed layout plus one zero slot that
+// rejected samples read (branch-free accumulation), per wave the packed first/last column
+// of every row of its current section, and the workgroup's next direction.
+struct radon_lds {
+  float sb[FB * FR_PX + 1];
+  int rowlo[FR_WAVES][136];
+  int next_dir;
+};
+
+// One task: brick `brick` of a.bricks against directions [grp*dgroup, (grp+1)*dgroup).
+// Called by every thread of a FR_WAVES*64-thread workgroup; returns with the workgroup's
+// accumulator updates issued (no barrier at the end: a caller that reuses L waits first).
+template <bool BND, bool BF16, bool POW2>
+__device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, int brick, int grp,
+                                                 double h, double dc, double inv_dc,
+                                                 radon_lds& L) {
+  float* const sb = L.sb;
+  int (*const rowlo)[136] = L.rowlo;
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int nb = a.nb, n = a.n;
+  const int bxi = brick / (nb * nb), byi = (brick / nb) % nb, bzi = brick % nb;
+  const int b0x = bxi * FB, b0y = byi * FB, b0z = bzi * FB;
+
+  // stage the brick into the skewed LDS layout
+  {
+    const size_t base = (size_t)brick * (FB * FB * FB);
+    for (int i = threadIdx.x; i < FB * FB * FB; i += blockDim.x) {
+      const int lx = i >> 10, ly = (i >> 5) & 31, lz = i & 31;
+      float v;
+      if (BF16) v = bf16_to_f32(((const unsigned short*)a.bricks)[base + i]);
+      else v = ((const float*)a.bricks)[base + i];
+      sb[1 + lx * FR_PX + ly * FR_PY + lz] = v;
+    }
+    if (threadIdx.x == 0) {
+      sb[0] = 0.f;
+      L.next_dir = grp * a.dgroup;
+    }
+  }
+  __syncthreads();
+
+  const double delta = (1.0 / (n));
+  const double lo[3] = {-1.0 + b0x * dc, -1.0 + b0y * dc, -1.0 + b0z * dc};
+  const double hi[3] = {-1.0 + (b0x + FB) * dc, -1.0 + (b0y + FB) * dc, -1.0 + (b0z + FB) * dc};
+  const double fb0x = (double)b0x, fb0y = (double)b0y, fb0z = (double)b0z;
+  const int L0 = -2 * n, L1 = 2 * n - 1;  // lattice index range of i1 and i2 (bounds of R0..R1, C0..C1)
+  // Faces of the cube the brick touches.  Only along those axes can a sample whose cell is
+  // in the brick fail the reference's cube test |x| <= 1 (elsewhere x < -1 truncates to cell
+  // 0, outside the brick, and x > 1 lands past it).  Face bricks apply the cube test exactly
+  // at the ends of every row (x is monotone along a row) and mask the lanes past a row's
+  // end; x == 1 exactly gives index n, which the reference maps to n - 1.
+  const bool flo[3] = {b0x == 0, b0y == 0, b0z == 0};
+  const bool fhi[3] = {b0x + FB >= n, b0y + FB >= n, b0z + FB >= n};
+  const int cxh = fhi[0] ? n - 1 : INT_MAX, cyh = fhi[1] ? n - 1 : INT_MAX,
+            czh = fhi[2] ? n - 1 : INT_MAX;
+  const int r_lane = lane >> 3, k_lane = lane & 7;
+  // the brick's farthest corner lies within the support sphere
+  bool inside;
+  {
+    double f2c = 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) f2c += fmax(lo[c] * lo[c], hi[c] * hi[c]);
+    inside = f2c * (1.0 + 1e-12) < a.rho2;
+  }
+
+  const int d_end = min((grp + 1) * a.dgroup, a.ndir);
+  // waves take the group's directions one at a time from a shared counter, so that they
+  // finish together however unequal the directions' sections are
+  for (;;) {
+    int d = 0;
+    if (lane == 0) d = atomicAdd(&L.next_dir, 1);
+    d = __builtin_amdgcn_readfirstlane(d);
+    if (d >= d_end) break;
+    const double* D = a.dirs + 9 * (size_t)d;
+    const double nrm[3] = {D[0], D[1], D[2]};
+    const double av[3] = {D[3], D[4], D[5]};
+    const double bv[3] = {D[6], D[7], D[8]};
+    // projections of the brick box on a, b and the normal (box corners extremise them)
+    double pa0 = 0, pa1 = 0, pb0 = 0, pb1 = 0, pn0 = 0, pn1 = 0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      pa0 += fmin(lo[c] * av[c], hi[c] * av[c]);
+      pa1 += fmax(lo[c] * av[c], hi[c] * av[c]);
+      pb0 += fmin(lo[c] * bv[c], hi[c] * bv[c]);
+      pb1 += fmax(lo[c] * bv[c], hi[c] * bv[c]);
+      pn0 += fmin(lo[c] * nrm[c], hi[c] * nrm[c]);
+      pn1 += fmax(lo[c] * nrm[c], hi[c] * nrm[c]);
+    }
+    const double aa = av[0] * av[0] + av[1] * av[1] + av[2] * av[2];
+    const double bb = bv[0] * bv[0] + bv[1] * bv[1] + bv[2] * bv[2];
+    const int R0 = max(L0, (int)floor(pa0 / aa) - 2), R1 = min(L1, (int)ceil(pa1 / aa) + 2);
+    const int C0 = max(L0, (int)floor(pb0 / bb) - 2), C1 = min(L1, (int)ceil(pb1 / bb) + 2);
+    if (R0 > R1 || C0 > C1) continue;
+    // shifts with s in [pn0 - eps, pn1 + eps] (sorted shift grid, binary search)
+    const double eps = 4.0 * delta;
+    int j0, j1;
+    {
+      int l = 0, r = a.nshift;
+      while (l < r) { const int m = (l + r) >> 1; if (a.shift[m] < pn0 - eps) l = m + 1; else r = m; }
+      j0 = l;
+      l = 0; r = a.nshift;
+      while (l < r) { const int m = (l + r) >> 1; if (a.shift[m] <= pn1 + eps) l = m + 1; else r = m; }
+      j1 = l - 1;
+    }
+    if (j0 > j1) continue;
+    // reciprocals of the column steps (for the conservative row intervals)
+    double ib[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) ib[c] = (fabs(bv[c]) * (4.0 * n) < 1e-9 * dc) ? 0.0 : 1.0 / bv[c];
+    const double ibb = 1.0 / bb, iaa = 1.0 / aa;
+    // this lane's box edge (lanes 0..11): axis ek, the other two coordinates at lo/hi
+    // corners.  The plane n.p = s meets it (if at all) at coordinate pk = (s - rest)/n_k,
+    // which is on the edge for s in [e_s0, e_s1]; the vertex's row a.p/|a|^2 is then the
+    // linear e_A s + e_B.  An edge almost parallel to the plane (|n_k| < 1e-9, where the
+    // linear form would cancel badly) contributes the rows of its whole length instead:
+    // e_A = 0, e_B its middle row, e_W its half-length in rows.
+    double e_A = 0.0, e_B = 0.0, e_W = 0.0, e_s0 = 1e300, e_s1 = -1e300;
+    {
+      const int ek = lane >> 2, ek1 = ek == 2 ? 0 : ek + 1, ek2 = ek == 0 ? 2 : ek - 1;
+      const double e_n = ek == 0 ? nrm[0] : (ek == 1 ? nrm[1] : nrm[2]);
+      const double e_n1 = ek1 == 0 ? nrm[0] : (ek1 == 1 ? nrm[1] : nrm[2]);
+      const double e_n2 = ek2 == 0 ? nrm[0] : (ek2 == 1 ? nrm[1] : nrm[2]);
+      const double e_a = ek == 0 ? av[0] : (ek == 1 ? av[1] : av[2]);
+      const double e_a1 = ek1 == 0 ? av[0] : (ek1 == 1 ? av[1] : av[2]);
+      const double e_a2 = ek2 == 0 ? av[0] : (ek2 == 1 ? av[1] : av[2]);
+      const double e_lo = ek == 0 ? lo[0] : (ek == 1 ? lo[1] : lo[2]);
+      const double e_hi = ek == 0 ? hi[0] : (ek == 1 ? hi[1] : hi[2]);
+      const double e_p1 = (lane & 1) ? (ek1 == 0 ? hi[0] : (ek1 == 1 ? hi[1] : hi[2]))
+                                     : (ek1 == 0 ? lo[0] : (ek1 == 1 ? lo[1] : lo[2]));
+      const double e_p2 = (lane & 2) ? (ek2 == 0 ? hi[0] : (ek2 == 1 ? hi[1] : hi[2]))
+                                     : (ek2 == 0 ? lo[0] : (ek2 == 1 ? lo[1] : lo[2]));
+      if (lane < 12 && e_n != 0.0) {
+        const double tol = 1e-9 * dc;
+        const double rest = e_n1 * e_p1 + e_n2 * e_p2, arest = e_a1 * e_p1 + e_a2 * e_p2;
+        const double u = rest + (e_lo - tol) * e_n, v = rest + (e_hi + tol) * e_n;
+        e_s0 = fmin(u, v);
+        e_s1 = fmax(u, v);
+        if (fabs(e_n) >= 1e-9) {
+          e_A = e_a / e_n * iaa;
+          e_B = (arest - e_a * rest / e_n) * iaa;
+        } else {
+          const double r0 = (e_a * e_lo + arest) * iaa, r1 = (e_a * e_hi + arest) * iaa;
+          e_B = 0.5 * (r0 + r1);
+          e_W = 0.5 * fabs(r1 - r0);
+        }
+      }
+    }
+    const double ia = 1.0 / sqrt(aa);
+    for (int j = j0; j <= j1; ++j) {
+      const double s = a.shift[j];
+      // the plane meets the support sphere in a circle of radius sqrt(rho2 - s^2) about the
+      // foot point s*n, where row 0 passes: rows farther than that (+1) are empty
+      const double rc2 = a.rho2 - s * s;
+      if (rc2 < 0.0) continue;
+      const double rr = sqrt(rc2) * ia + 1.0;
+      const double S[3] = {s * nrm[0], s * nrm[1], s * nrm[2]};
+      // rows of this section: the section polygon's vertices lie on the box edges; the row
+      // of a point p of the plane is a.p / |a|^2 (a is orthogonal to n and b), +-2 rows margin
+      int r_lo = INT_MAX, r_hi = INT_MIN;
+      if (s >= e_s0 && s <= e_s1) {
+        const double r = e_A * s + e_B;
+        r_lo = (int)fmax(floor(r - e_W) - 2.0, (double)L0);
+        r_hi = (int)fmin(ceil(r + e_W) + 2.0, (double)L1);
+      }
+      r_lo = __builtin_amdgcn_readlane(row_min16(r_lo), 0);
+      r_hi = __builtin_amdgcn_readlane(row_max16(r_hi), 0);
+      const int SR0 = max(max(R0, r_lo), (int)floor(-rr)), SR1 = min(min(R1, r_hi), (int)ceil(rr));
+      if (SR0 > SR1) continue;
+      const int nrows = SR1 - SR0 + 1;  // <= 122 by geometry (brick diameter + margins)
+      const int nbands = (nrows + 7) >> 3;
+      // conservative column interval of every row (rows SR0 + lane and SR0 + 64 + lane): first
+      // column into the wave's row table, 8-column step count reduced to a per-band maximum
+      int bsteps[2];
+#pragma unroll
+      for (int half = 0; half < 2; ++half) {
+        if (64 * half >= nrows) {  // uniform: the section has at most 64 rows
+          bsteps[half] = 0;
+          continue;
+        }
+        const int t = lane + 64 * half;
+        int rl = C0, rh = C0 - 1, nst = 0;
+        if (t < nrows) {
+          const double f1 = (double)(SR0 + t);
+          double tlo = -1e300, thi = 1e300;
+          bool empty = false;
+          // the row's points p(t) = w + b t within the support sphere |p|^2 <= rho2 (every
+          // sample outside it reads a zero cell, so clipping the row there changes no sum);
+          // nothing to clip when the whole brick lies inside the sphere
+          if (!inside) {
+            const double wx = S[0] + av[0] * f1, wy = S[1] + av[1] * f1, wz = S[2] + av[2] * f1;
+            const double qb = wx * bv[0] + wy * bv[1] + wz * bv[2];
+            const double qc = wx * wx + wy * wy + wz * wz - a.rho2;
+            const double disc = qb * qb - bb * qc;
+            if (disc < 0.0) {
+              empty = true;
+            } else {
+              const double sq = sqrt(disc);
+              tlo = (-qb - sq) * ibb;
+              thi = (-qb + sq) * ibb;
+            }
+          }
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            const double w = S[c] + av[c] * f1;
+            const double tol = 1e-9 * dc;
+            const double hc = fmin(hi[c], 1.0);  // the cube's face (n not a multiple of 32)
+            if (ib[c] == 0.0) {
+              if (w < lo[c] - tol || w > hc + tol) empty = true;
+            } else if (bv[c] > 0) {
+              tlo = fmax(tlo, (lo[c] - tol - w) * ib[c]);
+              thi = fmin(thi, (hc + tol - w) * ib[c]);
+            } else {
+              tlo = fmax(tlo, (hc + tol - w) * ib[c]);
+              thi = fmin(thi, (lo[c] - tol - w) * ib[c]);
+            }
+          }
+          if (!empty && tlo <= thi + 2.0) {
+            // the continuous bounds are accurate to ~1e-12 columns (and the slab tolerance
+            // widens them), so the integer columns within 1e-6 of them are the candidates
+            tlo = fmax(tlo, (double)C0 - 1.0);
+            thi = fmin(thi, (double)C1 + 1.0);
+            rl = max(C0, (int)ceil(tlo - 1e-6));
+            rh = min(C1, (int)floor(thi + 1e-6));
+            if (BND) {
+#pragma unroll
+              for (int c = 0; c < 3; ++c)
+                if (flo[c] || fhi[c]) cube_ends(S[c] + av[c] * f1, bv[c], flo[c], fhi[c], rl, rh);
+            }
+            nst = rh >= rl ? ((rh - rl + 8) >> 3) : 0;
+          }
+        }
+        // first and last column, 16 bits each (|i2| <= 2n < 2^15)
+        rowlo[wv][t] = (int)(((unsigned)rl & 0xffffu) | ((unsigned)rh << 16));
+        // max over the 8 consecutive rows of a band (lanes 8b .. 8b+7)
+        nst = max(nst, __shfl_xor(nst, 1, 64));
+        nst = max(nst, __shfl_xor(nst, 2, 64));
+        nst = max(nst, __shfl_xor(nst, 4, 64));
+        bsteps[half] = nst;
+      }
+      if (__ballot((bsteps[0] | bsteps[1]) != 0) == 0)
+        continue;  // no row of the section meets the brick and the sphere
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      float ks = 0.f, kc = 0.f;  // Kahan-compensated lane sum of the section
+      int rlo_next = rowlo[wv][r_lane];
+      for (int k = 0; k < nbands; ++k) {
+        const int t = 8 * k + r_lane;  // this lane's row within the section
+        const int rlo = (rlo_next << 16) >> 16;
+        const double frh = (double)(rlo_next >> 16);  // the row's last column
+        if (k + 1 < nbands) rlo_next = rowlo[wv][t + 8];
+        const int steps = (k < 8) ? __builtin_amdgcn_readlane(bsteps[0], 8 * k)
+                                  : __builtin_amdgcn_readlane(bsteps[1], 8 * (k - 8));
+        if (steps == 0) continue;
+        const int i1 = SR0 + t;
+        // Rows past R1 and columns past the lattice end need no test: a plane's cube section
+        // lies within sqrt(3) < 2 of the plane centre, inside the [-2n, 2n) lattice, so only
+        // the brick test below can reject a sample.
+        const double f1 = (double)i1;
+        const double Rx = S[0] + av[0] * f1, Ry = S[1] + av[1] * f1, Rz = S[2] + av[2] * f1;
+        float bs = 0.f;
+        double f2 = (double)(rlo + k_lane);  // column i2, stepped exactly in binary64
+        for (int m = 0; m < steps; ++m, f2 += 8.0) {
+          // per-column products b*i2, rounded as the reference rounds delta*v2*i2
+          const double x = Rx + bv[0] * f2, y = Ry + bv[1] * f2, z = Rz + bv[2] * f2;
+          double qx, qy, qz;
+          if (POW2) {
+            // RN(x + 1) * h == RN(x*h + h) for h = n/2 a power of two: scaling by 2^k
+            // commutes with rounding, so one fma gives the reference's quotient exactly
+            qx = fma(x, h, h); qy = fma(y, h, h); qz = fma(z, h, h);
+          } else {
+            qx = div_rn(x + 1.0, dc, inv_dc);
+            qy = div_rn(y + 1.0, dc, inv_dc);
+            qz = div_rn(z + 1.0, dc, inv_dc);
+          }
+          // truncation as in zero_interp; for an interior brick a negative quotient (x < -1)
+          // truncates to 0 <= b0 - 32 and a quotient above n lands past the brick, so the
+          // brick test below also is the cube test.  Face bricks: the row ends hold the cube
+          // test, and x == 1 (index n) maps to n - 1.
+          int ix = (int)qx, iy = (int)qy, iz = (int)qz;
+          if (BND) {
+            ix = min(ix, cxh);
+            iy = min(iy, cyh);
+            iz = min(iz, czh);
+          }
+          const int lx = ix - b0x, ly = iy - b0y, lz = iz - b0z;
+          // all three offsets in [0, 32): a negative offset sets high bits of the OR
+          bool ok = ((unsigned)lx | (unsigned)ly | (unsigned)lz) < (unsigned)FB;
+          if (BND) ok = ok && f2 <= frh;
+          // byte offset of the cell: slot 0 holds the zero that rejected samples read (an
+          // inline constant in the select), the brick starts at slot 1
+          const unsigned off = mad_u24((unsigned)lx, 4u * FR_PX,
+                                       mad_u24((unsigned)ly, 4u * FR_PY, (unsigned)lz * 4u + 4u));
+          bs += *(const float*)((const char*)sb + (ok ? off : 0u));
+        }
+        const float y = bs - kc;
+        const float tt = ks + y;
+        kc = (tt - ks) - y;
+        ks = tt;
+      }
+      const double tot = wave_sum_dpp((double)ks - (double)kc);
+      if (lane == 0 && tot != 0.0) unsafeAtomicAdd(a.acc + (size_t)j * a.ndir + d, tot * delta * delta);
+    }
+  }
+}
+
+#endif
