@@ -184,10 +184,20 @@ int wrt_job_wait(wrt_job* job, wrt_stats* st);
 int wrt_job_fetch(wrt_job* job, double* out);
 void wrt_job_destroy(wrt_job* job);
 
-/* Run many jobs (forward and adjoint, uneven sizes) as one persistent launch captured in
- * a HIP graph; the graph is replayed on later calls with the same job list.  Each job's
- * output equals its wrt_job_run output within the documented tolerance. */
+/* Run many jobs (forward and adjoint, uneven sizes) as one submission (no reference
+ * counterpart).  The FP32/BF16 plane-integral jobs and the FP32 backprojection jobs run as
+ * one persistent launch captured in a HIP graph: their tasks (brick x direction group,
+ * 1024-voxel blocks) are ordered longest first and taken by the CUs from a device counter.
+ * The direction-group size is autotuned on the first call for a list shape (cached in
+ * memory and in the file WRT_TUNE_CACHE or $HOME/.cache/wrt_batch_tune.txt; WRT_AUTOTUNE=0
+ * disables tuning).  The graph is replayed while the list and the jobs' loads stay the same;
+ * any other job (plain FP64 forms, reductions) runs on its own stream beside the graph.
+ * Each job's output then equals its wrt_job_run output up to the summation order of the
+ * FP64 plane accumulators (relative L2 <= 1e-6).  st: kernel_seconds = the graph's device
+ * time, seconds = the longest of it and the other jobs' device times. */
 int wrt_batch_run(wrt_ctx* ctx, wrt_job* const* jobs, int njobs, wrt_stats* st);
+/* direction-group size of the context's current batch plan (0 if none) */
+int wrt_batch_dgroup(wrt_ctx* ctx);
 
 /* ------------------------------------------------------------------- serve loop --- */
 /* A request stream served from one context (no reference counterpart: the reference runs

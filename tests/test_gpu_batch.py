@@ -1,0 +1,91 @@
+"""GPU tests of the batch runner: many fast jobs (plane integrals of uneven sizes in FP32
+and BF16, FP32 backprojections) in one persistent graph launch, beside plain FP64 jobs on
+their own streams.  Each job's batched output equals its own run: backprojections and FP64
+jobs bitwise, plane integrals to relative L2 <= 1e-6 (the FP64 accumulators' atomic
+summation order is the only difference).  Replays, reloads and the autotuner's cache are
+covered too."""
+import os
+
+import numpy as np
+import pytest
+
+import orc
+import wrt
+
+pytestmark = pytest.mark.gpu
+
+
+def _jobs(ctx):
+    jobs = []
+    for k, (op, prec, e) in enumerate([("radon", "f32", 32), ("radon", "f32", 48),
+                                        ("radon", "f32", 64), ("radon", "bf16", 64),
+                                        ("adjoint", "f32", 32), ("adjoint", "f32", 48),
+                                        ("radon", "f64", 32), ("adjoint", "f64", 24)]):
+        phi, th, w, s = ctx.radon_grid_init(e // 2, e // 4, e + 1)
+        if op == "radon":
+            j = ctx.job("radon", prec, n=e, phi=phi, theta=th, shift=s)
+            j.load(wrt.phantom(e, True, 10, 100 + k, 0.01))
+        else:
+            j = ctx.job("adjoint", prec, n=e, phi=phi, theta=th, theta_w=w, shift=s)
+            j.load(wrt.gaussian((len(s), len(phi), len(th)), 200 + k))
+        jobs.append((op, prec, j))
+    return jobs
+
+
+def _alone(jobs):
+    out = []
+    for _, _, j in jobs:
+        j.run()
+        out.append(j.fetch())
+    return out
+
+
+def _check(jobs, want):
+    for (op, prec, j), w in zip(jobs, want):
+        got = j.fetch()
+        if op == "radon" and prec != "f64":
+            assert orc.rel_l2(got, w) <= 1e-6, (op, prec, orc.rel_l2(got, w))
+        else:
+            assert np.array_equal(got, w), (op, prec)
+
+
+def test_batch_equals_alone_and_replays(ctx, tmp_path):
+    os.environ["WRT_TUNE_CACHE"] = str(tmp_path / "tune.txt")
+    try:
+        jobs = _jobs(ctx)
+        want = _alone(jobs)
+        st = ctx.batch_run([j for _, _, j in jobs])
+        assert st["kernel_seconds"] > 0 and st["logical_samples"] > 0
+        _check(jobs, want)
+        g = ctx.batch_dgroup()
+        assert g in (16, 32, 64, 128, 256)
+        line = open(tmp_path / "tune.txt").read().split()
+        assert int(line[1]) == g
+        st2 = ctx.batch_run([j for _, _, j in jobs])  # graph replay
+        _check(jobs, want)
+        assert st2["logical_samples"] == st["logical_samples"]
+        # a reload invalidates the plan: the next batch uses the new data
+        _, _, j0 = jobs[2]
+        j0.load(wrt.phantom(64, True, 10, 999, 0.0))
+        j0.run()
+        want[2] = j0.fetch()
+        ctx.batch_run([j for _, _, j in jobs])
+        _check(jobs, want)
+        for _, _, j in jobs:
+            j.close()
+    finally:
+        del os.environ["WRT_TUNE_CACHE"]
+
+
+def test_batch_without_autotune_uses_default(ctx):
+    os.environ["WRT_AUTOTUNE"] = "0"
+    try:
+        jobs = _jobs(ctx)[:3]
+        want = _alone(jobs)
+        ctx.batch_run([j for _, _, j in jobs])
+        assert ctx.batch_dgroup() == 64
+        _check(jobs, want)
+        for _, _, j in jobs:
+            j.close()
+    finally:
+        del os.environ["WRT_AUTOTUNE"]

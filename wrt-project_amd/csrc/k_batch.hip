@@ -1,0 +1,100 @@
+// k_batch.hip -- one persistent launch over the tasks of many jobs (plane integrals and
+// backprojections of volumes of any size), the form the batch runner captures in a HIP
+// graph.
+//
+// The grid is one workgroup of 1024 threads per CU (the brick task's LDS allows no more).
+// A workgroup takes the next task from a device counter, runs it, and takes another until
+// the list is exhausted, so uneven tasks balance across the CUs; the host orders the list
+// longest first.  A plane-integral task is the brick task of the per-job launch
+// (radon_brick.hpp) with the job's arguments; a backprojection task is up to 1024 voxels of
+// one job, one per thread (adjoint_voxel.hpp).  Results are the per-job launches' up to the
+// order of the FP64 accumulator atomics.
+#include "adjoint_voxel.hpp"
+#include "radon_brick.hpp"
+
+// zero the FP64 accumulators (segment blockIdx.y < nseg) and the task counter
+__global__ __launch_bounds__(256) void k_batch_prologue(const wrt_batch_seg* __restrict__ segs,
+                                                        int nseg, int* __restrict__ counter) {
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *counter = 0;
+  if ((int)blockIdx.y >= nseg) return;
+  const wrt_batch_seg sg = segs[blockIdx.y];
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < sg.count;
+       i += (long long)gridDim.x * blockDim.x)
+    sg.acc[i] = 0.0;
+}
+
+__global__ __launch_bounds__(256) void k_batch_epilogue(const wrt_batch_seg* __restrict__ segs) {
+  const wrt_batch_seg sg = segs[blockIdx.y];
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < sg.count;
+       i += (long long)gridDim.x * blockDim.x)
+    sg.out[i] = (float)sg.acc[i];
+}
+
+template <bool BF16, bool POW2>
+__device__ __forceinline__ void radon_item(const wrt_batch_item& it, const wrt_batch_task& t,
+                                           radon_lds& L) {
+  const wrt_radon_fast_args ra = it.ra;
+  if (t.c)
+    radon_brick_task<true, BF16, POW2>(ra, t.a, t.b, it.h, it.dc, it.inv_dc, L);
+  else
+    radon_brick_task<false, BF16, POW2>(ra, t.a, t.b, it.h, it.dc, it.inv_dc, L);
+}
+
+__global__ __launch_bounds__(FR_WAVES * 64) void k_batch(const wrt_batch_item* __restrict__ items,
+                                                         const wrt_batch_task* __restrict__ tasks,
+                                                         int ntasks, int* __restrict__ counter) {
+  __shared__ radon_lds L;
+  __shared__ int s_task;
+  for (;;) {
+    __syncthreads();  // the previous task is done with L and s_task
+    if (threadIdx.x == 0) s_task = atomicAdd(counter, 1);
+    __syncthreads();
+    const int ti = s_task;
+    if (ti >= ntasks) break;  // every workgroup reaches this: the counter only grows
+    const wrt_batch_task t = tasks[ti];
+    const wrt_batch_item& it = items[t.item];
+    if (it.kind == WRT_ITEM_RADON) {
+      const bool bf16 = it.ra.bf16 != 0;
+      if (it.pow2) {
+        if (bf16) radon_item<true, true>(it, t, L);
+        else radon_item<false, true>(it, t, L);
+      } else {
+        if (bf16) radon_item<true, false>(it, t, L);
+        else radon_item<false, false>(it, t, L);
+      }
+    } else {
+      if ((int)threadIdx.x < t.b) {
+        const long long k = (long long)t.a + threadIdx.x;
+        it.aout[k] = adjoint_voxel_f32((const float4*)it.adirs, (const double4*)it.atrig, it.andir,
+                                       (const float4*)it.acub, it.ans, (float)it.as0,
+                                       (float)(1.0 / it.ads), (float)it.ads, it.an, k);
+      }
+    }
+  }
+}
+
+extern "C" int wrtk_batch_prologue(const wrt_batch_seg* segs, int nseg, int* counter,
+                                   hipStream_t st) {
+  if (nseg < 0 || nseg > 65535) return WRT_E_ARG;
+  // with no segment, one block still resets the counter
+  k_batch_prologue<<<dim3(64, (unsigned)(nseg > 0 ? nseg : 1)), 256, 0, st>>>(segs, nseg, counter);
+  WRT_LAUNCH_CHECK();
+  return WRT_OK;
+}
+
+extern "C" int wrtk_batch_run(const wrt_batch_item* items, const wrt_batch_task* tasks,
+                              int ntasks, int* counter, int grid, hipStream_t st) {
+  if (ntasks <= 0) return WRT_OK;
+  if (grid < 1) return WRT_E_ARG;
+  k_batch<<<(unsigned)grid, FR_WAVES * 64, 0, st>>>(items, tasks, ntasks, counter);
+  WRT_LAUNCH_CHECK();
+  return WRT_OK;
+}
+
+extern "C" int wrtk_batch_epilogue(const wrt_batch_seg* segs, int nseg, hipStream_t st) {
+  if (nseg <= 0) return WRT_OK;
+  if (nseg > 65535) return WRT_E_ARG;
+  k_batch_epilogue<<<dim3(64, (unsigned)nseg), 256, 0, st>>>(segs);
+  WRT_LAUNCH_CHECK();
+  return WRT_OK;
+}

@@ -45,6 +45,42 @@ int wrtk_radon_fast(const wrt_radon_fast_args* a, int boundary, hipStream_t st);
 /* out[i] = (float)acc[i] */
 int wrtk_acc_to_f32(const double* acc, float* out, size_t count, hipStream_t st);
 
+/* ---- the persistent batch launch: many jobs' tasks in one grid ---- */
+enum { WRT_ITEM_RADON = 0, WRT_ITEM_ADJOINT = 1 };
+typedef struct {
+  int kind;                 /* WRT_ITEM_RADON: brick tasks; WRT_ITEM_ADJOINT: voxel tasks  */
+  int pow2;                 /* radon: n is a power of two                                 */
+  double h, dc, inv_dc;     /* radon: n/2, RN(2/n), 1/RN(2/n)                            */
+  wrt_radon_fast_args ra;   /* radon (tasks/ntasks unused)                               */
+  const void* adirs;        /* adjoint: [ndir] float4 (n.x, n.y, n.z, weight)            */
+  const void* atrig;        /* adjoint: [ndir] double4 (sin t, cos p, sin p, cos t)      */
+  const void* acub;         /* adjoint: [ndir][ns-1] float4 cubics                       */
+  float* aout;              /* adjoint: n^3 [z][x][y]                                    */
+  int andir, ans, an;
+  double as0, ads;          /* adjoint: uniform shift grid s_i = s0 + i*ds               */
+} wrt_batch_item;
+
+/* radon: (item, brick, direction group, 1 if the brick touches a cube face);
+ * adjoint: (item, first voxel, voxel count <= 1024, 0) */
+typedef struct {
+  int item, a, b, c;
+} wrt_batch_task;
+
+/* an FP64 accumulator of a radon item and its FP32 output */
+typedef struct {
+  double* acc;
+  float* out;
+  long long count;
+} wrt_batch_seg;
+
+/* zero every segment's accumulator and the task counter */
+int wrtk_batch_prologue(const wrt_batch_seg* segs, int nseg, int* counter, hipStream_t st);
+/* grid workgroups of 1024 threads take tasks from *counter until none is left */
+int wrtk_batch_run(const wrt_batch_item* items, const wrt_batch_task* tasks, int ntasks,
+                   int* counter, int grid, hipStream_t st);
+/* round every segment's accumulator to its FP32 output */
+int wrtk_batch_epilogue(const wrt_batch_seg* segs, int nseg, hipStream_t st);
+
 #ifdef __cplusplus
 }
 #endif

@@ -71,8 +71,8 @@ void wrt_batch_release(wrt_ctx* ctx); /* wrt_batch.c */
 void wrt_free(wrt_ctx* c) {
   if (!c) return;
   hipSetDevice(c->device);
-  wrt_batch_release(c);
   hipStreamSynchronize(c->stream);
+  wrt_batch_release(c);
   hipEventDestroy(c->ev0);
   hipEventDestroy(c->ev1);
   hipEventDestroy(c->ev2);

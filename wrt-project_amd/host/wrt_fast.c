@@ -35,6 +35,9 @@ typedef struct {
   double* acc;      /* [nshift][ndir] */
   float* out;       /* [nshift][ndir] */
   double* staging;  /* n^3 FP64 input */
+  /* host copy of the non-empty bricks, heaviest first (batch planning) */
+  int *ne_brick, *ne_count, n_ne;
+  unsigned char* ne_bnd;
 } radon_fast;
 
 /* ADJOINT in FP32: per-interval FP32 cubics of the FP64 spline, float4 direction table */
@@ -190,6 +193,9 @@ static void radon_fast_destroy(wrt_job* j) {
   hipFree(f->acc);
   hipFree(f->out);
   hipFree(f->staging);
+  free(f->ne_brick);
+  free(f->ne_count);
+  free(f->ne_bnd);
   free(f);
   j->fast = NULL;
 }
@@ -268,13 +274,30 @@ static int radon_fast_load(wrt_job* j, const double* in) {
     while (q >= 0 && flags[order[q]] < flags[v]) { order[q + 1] = order[q]; --q; }
     order[q + 1] = v;
   }
+  free(f->ne_brick);
+  free(f->ne_count);
+  free(f->ne_bnd);
+  f->ne_brick = (int*)malloc(sizeof(int) * (size_t)(nne + 1));
+  f->ne_count = (int*)malloc(sizeof(int) * (size_t)(nne + 1));
+  f->ne_bnd = (unsigned char*)malloc((size_t)(nne + 1));
+  f->n_ne = nne;
+  if (!f->ne_brick || !f->ne_count || !f->ne_bnd) {
+    free(order); free(flags); free(t);
+    return WRT_E_NOMEM;
+  }
+  for (int i = 0; i < nne; ++i) {
+    const int b = order[i];
+    const int bx = b / (f->nb * f->nb), by = (b / f->nb) % f->nb, bz = b % f->nb;
+    f->ne_brick[i] = b;
+    f->ne_count[i] = flags[b];
+    f->ne_bnd[i] = bx == 0 || by == 0 || bz == 0 || (bx + 1) * WRT_BRICK >= d->n ||
+                   (by + 1) * WRT_BRICK >= d->n || (bz + 1) * WRT_BRICK >= d->n;
+  }
   int k = 0;
   for (int pass = 0; pass < 2; ++pass) {
     for (int i = 0; i < nne; ++i) {
       const int b = order[i];
-      const int bx = b / (f->nb * f->nb), by = (b / f->nb) % f->nb, bz = b % f->nb;
-      const int bnd = bx == 0 || by == 0 || bz == 0 || (bx + 1) * WRT_BRICK >= d->n ||
-                      (by + 1) * WRT_BRICK >= d->n || (bz + 1) * WRT_BRICK >= d->n;
+      const int bnd = f->ne_bnd[i];
       if (bnd != pass) continue;
       for (int g = 0; g < ngroups; ++g) {
         t[2 * k] = b;
@@ -363,4 +386,52 @@ int wrt_fast_fetch(wrt_job* j, double* out) {
   if (d->op == WRT_OP_RADON)
     return fetch_f32(j, ((radon_fast*)j->fast)->out, (size_t)d->nshift * d->nphi * d->ntheta, out);
   return fetch_f32(j, ((adjoint_fast*)j->fast)->out, (size_t)d->n * d->n * d->n, out);
+}
+
+/* a loaded fast job as an item of the persistent batch launch (ra.dgroup left to the
+ * planner), with the work the planner splits into tasks */
+int wrt_fast_describe(wrt_job* j, wrt_fast_desc* fd) {
+  if (!j || !j->fast || !fd) return WRT_E_ARG;
+  const wrt_job_desc* d = &j->d;
+  memset(fd, 0, sizeof(*fd));
+  wrt_batch_item* it = &fd->item;
+  if (d->op == WRT_OP_RADON) {
+    radon_fast* f = (radon_fast*)j->fast;
+    it->kind = WRT_ITEM_RADON;
+    it->pow2 = (d->n & (d->n - 1)) == 0;
+    it->h = d->n / 2.0;
+    it->dc = 2.0 / d->n;
+    it->inv_dc = 1.0 / it->dc;
+    it->ra.bricks = f->bricks;
+    it->ra.n = d->n;
+    it->ra.nb = f->nb;
+    it->ra.bf16 = d->prec == WRT_BF16;
+    it->ra.dirs = j->d_dirs;
+    it->ra.ndir = d->nphi * d->ntheta;
+    it->ra.shift = j->d_shift;
+    it->ra.nshift = d->nshift;
+    it->ra.acc = f->acc;
+    it->ra.rho2 = f->rho2;
+    fd->brick = f->ne_brick;
+    fd->weight = f->ne_count;
+    fd->bnd = f->ne_bnd;
+    fd->nbrick = f->n_ne;
+    fd->acc = f->acc;
+    fd->out = f->out;
+    fd->nout = (long long)d->nshift * d->nphi * d->ntheta;
+  } else {
+    adjoint_fast* f = (adjoint_fast*)j->fast;
+    it->kind = WRT_ITEM_ADJOINT;
+    it->adirs = f->dirs;
+    it->atrig = f->trig;
+    it->acub = f->cub;
+    it->aout = f->out;
+    it->andir = d->nphi * d->ntheta;
+    it->ans = d->nshift;
+    it->an = d->n;
+    it->as0 = f->s0;
+    it->ads = f->ds;
+    fd->nvox = (long long)d->n * d->n * d->n;
+  }
+  return WRT_OK;
 }

@@ -9,6 +9,7 @@
 
 #include "../../include/wrt.h"
 #include "../csrc/wrt_dev.h"
+#include "../csrc/wrt_fast.h"
 
 
 struct wrt_ctx {
@@ -47,8 +48,24 @@ struct wrt_job {
   size_t work_bytes;
   void* fast;      /* fast-path plan (wrt_fast.h) */
   int loaded;
+  unsigned long long gen; /* incremented by every load (batch plans key on it) */
   size_t bytes;
 };
+
+/* a loaded fast job described for the batch planner (host/wrt_fast.c) */
+typedef struct {
+  wrt_batch_item item;
+  const int* brick;           /* radon: non-empty bricks, heaviest first */
+  const int* weight;          /* radon: their non-zero voxel counts */
+  const unsigned char* bnd;   /* radon: 1 if the brick touches a cube face */
+  int nbrick;
+  double* acc;                /* radon: FP64 accumulators and FP32 output */
+  float* out;
+  long long nout;
+  long long nvox;             /* adjoint: n^3 */
+} wrt_fast_desc;
+int wrt_fast_supported(const wrt_job_desc* d);
+int wrt_fast_describe(wrt_job* j, wrt_fast_desc* fd);
 
 /* host-side tables (C library sin/cos, the reference's bits) */
 void wrt_host_radon_dirs(int n, const double* phi, int nphi, const double* theta, int ntheta,

@@ -253,7 +253,10 @@ int wrt_job_load(wrt_job* j, const double* in) {
   } else {
     rc = wrt_fast_load(j, in);
   }
-  if (rc == WRT_OK) j->loaded = 1;
+  if (rc == WRT_OK) {
+    j->loaded = 1;
+    j->gen++;
+  }
   return rc;
 }
 

@@ -80,7 +80,7 @@ ABI_SYMBOLS = [
     "wrt_job_fetch", "wrt_job_destroy", "wrt_batch_run", "wrt_phantom",
     "wrt_phantom_params", "wrt_gaussian", "wrt_radon_count_incube", "wrt_job_launch",
     "wrt_job_poll", "wrt_job_wait", "wrt_serve_describe", "wrt_serve_grids", "wrt_serve_desc",
-    "wrt_serve_input", "wrt_serve_run",
+    "wrt_serve_input", "wrt_serve_run", "wrt_batch_dgroup",
 ]
 
 
@@ -130,6 +130,7 @@ def lib():
         L.wrt_phantom_params.argtypes = [I, C.c_ulonglong, P]
         L.wrt_gaussian.argtypes = [C.c_size_t, C.c_ulonglong, P]
         L.wrt_radon_count_incube.argtypes = [P, I, P, I, P, I, P, I, P]
+        L.wrt_batch_dgroup.argtypes = [P]
         L.wrt_serve_describe.argtypes = [P, I, P]
         L.wrt_serve_grids.argtypes = [P, I, P, P, P, P, P]
         L.wrt_serve_desc.argtypes = [P, P, P, P, P, P, P]
@@ -338,6 +339,9 @@ class Context:
         st = Stats()
         _chk(lib().wrt_batch_run(self.h, arr, len(jobs), C.byref(st)), "wrt_batch_run")
         return st.as_dict()
+
+    def batch_dgroup(self):
+        return lib().wrt_batch_dgroup(self.h)
 
 
 def make_desc(op, prec="f64", n=0, phi=None, theta=None, theta_w=None, shift=None, z=None,
