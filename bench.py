@@ -15,6 +15,13 @@ FP32 / 20% reduction FP64, edges {64,96,128,192,256}), and the throughput of the
 requests of that stream arriving at once, served (overlapping, resident data) and in the
 plain per-request FP64 form (one at a time).
 
+The "batch" object is the batched form: 48 seeded volumes with edges drawn from
+{32,48,64,96,128,160,192}, each with a forward pass (FP32 bricks, (e/2) phi x (e/4) GL theta
+x (e+1) shifts) and a backprojection (FP32) of a seeded Gaussian sinogram on the same grid,
+all in one persistent graph-captured launch (direction groups autotuned on the first call);
+the replay's time is compared with the same 96 jobs launched one by one, and with their
+plain FP64 forms.
+
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
 import argparse
@@ -63,6 +70,41 @@ def cpu_baseline(vol, phi, th, s, seconds=15.0):
             "sample": f"{m} seeded uniform planes of the {n}^3 grid, {dt:.1f} s"}
 
 
+def batch_leg(ctx, wrt, nvol, seed=3):
+    rng = np.random.default_rng(seed)
+    edges = [int(e) for e in rng.choice([32, 48, 64, 96, 128, 160, 192], size=nvol)]
+    specs = []
+    for i, e in enumerate(edges):
+        phi, th, w, s = ctx.radon_grid_init(e // 2, e // 4, e + 1)
+        specs.append(("radon", e, dict(n=e, phi=phi, theta=th, shift=s),
+                      lambda e=e, i=i: wrt.phantom(e, True, 10, 1000 + i, 0.0)))
+        specs.append(("adjoint", e, dict(n=e, phi=phi, theta=th, theta_w=w, shift=s),
+                      lambda e=e, i=i, n=len(s) * len(phi) * len(th): wrt.gaussian(n, 2000 + i)))
+    jobs = []
+    for op, e, kw, gen in specs:
+        j = ctx.job(op, "f32", **kw)
+        j.load(gen())
+        jobs.append(j)
+    st0 = ctx.batch_run(jobs)  # plan, autotune, capture
+    st = ctx.batch_run(jobs)   # graph replay
+    per_job = sum(j.run()["seconds"] for j in jobs)
+    for j in jobs:
+        j.close()
+    plain = 0.0
+    for op, e, kw, gen in specs:
+        j = ctx.job(op, "f64", **kw)
+        j.load(gen())
+        plain += j.run()["seconds"]
+        j.close()
+    return {"volumes": nvol, "jobs": len(jobs), "edges": edges, "dgroup": ctx.batch_dgroup(),
+            "first_call_s": st0["kernel_seconds"], "graph_s": st["kernel_seconds"],
+            "samples": st["logical_samples"],
+            "samples_per_s": st["logical_samples"] / st["kernel_seconds"],
+            "per_job_fast_s": per_job, "plain_f64_s": plain,
+            "speedup_vs_per_job": per_job / st["kernel_seconds"],
+            "speedup_vs_plain": plain / st["kernel_seconds"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,6 +122,8 @@ def main():
     ap.add_argument("--serve-qps", type=float, default=8.0)
     ap.add_argument("--serve-requests", type=int, default=64)
     ap.add_argument("--serve-burst", type=int, default=32)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-volumes", type=int, default=48)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -155,6 +199,12 @@ def main():
                  "speedup_vs_plain": brep["samples_per_s"] / prep["samples_per_s"]}
         log(f"bench: serve p50 {rep['p50_ms']:.1f} ms p99 {rep['p99_ms']:.1f} ms, burst "
             f"{serve['speedup_vs_plain']:.2f}x plain")
+    batch = None
+    if not args.no_batch:
+        log("bench: batch")
+        batch = batch_leg(ctx, wrt, args.batch_volumes)
+        log(f"bench: batch {batch['graph_s']:.3f} s, {batch['speedup_vs_per_job']:.2f}x per-job, "
+            f"{batch['speedup_vs_plain']:.2f}x plain")
     ctx.close()
 
     out = {
@@ -186,6 +236,7 @@ def main():
                      "kernel_seconds": kavg},
         "cpu_baseline": cpu,
         "serve": serve,
+        "batch": batch,
     }
     print(json.dumps(out), flush=True)
 

@@ -187,10 +187,10 @@ void wrt_job_destroy(wrt_job* job);
 /* Run many jobs (forward and adjoint, uneven sizes) as one submission (no reference
  * counterpart).  The FP32/BF16 plane-integral jobs and the FP32 backprojection jobs run as
  * one persistent launch captured in a HIP graph: their tasks (brick x direction group,
- * 1024-voxel blocks) are ordered longest first and taken by the CUs from a device counter.
+ * 2048-voxel blocks) are ordered longest first and taken by the CUs from a device counter.
  * The direction-group size is autotuned on the first call for a list shape (cached in
  * memory and in the file WRT_TUNE_CACHE or $HOME/.cache/wrt_batch_tune.txt; WRT_AUTOTUNE=0
- * disables tuning).  The graph is replayed while the list and the jobs' loads stay the same;
+ * disables tuning, WRT_BATCH_DGROUP=G >= 16 fixes the size).  The graph is replayed while the list and the jobs' loads stay the same;
  * any other job (plain FP64 forms, reductions) runs on its own stream beside the graph.
  * Each job's output then equals its wrt_job_run output up to the summation order of the
  * FP64 plane accumulators (relative L2 <= 1e-6).  st: kernel_seconds = the graph's device

@@ -15,43 +15,105 @@
 #define WRT_ADJOINT_VOXEL_HPP
 #include "wrt_common.hpp"
 
-// voxel k of the [z][x][y] output; dirs / trig / cub as in wrt_batch_item
-__device__ __forceinline__ float adjoint_voxel_f32(const float4* __restrict__ dirs,
+// V voxels per thread, k[0..V-1] of the [z][x][y] output (valid[v] = 0 skips one: its
+// result is unspecified and must not be stored); dirs / trig / cub as in wrt_batch_item.
+// Directions go in fours and every voxel's four cubic gathers are issued before any is
+// used, so a thread keeps 4V table loads in flight; each voxel's sum still runs in
+// direction order, so the result of a voxel does not depend on V.
+template <int V>
+__device__ __forceinline__ void adjoint_voxels_f32(const float4* __restrict__ dirs,
                                                    const double4* __restrict__ trig, int ndir,
                                                    const float4* __restrict__ cub, int ns,
                                                    float s0, float inv_ds, float ds, int n,
-                                                   long long k) {
-  const int iy = (int)(k % n);
-  const int ix = (int)((k / n) % n);
-  const int iz = (int)(k / ((long long)n * n));
-  const double xd = (-1.0) + (1.0 / n) + ix * (2.0 / n);
-  const double yd = (-1.0) + (1.0 / n) + iy * (2.0 / n);
-  const double zd = (-1.0) + (1.0 / n) + iz * (2.0 / n);
-  const float x = (float)xd, y = (float)yd, z = (float)zd;
+                                                   const long long* k, float* out) {
+  double xd[V], yd[V], zd[V];
+  float x[V], y[V], z[V], acc[V], comp[V];
+#pragma unroll
+  for (int v = 0; v < V; ++v) {
+    const int iy = (int)(k[v] % n);
+    const int ix = (int)((k[v] / n) % n);
+    const int iz = (int)(k[v] / ((long long)n * n));
+    xd[v] = (-1.0) + (1.0 / n) + ix * (2.0 / n);
+    yd[v] = (-1.0) + (1.0 / n) + iy * (2.0 / n);
+    zd[v] = (-1.0) + (1.0 / n) + iz * (2.0 / n);
+    x[v] = (float)xd[v];
+    y[v] = (float)yd[v];
+    z[v] = (float)zd[v];
+    acc[v] = 0.f;
+    comp[v] = 0.f;
+  }
   const int last = ns - 2;
-  float acc = 0.f, comp = 0.f;
-  for (int d = 0; d < ndir; ++d) {
-    const float4 D = dirs[d];  // uniform across the wavefront
-    const float s = fmaf(x, D.x, fmaf(y, D.y, z * D.z));
-    bool in = fabsf(s) <= 1.0f;
-    if (fabsf(fabsf(s) - 1.0f) <= 1e-5f) {
-      const double4 T = trig[d];
-      const double sd = xd * T.x * T.y + yd * T.x * T.z + zd * T.w;
-      in = !(fabs(sd) > 1.0);
-    }
-    if (in) {
-      int i = (int)((s - s0) * inv_ds);
-      i = i < 0 ? 0 : (i > last ? last : i);
-      const float t = s - fmaf((float)i, ds, s0);
-      const float4 C = cub[(long long)d * (ns - 1) + i];
-      const float v = fmaf(fmaf(fmaf(C.w, t, C.z), t, C.y), t, C.x);
-      const float yk = fmaf(D.w, v, -comp);
-      const float tk = acc + yk;
-      comp = (tk - acc) - yk;
-      acc = tk;
+  int d = 0;
+  for (; d + 4 <= ndir; d += 4) {
+    float4 D[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) D[u] = dirs[d + u];  // uniform across the wavefront
+    float sv[V][4];
+    int iv[V][4];
+    bool inv[V][4];
+    float4 Cv[V][4];
+#pragma unroll
+    for (int v = 0; v < V; ++v)
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const float s = fmaf(x[v], D[u].x, fmaf(y[v], D[u].y, z[v] * D[u].z));
+        bool in = fabsf(s) <= 1.0f;
+        if (fabsf(fabsf(s) - 1.0f) <= 1e-5f) {
+          const double4 T = trig[d + u];
+          const double sd = xd[v] * T.x * T.y + yd[v] * T.x * T.z + zd[v] * T.w;
+          in = !(fabs(sd) > 1.0);
+        }
+        int i = (int)((s - s0) * inv_ds);
+        i = i < 0 ? 0 : (i > last ? last : i);  // a valid interval even where s is cut
+        sv[v][u] = s;
+        iv[v][u] = i;
+        inv[v][u] = in;
+      }
+#pragma unroll
+    for (int v = 0; v < V; ++v)
+#pragma unroll
+      for (int u = 0; u < 4; ++u) Cv[v][u] = cub[(long long)(d + u) * (ns - 1) + iv[v][u]];
+#pragma unroll
+    for (int v = 0; v < V; ++v)
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (inv[v][u]) {
+          const float t = sv[v][u] - fmaf((float)iv[v][u], ds, s0);
+          const float4 C = Cv[v][u];
+          const float val = fmaf(fmaf(fmaf(C.w, t, C.z), t, C.y), t, C.x);
+          const float yk = fmaf(D[u].w, val, -comp[v]);
+          const float tk = acc[v] + yk;
+          comp[v] = (tk - acc[v]) - yk;
+          acc[v] = tk;
+        }
+      }
+  }
+  for (; d < ndir; ++d) {
+    const float4 D = dirs[d];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      const float s = fmaf(x[v], D.x, fmaf(y[v], D.y, z[v] * D.z));
+      bool in = fabsf(s) <= 1.0f;
+      if (fabsf(fabsf(s) - 1.0f) <= 1e-5f) {
+        const double4 T = trig[d];
+        const double sd = xd[v] * T.x * T.y + yd[v] * T.x * T.z + zd[v] * T.w;
+        in = !(fabs(sd) > 1.0);
+      }
+      if (in) {
+        int i = (int)((s - s0) * inv_ds);
+        i = i < 0 ? 0 : (i > last ? last : i);
+        const float t = s - fmaf((float)i, ds, s0);
+        const float4 C = cub[(long long)d * (ns - 1) + i];
+        const float val = fmaf(fmaf(fmaf(C.w, t, C.z), t, C.y), t, C.x);
+        const float yk = fmaf(D.w, val, -comp[v]);
+        const float tk = acc[v] + yk;
+        comp[v] = (tk - acc[v]) - yk;
+        acc[v] = tk;
+      }
     }
   }
-  return acc;
+#pragma unroll
+  for (int v = 0; v < V; ++v) out[v] = acc[v];
 }
 
 #endif

@@ -33,22 +33,29 @@ extern "C" int wrtk_adjoint_cubics(const double* x, int ns, const double* tab, v
   return WRT_OK;
 }
 
-// one thread per voxel of the [z][x][y] output
+// two voxels per thread (k and k + half of the block's span) of the [z][x][y] output
 __global__ __launch_bounds__(256) void k_adjoint_f32(const float4* __restrict__ dirs,
                                                      const double4* __restrict__ trig, int ndir,
                                                      const float4* __restrict__ cub, int ns,
                                                      float s0, float inv_ds, float ds, int n,
                                                      float* __restrict__ out) {
-  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= (long long)n * n * n) return;
-  out[k] = adjoint_voxel_f32(dirs, trig, ndir, cub, ns, s0, inv_ds, ds, n, k);
+  const long long total = (long long)n * n * n;
+  const long long k0 = (long long)blockIdx.x * 512 + threadIdx.x;
+  long long k[2] = {k0, k0 + 256};
+  if (k[0] >= total) return;
+  const bool second = k[1] < total;
+  if (!second) k[1] = k[0];  // computed twice, stored once
+  float r[2];
+  adjoint_voxels_f32<2>(dirs, trig, ndir, cub, ns, s0, inv_ds, ds, n, k, r);
+  out[k[0]] = r[0];
+  if (second) out[k[1]] = r[1];
 }
 
 extern "C" int wrtk_adjoint_f32(const void* dirs, const void* trig, int ndir, const void* cub,
                                 int ns, double s0, double ds, int n, float* out, hipStream_t st) {
   const long long total = (long long)n * n * n;
   if (total <= 0) return WRT_OK;
-  k_adjoint_f32<<<dim3((unsigned)((total + 255) / 256)), 256, 0, st>>>(
+  k_adjoint_f32<<<dim3((unsigned)((total + 511) / 512)), 256, 0, st>>>(
       (const float4*)dirs, (const double4*)trig, ndir, (const float4*)cub, ns, (float)s0, (float)(1.0 / ds), (float)ds,
       n, out);
   WRT_LAUNCH_CHECK();

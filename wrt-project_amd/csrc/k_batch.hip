@@ -6,8 +6,8 @@
 // A workgroup takes the next task from a device counter, runs it, and takes another until
 // the list is exhausted, so uneven tasks balance across the CUs; the host orders the list
 // longest first.  A plane-integral task is the brick task of the per-job launch
-// (radon_brick.hpp) with the job's arguments; a backprojection task is up to 1024 voxels of
-// one job, one per thread (adjoint_voxel.hpp).  Results are the per-job launches' up to the
+// (radon_brick.hpp) with the job's arguments; a backprojection task is up to 2048 voxels of
+// one job, two per thread (adjoint_voxel.hpp).  Results are the per-job launches' up to the
 // order of the FP64 accumulator atomics.
 #include "adjoint_voxel.hpp"
 #include "radon_brick.hpp"
@@ -63,11 +63,17 @@ __global__ __launch_bounds__(FR_WAVES * 64) void k_batch(const wrt_batch_item* _
         else radon_item<false, false>(it, t, L);
       }
     } else {
+      // voxels a + tid and a + 1024 + tid of the task's (up to 2048) voxels
       if ((int)threadIdx.x < t.b) {
-        const long long k = (long long)t.a + threadIdx.x;
-        it.aout[k] = adjoint_voxel_f32((const float4*)it.adirs, (const double4*)it.atrig, it.andir,
-                                       (const float4*)it.acub, it.ans, (float)it.as0,
-                                       (float)(1.0 / it.ads), (float)it.ads, it.an, k);
+        long long k[2] = {(long long)t.a + threadIdx.x, (long long)t.a + 1024 + threadIdx.x};
+        const bool second = (int)threadIdx.x + 1024 < t.b;
+        if (!second) k[1] = k[0];  // computed twice, stored once
+        float r[2];
+        adjoint_voxels_f32<2>((const float4*)it.adirs, (const double4*)it.atrig, it.andir,
+                              (const float4*)it.acub, it.ans, (float)it.as0,
+                              (float)(1.0 / it.ads), (float)it.ads, it.an, k, r);
+        it.aout[k[0]] = r[0];
+        if (second) it.aout[k[1]] = r[1];
       }
     }
   }

@@ -61,7 +61,7 @@ typedef struct {
 } wrt_batch_item;
 
 /* radon: (item, brick, direction group, 1 if the brick touches a cube face);
- * adjoint: (item, first voxel, voxel count <= 1024, 0) */
+ * adjoint: (item, first voxel, voxel count <= 2048, 0) */
 typedef struct {
   int item, a, b, c;
 } wrt_batch_task;

@@ -7,7 +7,7 @@
  * generation stay the same):
  *   items    one per batched job: its arguments (wrt_fast_describe);
  *   tasks    plane integrals: (job, non-empty brick, direction group of G directions);
- *            backprojections: (job, 1024 consecutive voxels);
+ *            backprojections: (job, 2048 consecutive voxels);
  *            ordered by an estimate of their device time, longest first, so that the
  *            persistent workgroups, which take tasks from a counter, finish together;
  *   segments the FP64 accumulators the prologue zeroes and the epilogue rounds to FP32;
@@ -15,7 +15,7 @@
  * The direction-group size G is the tunable: the autotuner times the batch at each
  * candidate on the first call for a list shape and keeps the fastest, in memory and in a
  * small text cache file (WRT_TUNE_CACHE, else $HOME/.cache/wrt_batch_tune.txt; set
- * WRT_AUTOTUNE=0 to skip tuning and use G = 64).
+ * WRT_AUTOTUNE=0 to skip tuning and use G = 64, or WRT_BATCH_DGROUP=G (>= 16) to fix G).
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -69,9 +69,10 @@ void wrt_batch_release(wrt_ctx* ctx) {
   ctx->batch = NULL;
 }
 
+/* tasks address voxels with 32-bit offsets: larger backprojections run beside the graph */
 static int batchable(const wrt_job* j) {
   return j->d.prec != WRT_F64 && wrt_fast_supported(&j->d) &&
-         (j->d.op == WRT_OP_RADON || j->d.op == WRT_OP_ADJOINT);
+         (j->d.op == WRT_OP_RADON || (j->d.op == WRT_OP_ADJOINT && j->d.n <= 1024));
 }
 
 typedef struct {
@@ -92,7 +93,7 @@ static int build_tasks(const wrt_fast_desc* fd, int nitems, int G, wrt_batch_tas
       const int ndir = fd[i].item.ra.ndir;
       total += (long long)fd[i].nbrick * ((ndir + G - 1) / G);
     } else {
-      total += (fd[i].nvox + 1023) / 1024;
+      total += (fd[i].nvox + 2047) / 2048;
     }
   }
   if (total > 0x7fffffff) return WRT_E_ARG;
@@ -115,8 +116,8 @@ static int build_tasks(const wrt_fast_desc* fd, int nitems, int G, wrt_batch_tas
           ++k;
         }
     } else {
-      for (long long v = 0; v < fd[i].nvox; v += 1024) {
-        const int cnt = (int)(fd[i].nvox - v < 1024 ? fd[i].nvox - v : 1024);
+      for (long long v = 0; v < fd[i].nvox; v += 2048) {
+        const int cnt = (int)(fd[i].nvox - v < 2048 ? fd[i].nvox - v : 2048);
         ct[k].cost = COST_VOXEL_DIR * cnt * it->andir;
         ct[k].t.item = i;
         ct[k].t.a = (int)v;
@@ -292,7 +293,11 @@ static int plan_build(wrt_ctx* c, wrt_job* const* jobs, int njobs, batch_plan** 
 
   /* direction-group size: cached, tuned, or the default */
   int G = 64;
-  if (rc == WRT_OK && nitems > 0) {
+  const char* force = getenv("WRT_BATCH_DGROUP");
+  const int forced = force ? atoi(force) : 0;
+  if (rc == WRT_OK && nitems > 0 && forced >= 16) {
+    G = forced;
+  } else if (rc == WRT_OK && nitems > 0) {
     const unsigned long long sig = signature(fd, nitems);
     const char* at = getenv("WRT_AUTOTUNE");
     const int tune = !(at && at[0] == '0');
