@@ -143,3 +143,31 @@ def test_fast_f32_structured_directions_on_faces(ctx, n):
     scale = np.max(np.abs(ref))
     assert orc.rel_l2(got, ref) <= 1e-5, orc.rel_l2(got, ref)
     assert np.max(np.abs(got - ref)) <= 1e-5 * scale, np.max(np.abs(got - ref)) / scale
+
+
+@pytest.mark.parametrize("n", [33, 64, 128, 200])
+def test_fast_ray_f32_matches_plain(ctx, n):
+    # node-centred phantom (n = 33: power-of-two spacing; 200: the plane does not fit LDS),
+    # phi grid with 0 and pi/2 (the trilinear index quirk's rays), slices outside [-1, 1]
+    vol = wrt.phantom(n, False, 10, n, 0.01)
+    s, phi = wrt.ray_grid_init(n + 1, 16)
+    z = np.concatenate([wrt.zslice_grid(24), [1.5, -1.0000001]])
+    ref, _ = ctx.ray_grid(vol, z, s, phi, prec="f64")
+    got, st = ctx.ray_grid(vol, z, s, phi, prec="f32")
+    assert orc.rel_l2(got, ref) <= 1e-5, orc.rel_l2(got, ref)
+    assert np.max(np.abs(got - ref)) <= 1e-5 * np.max(np.abs(ref))
+    assert np.all(got[-2:] == 0.0)
+    assert st["logical_samples"] == (4.0 * n + 1) * got.size
+
+
+def test_fast_ray_f32_random_volume_and_angles(ctx):
+    rng = np.random.default_rng(9)
+    n = 48
+    vol = rng.standard_normal((n, n, n))
+    phi = np.concatenate([rng.uniform(0, np.pi, 9), [0.0, np.pi / 2, np.pi / 4]])
+    s = np.sort(rng.uniform(-1.2, 1.2, 13))
+    z = rng.uniform(-1, 1, 7)
+    ref, _ = ctx.ray_grid(vol, z, s, phi, prec="f64")
+    got, _ = ctx.ray_grid(vol, z, s, phi, prec="f32")
+    assert orc.rel_l2(got, ref) <= 1e-5, orc.rel_l2(got, ref)
+    assert np.max(np.abs(got - ref)) <= 1e-5 * np.max(np.abs(ref))

@@ -1,0 +1,179 @@
+// k_ray_fast.hip -- FP32 form of the layer-by-layer line integrals (ray_grid).
+//
+// ray_transform (ray_grid/src/integral.c:11-50) sums cube_trilinear_interp
+// (ray_grid/src/interp.c:25-60) at the 4n+1 points p_i = (s cos phi, s sin phi, z) +
+// (delta dir) i of a ray in the slice z, delta = (2/(n-1))/2.  Every index and fraction
+// decision is the reference's, taken in binary64 with its rounding:
+//   a = x + 1,  ix = (int)(a / dg),  cx = fmod(a, dg) / dg,  ix == n-1 -> (n-2, cx = 1),
+// including the case where a/dg rounds up to an integer k while the exact quotient is below
+// it: ix = k and fmod gives nearly dg, so the point takes nearly node k+1 (the reference's
+// quirk).  Here the quotient is one corrected-reciprocal division (or an exact power-of-two
+// scaling), the exact remainder is a fused multiply-add with the truncated quotient, and the
+// z decision is the slice's, made once on the host with the C library's fmod.
+//
+// Work layout: one workgroup of 512 threads per (slice, 512 rays); a thread walks one ray.
+// The points with |x| > 1 or |y| > 1 contribute 0 in the reference; x and y are monotone in
+// i, so each ray's range of i with |x|, |y| <= 1 is found exactly at its two ends and only
+// that range is walked.  For n <= 181 the workgroup stages the slice's plane interpolated in
+// z, P = (1 - cz) V[iz] + cz V[iz+1] (FP32), in LDS; every point is then a bilinear
+// interpolation of P from LDS.  Larger volumes read both planes from the FP32 [z][x][y] copy
+// in HBM.  Values and weights are FP32 (lerps reordered: z first), the sum along a ray is
+// Kahan-compensated in FP32, times delta in FP64 at the end.
+#include "radon_brick.hpp"  // cube_ends, div_rn
+#include "wrt_dev.h"
+
+#define RAY_THREADS 512
+#define RAY_LDS_N 181  // largest n whose n^2 FP32 plane fits the 128 KiB stage
+
+// one axis: the reference's index and fraction of coordinate x (|x| <= 1)
+template <bool POW2>
+__device__ __forceinline__ void ray_axis(double x, int n, double dg, double inv_dg, double h,
+                                         int& i, float& f) {
+  const double a = x + 1.0;
+  // RN(a / dg): h = (n-1)/2 a power of two scales exactly; otherwise the corrected reciprocal
+  const double q = POW2 ? a * h : div_rn(a, dg, inv_dg);
+  const double qt = trunc(q);
+  double r = fma(-qt, dg, a);  // exact: a - qt*dg is representable
+  if (r < 0.0) r = fma(-(qt - 1.0), dg, a);  // a/dg rounded up to an integer
+  i = (int)qt;
+  f = (float)(r * inv_dg);
+  if (i == n - 1) {
+    i = n - 2;
+    f = 1.0f;
+  }
+}
+
+__device__ __forceinline__ float lerpf(float t, float a, float b) { return fmaf(t, b - a, a); }
+
+// vol32: [z][x][y] FP32; zi / zc: the slices' index and fraction (reference rounding;
+// zi < 0 marks a slice with |z| > 1, whose rays are 0);
+// trig: [nphi][4] cos p, sin p, cos(p+pi/2), sin(p+pi/2) (host libm); out [z][s][phi] FP32
+template <bool LDS, bool POW2>
+__global__ __launch_bounds__(RAY_THREADS) void k_ray_f32(const float* __restrict__ vol32, int n,
+                                                         const int* __restrict__ zi,
+                                                         const double* __restrict__ zc,
+                                                         const double* __restrict__ shift,
+                                                         int nshift,
+                                                         const double* __restrict__ trig,
+                                                         int nphi, float* __restrict__ out) {
+  extern __shared__ float plane[];
+  const int iz = blockIdx.y;
+  const int kz = zi[iz];
+  const long long nrays = (long long)nshift * nphi;
+  const long long t = (long long)blockIdx.x * RAY_THREADS + threadIdx.x;
+  if (kz < 0) {  // |z| > 1: every point fails the cube test
+    if (t < nrays) out[((size_t)iz * nshift + (int)(t % nshift)) * nphi + (int)(t / nshift)] = 0.f;
+    return;
+  }
+  const float cz = (float)zc[iz];
+  const size_t nn = (size_t)n * n;
+  const float* __restrict__ v0 = vol32 + (size_t)kz * nn;
+  const float* __restrict__ v1 = v0 + nn;
+  if (LDS) {
+    for (int k = threadIdx.x; k < (int)nn; k += RAY_THREADS) plane[k] = lerpf(cz, v0[k], v1[k]);
+    __syncthreads();
+  }
+  if (t >= nrays) return;
+  const int is = (int)(t % nshift), ip = (int)(t / nshift);
+  const double c = trig[4 * ip], sn = trig[4 * ip + 1], dx = trig[4 * ip + 2],
+               dy = trig[4 * ip + 3];
+  const double s = shift[is];
+  const double px = s * c, py = s * sn;
+  const double delta = (2.0 / (n - 1)) / (2.0);
+  const double dg = 2.0 / (n - 1), inv_dg = 1.0 / dg, h = (n - 1) / 2.0;
+  const double sx = delta * dx, sy = delta * dy;
+  // the exact range of i with |x| <= 1 and |y| <= 1
+  int lo = -2 * n, hi = 2 * n;
+  {
+    const double tl = fmax(fabs(sx) > 0 ? fmin((-1.0 - px) / sx, (1.0 - px) / sx) : -1e300,
+                           fabs(sy) > 0 ? fmin((-1.0 - py) / sy, (1.0 - py) / sy) : -1e300);
+    const double th = fmin(fabs(sx) > 0 ? fmax((-1.0 - px) / sx, (1.0 - px) / sx) : 1e300,
+                           fabs(sy) > 0 ? fmax((-1.0 - py) / sy, (1.0 - py) / sy) : 1e300);
+    if (tl > -1e299) lo = max(lo, (int)fmax(ceil(tl) - 1.0, -4.0 * n));
+    if (th < 1e299) hi = min(hi, (int)fmin(floor(th) + 1.0, 4.0 * n));
+    cube_ends(px, sx, true, true, lo, hi);
+    cube_ends(py, sy, true, true, lo, hi);
+  }
+  float acc = 0.f, comp = 0.f;
+  for (int i = lo; i <= hi; ++i) {
+    const double fi = (double)i;
+    const double x = px + sx * fi, y = py + sy * fi;
+    int ix, iy;
+    float fx, fy;
+    ray_axis<POW2>(x, n, dg, inv_dg, h, ix, fx);
+    ray_axis<POW2>(y, n, dg, inv_dg, h, iy, fy);
+    float p00, p10, p01, p11;
+    const int o = ix * n + iy;
+    if (LDS) {
+      p00 = plane[o];
+      p10 = plane[o + n];
+      p01 = plane[o + 1];
+      p11 = plane[o + n + 1];
+    } else {
+      p00 = lerpf(cz, v0[o], v1[o]);
+      p10 = lerpf(cz, v0[o + n], v1[o + n]);
+      p01 = lerpf(cz, v0[o + 1], v1[o + 1]);
+      p11 = lerpf(cz, v0[o + n + 1], v1[o + n + 1]);
+    }
+    const float v = lerpf(fy, lerpf(fx, p00, p10), lerpf(fx, p01, p11));
+    const float yk = v - comp;
+    const float tk = acc + yk;
+    comp = (tk - acc) - yk;
+    acc = tk;
+  }
+  out[((size_t)iz * nshift + is) * nphi + ip] = (float)(((double)acc - (double)comp) * delta);
+}
+
+template <bool LDS>
+static void launch_ray(const float* vol32, int n, const int* zi, const double* zc, int nz,
+                       const double* shift, int nshift, const double* trig, int nphi, float* out,
+                       hipStream_t st) {
+  const long long nrays = (long long)nshift * nphi;
+  const dim3 grid((unsigned)((nrays + RAY_THREADS - 1) / RAY_THREADS), (unsigned)nz);
+  const size_t lds = LDS ? (size_t)n * n * sizeof(float) : 0;
+  const int m = n - 1;
+  if ((m & (m - 1)) == 0) {
+    if (lds > 65536)
+      (void)hipFuncSetAttribute((const void*)k_ray_f32<LDS, true>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    k_ray_f32<LDS, true><<<grid, RAY_THREADS, lds, st>>>(vol32, n, zi, zc, shift, nshift, trig, nphi, out);
+  } else {
+    if (lds > 65536)
+      (void)hipFuncSetAttribute((const void*)k_ray_f32<LDS, false>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    k_ray_f32<LDS, false><<<grid, RAY_THREADS, lds, st>>>(vol32, n, zi, zc, shift, nshift, trig, nphi, out);
+  }
+}
+
+extern "C" int wrtk_ray_f32(const float* vol32, int n, const int* zi, const double* zc, int nz,
+                            const double* shift, int nshift, const double* trig, int nphi,
+                            float* out, hipStream_t st) {
+  if (n < 2 || nz < 1 || nz > 65535 || nshift < 1 || nphi < 1) return WRT_E_ARG;
+  if ((long long)nshift * nphi > 65535LL * RAY_THREADS * 64) return WRT_E_ARG;
+  if (n <= RAY_LDS_N)
+    launch_ray<true>(vol32, n, zi, zc, nz, shift, nshift, trig, nphi, out, st);
+  else
+    launch_ray<false>(vol32, n, zi, zc, nz, shift, nshift, trig, nphi, out, st);
+  WRT_LAUNCH_CHECK();
+  return WRT_OK;
+}
+
+// FP64 vol[x][y][z] -> FP32 [z][x][y] (each z-plane contiguous)
+__global__ __launch_bounds__(256) void k_vol_zxy_f32(const double* __restrict__ vol, int n,
+                                                     float* __restrict__ out) {
+  const size_t total = (size_t)n * n * n;
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < total;
+       k += (size_t)gridDim.x * blockDim.x) {
+    const size_t z = k / ((size_t)n * n), r = k % ((size_t)n * n);  // r = x*n + y
+    out[k] = (float)vol[r * n + z];
+  }
+}
+
+extern "C" int wrtk_vol_zxy_f32(const double* vol, int n, float* out, hipStream_t st) {
+  const size_t total = (size_t)n * n * n;
+  size_t blocks = (total + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  k_vol_zxy_f32<<<(unsigned)blocks, 256, 0, st>>>(vol, n, out);
+  WRT_LAUNCH_CHECK();
+  return WRT_OK;
+}

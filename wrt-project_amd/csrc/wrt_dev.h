@@ -65,6 +65,13 @@ int wrtk_adjoint_cubics(const double* x, int ns, const double* tab, void* cub, l
 int wrtk_adjoint_f32(const void* dirs, const void* trig, int ndir, const void* cub, int ns,
                      double s0, double ds, int n, float* out, hipStream_t st);
 
+/* FP32 rays: vol32 is the FP32 volume in [z][x][y] order; zi / zc the slices' index and
+ * fraction as the reference decides them (zi < 0: |z| > 1); out [z][s][phi] FP32 */
+int wrtk_vol_zxy_f32(const double* vol, int n, float* out, hipStream_t st);
+int wrtk_ray_f32(const float* vol32, int n, const int* zi, const double* zc, int nz,
+                 const double* shift, int nshift, const double* trig, int nphi, float* out,
+                 hipStream_t st);
+
 /* number of lattice samples passing zero_interp's cube test, summed over all planes */
 int wrtk_radon_incube(int n, const double* dirs, int ndir, const double* shift, int nshift,
                       unsigned long long* total, hipStream_t st);

@@ -7,6 +7,9 @@
  * separate launches).  The kernel accumulates each plane in FP64; the result is rounded to
  * FP32 on the device, which is the output of the form.
  *
+ * RAY (FP32): the FP64 input is converted on the device to FP32 in [z][x][y] order; each
+ * slice's z index and fraction are decided on the host as the reference decides them.
+ *
  * ADJOINT (FP32): the Radon rows are uploaded in FP64 and splined in FP64 as in the plain
  * form; each interval's cubic is then rounded to FP32 (y, b, c, d), directions become float4
  * (unit normal, weight), and one thread per voxel sums the directions with Kahan compensation.
@@ -62,8 +65,103 @@ static int uniform_grid(const double* s, int n, double* s0, double* ds) {
   return d > 0;
 }
 
+/* RAY in FP32: FP32 volume in [z][x][y] order, slice decisions made on the host */
+typedef struct {
+  double* staging; /* n^3 FP64 input */
+  float* vol32;    /* n^3 FP32 [z][x][y] */
+  int* zi;         /* [nz] slice index (-1: |z| > 1) */
+  double* zc;      /* [nz] slice fraction */
+  float* out;      /* [nz][ns][nphi] */
+} ray_fast;
+
+static size_t ray_fast_bytes(const wrt_job_desc* d) {
+  const size_t n3 = (size_t)d->n * d->n * d->n;
+  return n3 * 12 + (size_t)d->nz * 12 + (size_t)d->nz * d->nshift * d->nphi * 4;
+}
+
+static void ray_fast_destroy(wrt_job* j) {
+  ray_fast* f = (ray_fast*)j->fast;
+  hipFree(f->staging);
+  hipFree(f->vol32);
+  hipFree(f->zi);
+  hipFree(f->zc);
+  hipFree(f->out);
+  free(f);
+  j->fast = NULL;
+}
+
+static int ray_fast_create(wrt_job* j) {
+  const wrt_job_desc* d = &j->d;
+  ray_fast* f = (ray_fast*)calloc(1, sizeof(ray_fast));
+  if (!f) return WRT_E_NOMEM;
+  j->fast = f;
+  const size_t n3 = (size_t)d->n * d->n * d->n;
+  if (hipMalloc((void**)&f->staging, n3 * 8) != hipSuccess ||
+      hipMalloc((void**)&f->vol32, n3 * 4) != hipSuccess ||
+      hipMalloc((void**)&f->zi, (size_t)d->nz * sizeof(int)) != hipSuccess ||
+      hipMalloc((void**)&f->zc, (size_t)d->nz * sizeof(double)) != hipSuccess ||
+      hipMalloc((void**)&f->out, (size_t)d->nz * d->nshift * d->nphi * 4) != hipSuccess)
+    return WRT_E_NOMEM;
+  /* the slice's index and fraction exactly as cube_trilinear_interp decides them for the
+   * constant z of every point of the slice (ray_grid/src/interp.c:25-47) */
+  int* zi = (int*)malloc((size_t)d->nz * sizeof(int));
+  double* zc = (double*)malloc((size_t)d->nz * sizeof(double));
+  if (!zi || !zc) {
+    free(zi);
+    free(zc);
+    return WRT_E_NOMEM;
+  }
+  const double dg = 2.0 / (d->n - 1);
+  for (int l = 0; l < d->nz; ++l) {
+    const double z = d->z[l];
+    if (fabs(z) > 1) {
+      zi[l] = -1;
+      zc[l] = 0.0;
+      continue;
+    }
+    int iz = (int)((z + 1.0) / dg);
+    double cz = fmod(z + 1.0, dg) / dg;
+    if (iz == d->n - 1) {
+      iz -= 1;
+      cz = 1.0;
+    }
+    zi[l] = iz;
+    zc[l] = cz;
+  }
+  int rc = WRT_OK;
+  if (hipMemcpyAsync(f->zi, zi, (size_t)d->nz * sizeof(int), hipMemcpyHostToDevice, j->st) != hipSuccess ||
+      hipMemcpyAsync(f->zc, zc, (size_t)d->nz * sizeof(double), hipMemcpyHostToDevice, j->st) != hipSuccess ||
+      hipStreamSynchronize(j->st) != hipSuccess)
+    rc = WRT_E_HIP;
+  free(zi);
+  free(zc);
+  return rc;
+}
+
+static int ray_fast_load(wrt_job* j, const double* in) {
+  ray_fast* f = (ray_fast*)j->fast;
+  const size_t n3 = (size_t)j->d.n * j->d.n * j->d.n;
+  HIPCHK(hipMemcpyAsync(f->staging, in, n3 * 8, hipMemcpyHostToDevice, j->st));
+  const int rc = wrtk_vol_zxy_f32(f->staging, j->d.n, f->vol32, j->st);
+  if (rc != WRT_OK) return rc;
+  HIPCHK(hipStreamSynchronize(j->st));
+  return WRT_OK;
+}
+
+static int ray_fast_enqueue(wrt_job* j) {
+  ray_fast* f = (ray_fast*)j->fast;
+  const wrt_job_desc* d = &j->d;
+  HIPCHK(hipEventRecord(j->e1, j->st));
+  const int rc = wrtk_ray_f32(f->vol32, d->n, f->zi, f->zc, d->nz, j->d_shift, d->nshift,
+                              j->d_dirs, d->nphi, f->out, j->st);
+  if (rc != WRT_OK) return rc;
+  HIPCHK(hipEventRecord(j->e3, j->st));
+  return WRT_OK;
+}
+
 int wrt_fast_supported(const wrt_job_desc* d) {
   if (!d) return 0;
+  if (d->op == WRT_OP_RAY) return d->prec == WRT_F32 && d->nz <= 65535;
   /* the brick kernel packs a row's first and last lattice column (|i2| <= 2n) in 16 bits */
   if (d->op == WRT_OP_RADON) return (d->prec == WRT_F32 || d->prec == WRT_BF16) && d->n <= 4096;
   if (d->op == WRT_OP_ADJOINT) {
@@ -355,11 +453,20 @@ static int radon_fast_enqueue(wrt_job* j) {
 /* dispatch by operation; each job owns one fast state */
 size_t wrt_fast_bytes(const wrt_job_desc* d) {
   if (!wrt_fast_supported(d)) return 0;
-  return d->op == WRT_OP_RADON ? radon_fast_bytes(d) : adjoint_fast_bytes(d);
+  switch (d->op) {
+    case WRT_OP_RADON: return radon_fast_bytes(d);
+    case WRT_OP_RAY: return ray_fast_bytes(d);
+    default: return adjoint_fast_bytes(d);
+  }
 }
 
 int wrt_fast_create(wrt_job* j) {
   if (j->d.op == WRT_OP_RADON) return radon_fast_create(j);
+  if (j->d.op == WRT_OP_RAY) {
+    const int rc = ray_fast_create(j);
+    if (rc != WRT_OK && j->fast) ray_fast_destroy(j);
+    return rc;
+  }
   const int rc = adjoint_fast_create(j);
   if (rc != WRT_OK && j->fast) adjoint_fast_destroy(j);
   return rc;
@@ -369,22 +476,34 @@ void wrt_fast_destroy(wrt_job* j) {
   if (!j->fast) return;
   if (j->d.op == WRT_OP_RADON)
     radon_fast_destroy(j);
+  else if (j->d.op == WRT_OP_RAY)
+    ray_fast_destroy(j);
   else
     adjoint_fast_destroy(j);
 }
 
 int wrt_fast_load(wrt_job* j, const double* in) {
-  return j->d.op == WRT_OP_RADON ? radon_fast_load(j, in) : adjoint_fast_load(j, in);
+  switch (j->d.op) {
+    case WRT_OP_RADON: return radon_fast_load(j, in);
+    case WRT_OP_RAY: return ray_fast_load(j, in);
+    default: return adjoint_fast_load(j, in);
+  }
 }
 
 int wrt_fast_enqueue(wrt_job* j) {
-  return j->d.op == WRT_OP_RADON ? radon_fast_enqueue(j) : adjoint_fast_enqueue(j);
+  switch (j->d.op) {
+    case WRT_OP_RADON: return radon_fast_enqueue(j);
+    case WRT_OP_RAY: return ray_fast_enqueue(j);
+    default: return adjoint_fast_enqueue(j);
+  }
 }
 
 int wrt_fast_fetch(wrt_job* j, double* out) {
   const wrt_job_desc* d = &j->d;
   if (d->op == WRT_OP_RADON)
     return fetch_f32(j, ((radon_fast*)j->fast)->out, (size_t)d->nshift * d->nphi * d->ntheta, out);
+  if (d->op == WRT_OP_RAY)
+    return fetch_f32(j, ((ray_fast*)j->fast)->out, (size_t)d->nz * d->nshift * d->nphi, out);
   return fetch_f32(j, ((adjoint_fast*)j->fast)->out, (size_t)d->n * d->n * d->n, out);
 }
 
