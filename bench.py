@@ -23,6 +23,13 @@ the replay's time is compared with the same 96 jobs launched one by one, and wit
 plain FP64 forms.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
+    python bench.py --traffic-probe     (run under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)
+
+The roofline's "traffic" (HBM bytes per pass) comes from those two counter runs: the probe
+runs a plain conversion kernel of known size (reads 8 B, writes 4 B per element) and then
+one pass of the workload; each counter is scaled by what it reads for the conversion
+kernel.  The committed summary profiles/radon_fast_256_traffic.md holds the result, which
+the bench reports.
 """
 import argparse
 import json
@@ -105,6 +112,51 @@ def batch_leg(ctx, wrt, nvol, seed=3):
             "speedup_vs_plain": plain / st["kernel_seconds"]}
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "radon_fast_256_traffic.md")
+CALIB_COUNT = 1 << 27  # elements of the calibration conversion: 1 GiB read, 512 MiB written
+
+
+def traffic_probe(args):
+    """Calibration kernel of known size, then one pass of the workload (for rocprofv3 --pmc)."""
+    import ctypes as C
+
+    import wrt
+
+    hip = C.CDLL("libamdhip64.so")
+    src, dst = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(src), C.c_size_t(CALIB_COUNT * 8)) == 0
+    assert hip.hipMalloc(C.byref(dst), C.c_size_t(CALIB_COUNT * 4)) == 0
+    L = wrt.lib()
+    L.wrtk_fill_f64.argtypes = [C.c_void_p, C.c_double, C.c_size_t, C.c_void_p]
+    L.wrtk_f64_to_f32.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]
+    assert L.wrtk_fill_f64(src, 1.0, CALIB_COUNT, None) == 0
+    assert L.wrtk_f64_to_f32(src, dst, CALIB_COUNT, None) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    ctx = wrt.Context(0)
+    vol = wrt.phantom(args.n, True, 10, 0, 0.01)
+    phi, th, w, s = ctx.radon_grid_init(args.nphi, args.ntheta, args.nshift)
+    job = ctx.job("radon", args.prec, n=args.n, phi=phi, theta=th, shift=s)
+    job.load(vol)
+    st = job.run()
+    job.close()
+    ctx.close()
+    print(f"traffic probe: calibration {CALIB_COUNT} elements, pass {st['kernel_seconds']:.3f} s",
+          flush=True)
+
+
+def traffic_from_profile():
+    """HBM bytes per pass from the committed counter summary (None if absent)."""
+    try:
+        for line in open(TRAFFIC_FILE):
+            if line.startswith("traffic_bytes_per_pass:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -124,11 +176,16 @@ def main():
     ap.add_argument("--serve-burst", type=int, default=32)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-volumes", type=int, default=48)
+    ap.add_argument("--traffic-probe", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(every box here has one GPU)", file=sys.stderr)
         sys.exit(2)
+
+    if args.traffic_probe:
+        traffic_probe(args)
+        return
 
     import wrt
 
@@ -230,7 +287,7 @@ def main():
                       "subset": f"every {args.plain_phi_stride}th phi ({plain.size} planes)"},
         "speedup_vs_plain": value / plain_rate,
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic_from_profile(),
                      "alg_bytes": alg_bytes,
                      "kernel": "k_radon_fast (interior + face-brick launches)",
                      "kernel_seconds": kavg},
