@@ -171,3 +171,22 @@ def test_fast_ray_f32_random_volume_and_angles(ctx):
     got, _ = ctx.ray_grid(vol, z, s, phi, prec="f32")
     assert orc.rel_l2(got, ref) <= 1e-5, orc.rel_l2(got, ref)
     assert np.max(np.abs(got - ref)) <= 1e-5 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("n", [17, 40])
+def test_fast_ray_f32_rays_on_cube_faces(ctx, n):
+    # Regression: rays lying on a face of the cube (s = +-1 along phi = k pi/2) have a
+    # direction component of ~1e-17, and x = s + sx*i rounds to exactly +-1 for dozens of
+    # points beyond the continuous bound; all of them pass the reference's cube test.  A
+    # volume that is non-zero on the faces makes every such point count.
+    rng = np.random.default_rng(n)
+    vol = rng.standard_normal((n, n, n))
+    s, phi = wrt.ray_grid_init(n + 1, 16)  # s from -1 to 1, phi multiples of pi/8
+    z = wrt.zslice_grid(7)
+    ref, _ = ctx.ray_grid(vol, z, s, phi, prec="f64")
+    got, _ = ctx.ray_grid(vol, z, s, phi, prec="f32")
+    assert orc.rel_l2(got, ref) <= 1e-5, orc.rel_l2(got, ref)
+    assert np.max(np.abs(got - ref)) <= 1e-5 * np.max(np.abs(ref))
+    # the face rays themselves
+    for js in (0, len(s) - 1):
+        assert orc.rel_l2(got[:, js, :], ref[:, js, :]) <= 1e-5

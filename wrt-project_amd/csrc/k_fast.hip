@@ -32,6 +32,72 @@ extern "C" int wrtk_radon_fast(const wrt_radon_fast_args* a, int boundary, hipSt
   return a->bf16 ? launch_pow2<false, true>(a, pow2, st) : launch_pow2<false, false>(a, pow2, st);
 }
 
+// ------------------------------------------------- transpose of the plane integrals ---
+// One workgroup per (brick, direction group): the brick task in transpose mode scatters
+// every section's value into the LDS brick, which is then stored whole (coalesced, in the
+// brick's own [x][y][z] order) to the group's slot.  No global atomics: the groups' slots
+// are summed afterwards in FP64 in group order.  (Within a task the LDS additions run in
+// whatever order the waves reach them, so repeated runs agree to FP32 rounding, not bits.)
+template <bool BND, bool POW2>
+__global__ __launch_bounds__(FR_WAVES * 64) void k_radon_T_fast(wrt_radon_fast_args a, double h,
+                                                                double dc, double inv_dc) {
+  __shared__ radon_lds L;
+  const int brick = a.tasks[2 * blockIdx.x], grp = a.tasks[2 * blockIdx.x + 1];
+  radon_brick_task<BND, false, POW2, true>(a, brick, grp, h, dc, inv_dc, L);
+  __syncthreads();
+  const size_t nbr = (size_t)a.nb * a.nb * a.nb;
+  float* __restrict__ dst = a.part + ((size_t)grp * nbr + brick) * (FB * FB * FB);
+  for (int i = threadIdx.x; i < FB * FB * FB; i += blockDim.x) {
+    const int lx = i >> 10, ly = (i >> 5) & 31, lz = i & 31;
+    dst[i] = L.sb[1 + lx * FR_PX + ly * FR_PY + lz];
+  }
+}
+
+template <bool BND>
+static int launch_T(const wrt_radon_fast_args* a, bool pow2, hipStream_t st) {
+  const double dc = 2.0 / a->n, inv = 1.0 / dc, h = a->n / 2.0;
+  if (pow2)
+    k_radon_T_fast<BND, true><<<a->ntasks, FR_WAVES * 64, 0, st>>>(*a, h, dc, inv);
+  else
+    k_radon_T_fast<BND, false><<<a->ntasks, FR_WAVES * 64, 0, st>>>(*a, h, dc, inv);
+  WRT_LAUNCH_CHECK();
+  return WRT_OK;
+}
+
+extern "C" int wrtk_radon_T_fast(const wrt_radon_fast_args* a, int boundary, hipStream_t st) {
+  if (a->ntasks <= 0) return WRT_OK;
+  if (!a->g || !a->part || a->rho2 < 3.0) return WRT_E_ARG;
+  const bool pow2 = (a->n & (a->n - 1)) == 0;
+  return boundary ? launch_T<true>(a, pow2, st) : launch_T<false>(a, pow2, st);
+}
+
+__global__ __launch_bounds__(256) void k_radon_T_reduce(const float* __restrict__ part,
+                                                        int ngroups, int n, int nb,
+                                                        double scale, float* __restrict__ out) {
+  const size_t total = (size_t)n * n * n;
+  const size_t slot = (size_t)nb * nb * nb * (FB * FB * FB);
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < total;
+       k += (size_t)gridDim.x * blockDim.x) {
+    const int z = (int)(k % n), y = (int)((k / n) % n), x = (int)(k / ((size_t)n * n));
+    const size_t brick = ((size_t)(x / FB) * nb + y / FB) * nb + z / FB;
+    const size_t cell = brick * (FB * FB * FB) + ((x % FB) * FB + y % FB) * FB + z % FB;
+    double acc = 0.0;
+    for (int g = 0; g < ngroups; ++g) acc += (double)part[(size_t)g * slot + cell];
+    out[k] = (float)(acc * scale);
+  }
+}
+
+extern "C" int wrtk_radon_T_reduce(const float* part, int ngroups, int n, int nb, double scale,
+                                   float* out, hipStream_t st) {
+  const size_t total = (size_t)n * n * n;
+  size_t blocks = (total + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  if (!blocks || ngroups < 1) return WRT_E_ARG;
+  k_radon_T_reduce<<<(unsigned)blocks, 256, 0, st>>>(part, ngroups, n, nb, scale, out);
+  WRT_LAUNCH_CHECK();
+  return WRT_OK;
+}
+
 // ------------------------------------------------------------------- brick loader ---
 __device__ __forceinline__ unsigned short f32_to_bf16_rne(float f) {
   unsigned u = __float_as_uint(f);

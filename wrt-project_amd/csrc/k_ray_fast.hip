@@ -45,6 +45,29 @@ __device__ __forceinline__ void ray_axis(double x, int n, double dg, double inv_
 
 __device__ __forceinline__ float lerpf(float t, float a, float b) { return fmaf(t, b - a, a); }
 
+// The exact range [lo, hi] of i in [-2n, 2n] whose point x = px + sx*i, y = py + sy*i (each
+// rounded as the reference rounds it) passes |x| <= 1 and |y| <= 1.  x and y are monotone
+// in i, so the range is an interval; the continuous bounds, widened by a tolerance far
+// above the rounding of x and y, contain it, and cube_ends settles its ends exactly.  The
+// tolerance matters for a ray on a face of the cube (s = +-1 along an axis): there sx is
+// ~1e-17 and x = s + sx*i rounds to exactly +-1 for dozens of points past the continuous
+// bound.  An empty range comes back as lo > hi.
+__device__ __forceinline__ void ray_range(double px, double py, double sx, double sy, int n,
+                                          int& lo, int& hi) {
+  const double tol = 1e-9;
+  lo = -2 * n;
+  hi = 2 * n;
+  const double tl = fmax(fabs(sx) > 0 ? fmin((-1.0 - tol - px) / sx, (1.0 + tol - px) / sx) : -1e300,
+                         fabs(sy) > 0 ? fmin((-1.0 - tol - py) / sy, (1.0 + tol - py) / sy) : -1e300);
+  const double th = fmin(fabs(sx) > 0 ? fmax((-1.0 - tol - px) / sx, (1.0 + tol - px) / sx) : 1e300,
+                         fabs(sy) > 0 ? fmax((-1.0 - tol - py) / sy, (1.0 + tol - py) / sy) : 1e300);
+  // clamped to +-4n before the conversion: a ray far outside gives bounds ~1e17
+  if (tl > -1e299) lo = max(lo, (int)fmin(fmax(ceil(tl) - 1.0, -4.0 * n), 4.0 * n));
+  if (th < 1e299) hi = min(hi, (int)fmax(fmin(floor(th) + 1.0, 4.0 * n), -4.0 * n));
+  cube_ends(px, sx, true, true, lo, hi);
+  cube_ends(py, sy, true, true, lo, hi);
+}
+
 // vol32: [z][x][y] FP32; zi / zc: the slices' index and fraction (reference rounding;
 // zi < 0 marks a slice with |z| > 1, whose rays are 0);
 // trig: [nphi][4] cos p, sin p, cos(p+pi/2), sin(p+pi/2) (host libm); out [z][s][phi] FP32
@@ -82,18 +105,8 @@ __global__ __launch_bounds__(RAY_THREADS) void k_ray_f32(const float* __restrict
   const double delta = (2.0 / (n - 1)) / (2.0);
   const double dg = 2.0 / (n - 1), inv_dg = 1.0 / dg, h = (n - 1) / 2.0;
   const double sx = delta * dx, sy = delta * dy;
-  // the exact range of i with |x| <= 1 and |y| <= 1
-  int lo = -2 * n, hi = 2 * n;
-  {
-    const double tl = fmax(fabs(sx) > 0 ? fmin((-1.0 - px) / sx, (1.0 - px) / sx) : -1e300,
-                           fabs(sy) > 0 ? fmin((-1.0 - py) / sy, (1.0 - py) / sy) : -1e300);
-    const double th = fmin(fabs(sx) > 0 ? fmax((-1.0 - px) / sx, (1.0 - px) / sx) : 1e300,
-                           fabs(sy) > 0 ? fmax((-1.0 - py) / sy, (1.0 - py) / sy) : 1e300);
-    if (tl > -1e299) lo = max(lo, (int)fmax(ceil(tl) - 1.0, -4.0 * n));
-    if (th < 1e299) hi = min(hi, (int)fmin(floor(th) + 1.0, 4.0 * n));
-    cube_ends(px, sx, true, true, lo, hi);
-    cube_ends(py, sy, true, true, lo, hi);
-  }
+  int lo, hi;
+  ray_range(px, py, sx, sy, n, lo, hi);
   float acc = 0.f, comp = 0.f;
   for (int i = lo; i <= hi; ++i) {
     const double fi = (double)i;
@@ -154,6 +167,129 @@ extern "C" int wrtk_ray_f32(const float* vol32, int n, const int* zi, const doub
     launch_ray<true>(vol32, n, zi, zc, nz, shift, nshift, trig, nphi, out, st);
   else
     launch_ray<false>(vol32, n, zi, zc, nz, shift, nshift, trig, nphi, out, st);
+  WRT_LAUNCH_CHECK();
+  return WRT_OK;
+}
+
+// ------------------------------------------------------ transpose of the line integrals ---
+// The exact transpose of k_ray_f32: every point of a ray adds g * w to the four (x, y)
+// corners of its cell with the bilinear weights w the forward reads them with, into the
+// slice's plane; the plane then goes to the two z-nodes of the slice with weights 1 - cz
+// and cz, times delta.  Index and fraction decisions are the forward's (the reference's).
+// Work layout: blockIdx.y = slice, blockIdx.x = one of the slice's ray blocks; a thread
+// walks rays t = blockIdx.x * RAY_THREADS + threadIdx.x + k * gridDim.x * RAY_THREADS.
+// For n <= 181 the plane is an LDS FP32 accumulator (LDS atomics), added to the FP64
+// output accumulator once per workgroup; larger volumes add every point's weights to the
+// FP64 accumulator directly.  acc: n^3 FP64 [x][y][z], zeroed by the caller.
+template <bool LDS, bool POW2>
+__global__ __launch_bounds__(RAY_THREADS) void k_ray_T_f32(const float* __restrict__ g, int n,
+                                                           const int* __restrict__ zi,
+                                                           const double* __restrict__ zc,
+                                                           const double* __restrict__ shift,
+                                                           int nshift,
+                                                           const double* __restrict__ trig,
+                                                           int nphi, double* __restrict__ acc) {
+  extern __shared__ float plane[];
+  const int iz = blockIdx.y;
+  const int kz = zi[iz];
+  if (kz < 0) return;  // |z| > 1: no point of the slice passes the cube test
+  const double czd = zc[iz];
+  const double delta = (2.0 / (n - 1)) / (2.0);
+  const size_t nn = (size_t)n * n;
+  if (LDS) {
+    for (int k = threadIdx.x; k < (int)nn; k += RAY_THREADS) plane[k] = 0.f;
+    __syncthreads();
+  }
+  const long long nrays = (long long)nshift * nphi;
+  const double dg = 2.0 / (n - 1), inv_dg = 1.0 / dg, h = (n - 1) / 2.0;
+  const double w0 = (1.0 - czd) * delta, w1 = czd * delta;  // z-node weights (global path)
+  for (long long t = (long long)blockIdx.x * RAY_THREADS + threadIdx.x; t < nrays;
+       t += (long long)gridDim.x * RAY_THREADS) {
+    const int is = (int)(t % nshift), ip = (int)(t / nshift);
+    const float gv = g[((size_t)iz * nshift + is) * nphi + ip];
+    if (gv == 0.f) continue;
+    const double c = trig[4 * ip], sn = trig[4 * ip + 1], dx = trig[4 * ip + 2],
+                 dy = trig[4 * ip + 3];
+    const double s = shift[is];
+    const double px = s * c, py = s * sn;
+    const double sx = delta * dx, sy = delta * dy;
+    int lo, hi;
+    ray_range(px, py, sx, sy, n, lo, hi);
+    for (int i = lo; i <= hi; ++i) {
+      const double fi = (double)i;
+      const double x = px + sx * fi, y = py + sy * fi;
+      int ix, iy;
+      float fx, fy;
+      ray_axis<POW2>(x, n, dg, inv_dg, h, ix, fx);
+      ray_axis<POW2>(y, n, dg, inv_dg, h, iy, fy);
+      // the forward's value: lerp(fy, lerp(fx, p00, p10), lerp(fx, p01, p11))
+      const float gy0 = gv * (1.f - fy), gy1 = gv * fy;
+      const float w00 = gy0 * (1.f - fx), w10 = gy0 * fx, w01 = gy1 * (1.f - fx), w11 = gy1 * fx;
+      const int o = ix * n + iy;
+      if (LDS) {
+        atomicAdd(&plane[o], w00);
+        atomicAdd(&plane[o + n], w10);
+        atomicAdd(&plane[o + 1], w01);
+        atomicAdd(&plane[o + n + 1], w11);
+      } else {
+        double* p = acc + (size_t)o * n + kz;  // [x][y][z]
+        const float wv[4] = {w00, w10, w01, w11};
+        const size_t off[4] = {0, nn, (size_t)n, nn + n};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          if (wv[q] == 0.f) continue;
+          unsafeAtomicAdd(p + off[q], (double)wv[q] * w0);
+          unsafeAtomicAdd(p + off[q] + 1, (double)wv[q] * w1);
+        }
+      }
+    }
+  }
+  if (LDS) {
+    __syncthreads();
+    for (int k = threadIdx.x; k < (int)nn; k += RAY_THREADS) {
+      const float v = plane[k];
+      if (v == 0.f) continue;
+      double* p = acc + (size_t)k * n + kz;  // k = x*n + y
+      unsafeAtomicAdd(p, (double)v * w0);
+      unsafeAtomicAdd(p + 1, (double)v * w1);
+    }
+  }
+}
+
+template <bool LDS>
+static void launch_ray_T(const float* g, int n, const int* zi, const double* zc, int nz,
+                         const double* shift, int nshift, const double* trig, int nphi,
+                         double* acc, hipStream_t st) {
+  const long long nrays = (long long)nshift * nphi;
+  // about 2048 workgroups in all, at least one ray per thread of each
+  long long per = (2048 + nz - 1) / nz;
+  const long long most = (nrays + RAY_THREADS - 1) / RAY_THREADS;
+  if (per > most) per = most;
+  if (per < 1) per = 1;
+  const dim3 grid((unsigned)per, (unsigned)nz);
+  const size_t lds = LDS ? (size_t)n * n * sizeof(float) : 0;
+  const int m = n - 1;
+  if ((m & (m - 1)) == 0) {
+    if (lds > 65536)
+      (void)hipFuncSetAttribute((const void*)k_ray_T_f32<LDS, true>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    k_ray_T_f32<LDS, true><<<grid, RAY_THREADS, lds, st>>>(g, n, zi, zc, shift, nshift, trig, nphi, acc);
+  } else {
+    if (lds > 65536)
+      (void)hipFuncSetAttribute((const void*)k_ray_T_f32<LDS, false>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    k_ray_T_f32<LDS, false><<<grid, RAY_THREADS, lds, st>>>(g, n, zi, zc, shift, nshift, trig, nphi, acc);
+  }
+}
+
+extern "C" int wrtk_ray_T_f32(const float* g, int n, const int* zi, const double* zc, int nz,
+                              const double* shift, int nshift, const double* trig, int nphi,
+                              double* acc, hipStream_t st) {
+  if (n < 2 || nz < 1 || nz > 65535 || nshift < 1 || nphi < 1) return WRT_E_ARG;
+  if (n <= RAY_LDS_N)
+    launch_ray_T<true>(g, n, zi, zc, nz, shift, nshift, trig, nphi, acc, st);
+  else
+    launch_ray_T<false>(g, n, zi, zc, nz, shift, nshift, trig, nphi, acc, st);
   WRT_LAUNCH_CHECK();
   return WRT_OK;
 }

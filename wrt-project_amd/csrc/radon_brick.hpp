@@ -151,10 +151,15 @@ struct radon_lds {
   int next_dir;
 };
 
-// One task: brick `brick` of a.bricks against directions [grp*dgroup, (grp+1)*dgroup).
+// One task: brick `brick` against directions [grp*dgroup, (grp+1)*dgroup).
+// Forward (T = false): the brick of a.bricks is staged and every section's sum is added to
+// its plane's FP64 accumulator.  Transpose (T = true): the LDS brick starts at zero and
+// every sample of a section whose cell lies in the brick adds the section's value a.g[plane]
+// to that cell (an LDS FP32 atomic), so that after the task the brick holds
+// sum_planes g_p * (number of the plane's samples in the cell); the caller stores it.
 // Called by every thread of a FR_WAVES*64-thread workgroup; returns with the workgroup's
-// accumulator updates issued (no barrier at the end: a caller that reuses L waits first).
-template <bool BND, bool BF16, bool POW2>
+// updates issued (no barrier at the end: a caller that reads or reuses L waits first).
+template <bool BND, bool BF16, bool POW2, bool T = false>
 __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, int brick, int grp,
                                                  double h, double dc, double inv_dc,
                                                  radon_lds& L) {
@@ -166,8 +171,11 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
   const int bxi = brick / (nb * nb), byi = (brick / nb) % nb, bzi = brick % nb;
   const int b0x = bxi * FB, b0y = byi * FB, b0z = bzi * FB;
 
-  // stage the brick into the skewed LDS layout
-  {
+  // stage the brick into the skewed LDS layout (transpose: clear the accumulator brick)
+  if (T) {
+    for (int i = threadIdx.x; i < FB * FR_PX + 1; i += blockDim.x) sb[i] = 0.f;
+    if (threadIdx.x == 0) L.next_dir = grp * a.dgroup;
+  } else {
     const size_t base = (size_t)brick * (FB * FB * FB);
     for (int i = threadIdx.x; i < FB * FB * FB; i += blockDim.x) {
       const int lx = i >> 10, ly = (i >> 5) & 31, lz = i & 31;
@@ -291,6 +299,11 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
     }
     const double ia = 1.0 / sqrt(aa);
     for (int j = j0; j <= j1; ++j) {
+      float gv = 0.f;  // transpose: the section's value
+      if (T) {
+        gv = a.g[(size_t)j * a.ndir + d];
+        if (gv == 0.f) continue;
+      }
       const double s = a.shift[j];
       // the plane meets the support sphere in a circle of radius sqrt(rho2 - s^2) about the
       // foot point s*n, where row 0 passes: rows farther than that (+1) are empty
@@ -435,15 +448,24 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
           // inline constant in the select), the brick starts at slot 1
           const unsigned off = mad_u24((unsigned)lx, 4u * FR_PX,
                                        mad_u24((unsigned)ly, 4u * FR_PY, (unsigned)lz * 4u + 4u));
-          bs += *(const float*)((const char*)sb + (ok ? off : 0u));
+          if (T) {
+            if (ok) atomicAdd((float*)((char*)sb + off), gv);
+          } else {
+            bs += *(const float*)((const char*)sb + (ok ? off : 0u));
+          }
         }
-        const float y = bs - kc;
-        const float tt = ks + y;
-        kc = (tt - ks) - y;
-        ks = tt;
+        if (!T) {
+          const float y = bs - kc;
+          const float tt = ks + y;
+          kc = (tt - ks) - y;
+          ks = tt;
+        }
       }
-      const double tot = wave_sum_dpp((double)ks - (double)kc);
-      if (lane == 0 && tot != 0.0) unsafeAtomicAdd(a.acc + (size_t)j * a.ndir + d, tot * delta * delta);
+      if (!T) {
+        const double tot = wave_sum_dpp((double)ks - (double)kc);
+        if (lane == 0 && tot != 0.0)
+          unsafeAtomicAdd(a.acc + (size_t)j * a.ndir + d, tot * delta * delta);
+      }
     }
   }
 }

@@ -71,6 +71,11 @@ int wrtk_vol_zxy_f32(const double* vol, int n, float* out, hipStream_t st);
 int wrtk_ray_f32(const float* vol32, int n, const int* zi, const double* zc, int nz,
                  const double* shift, int nshift, const double* trig, int nphi, float* out,
                  hipStream_t st);
+/* exact transpose of wrtk_ray_f32: g [z][s][phi] FP32 -> acc n^3 FP64 [x][y][z] (added to;
+ * zeroed by the caller), with the forward's index and fraction decisions */
+int wrtk_ray_T_f32(const float* g, int n, const int* zi, const double* zc, int nz,
+                   const double* shift, int nshift, const double* trig, int nphi, double* acc,
+                   hipStream_t st);
 
 /* number of lattice samples passing zero_interp's cube test, summed over all planes */
 int wrtk_radon_incube(int n, const double* dirs, int ndir, const double* shift, int nshift,

@@ -33,6 +33,9 @@ typedef struct {
   double rho2;        /* squared support radius: every sample farther from the
                          origin reads a zero cell (cell-centre radius of the
                          farthest non-zero cell plus half a cell diagonal)     */
+  /* transpose only */
+  const float* g;     /* [nshift][ndir] plane values                          */
+  float* part;        /* [ngroups][nb^3][WRT_BRICK^3] per-group brick partials */
 } wrt_radon_fast_args;
 
 /* FP64 vol[x][y][z] -> bricks; flags[brick] += its number of non-zero values (flags zeroed by the caller);
@@ -44,6 +47,13 @@ int wrtk_brickify(const double* vol, int n, int nb, int bf16, void* bricks, int*
 int wrtk_radon_fast(const wrt_radon_fast_args* a, int boundary, hipStream_t st);
 /* out[i] = (float)acc[i] */
 int wrtk_acc_to_f32(const double* acc, float* out, size_t count, hipStream_t st);
+
+/* Exact transpose of the plane integrals in FP32 (a.g, a.part set; a.rho2 >= 3 so that no
+ * row is clipped): every task (brick, group) stores its LDS brick to part[group][brick] */
+int wrtk_radon_T_fast(const wrt_radon_fast_args* a, int boundary, hipStream_t st);
+/* out[x][y][z] = (float)(scale * sum over groups of part[group][brick][cell]), in FP64 */
+int wrtk_radon_T_reduce(const float* part, int ngroups, int n, int nb, double scale, float* out,
+                        hipStream_t st);
 
 /* ---- the persistent batch launch: many jobs' tasks in one grid ---- */
 enum { WRT_ITEM_RADON = 0, WRT_ITEM_ADJOINT = 1 };

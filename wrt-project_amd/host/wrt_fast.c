@@ -14,6 +14,15 @@
  * form; each interval's cubic is then rounded to FP32 (y, b, c, d), directions become float4
  * (unit normal, weight), and one thread per voxel sums the directions with Kahan compensation.
  * The interval of s is found arithmetically, so the shift grid must be uniform.
+ *
+ * RADON_T (FP32): the exact transpose of the plane integrals.  The plane values are rounded
+ * to FP32; every (brick, direction group) task scatters its sections into an LDS brick
+ * (index decisions of the forward) and stores it to the group's slot; the slots are summed
+ * per voxel in FP64 and scaled by delta^2.  Every brick of the cube takes part.
+ *
+ * RAY_T (FP32): the exact transpose of the FP32 line integrals: per slice, bilinear weights
+ * scattered into an LDS plane, added to an FP64 volume accumulator at the slice's two
+ * z-nodes, rounded to FP32.
  */
 #include <stdlib.h>
 #include <math.h>
@@ -90,20 +99,9 @@ static void ray_fast_destroy(wrt_job* j) {
   j->fast = NULL;
 }
 
-static int ray_fast_create(wrt_job* j) {
-  const wrt_job_desc* d = &j->d;
-  ray_fast* f = (ray_fast*)calloc(1, sizeof(ray_fast));
-  if (!f) return WRT_E_NOMEM;
-  j->fast = f;
-  const size_t n3 = (size_t)d->n * d->n * d->n;
-  if (hipMalloc((void**)&f->staging, n3 * 8) != hipSuccess ||
-      hipMalloc((void**)&f->vol32, n3 * 4) != hipSuccess ||
-      hipMalloc((void**)&f->zi, (size_t)d->nz * sizeof(int)) != hipSuccess ||
-      hipMalloc((void**)&f->zc, (size_t)d->nz * sizeof(double)) != hipSuccess ||
-      hipMalloc((void**)&f->out, (size_t)d->nz * d->nshift * d->nphi * 4) != hipSuccess)
-    return WRT_E_NOMEM;
-  /* the slice's index and fraction exactly as cube_trilinear_interp decides them for the
-   * constant z of every point of the slice (ray_grid/src/interp.c:25-47) */
+/* a slice's index and fraction exactly as cube_trilinear_interp decides them for the
+ * constant z of every point of the slice (ray_grid/src/interp.c:25-47) */
+static int slice_decisions(const wrt_job_desc* d, int* d_zi, double* d_zc, hipStream_t st) {
   int* zi = (int*)malloc((size_t)d->nz * sizeof(int));
   double* zc = (double*)malloc((size_t)d->nz * sizeof(double));
   if (!zi || !zc) {
@@ -129,13 +127,28 @@ static int ray_fast_create(wrt_job* j) {
     zc[l] = cz;
   }
   int rc = WRT_OK;
-  if (hipMemcpyAsync(f->zi, zi, (size_t)d->nz * sizeof(int), hipMemcpyHostToDevice, j->st) != hipSuccess ||
-      hipMemcpyAsync(f->zc, zc, (size_t)d->nz * sizeof(double), hipMemcpyHostToDevice, j->st) != hipSuccess ||
-      hipStreamSynchronize(j->st) != hipSuccess)
+  if (hipMemcpyAsync(d_zi, zi, (size_t)d->nz * sizeof(int), hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipMemcpyAsync(d_zc, zc, (size_t)d->nz * sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
     rc = WRT_E_HIP;
   free(zi);
   free(zc);
   return rc;
+}
+
+static int ray_fast_create(wrt_job* j) {
+  const wrt_job_desc* d = &j->d;
+  ray_fast* f = (ray_fast*)calloc(1, sizeof(ray_fast));
+  if (!f) return WRT_E_NOMEM;
+  j->fast = f;
+  const size_t n3 = (size_t)d->n * d->n * d->n;
+  if (hipMalloc((void**)&f->staging, n3 * 8) != hipSuccess ||
+      hipMalloc((void**)&f->vol32, n3 * 4) != hipSuccess ||
+      hipMalloc((void**)&f->zi, (size_t)d->nz * sizeof(int)) != hipSuccess ||
+      hipMalloc((void**)&f->zc, (size_t)d->nz * sizeof(double)) != hipSuccess ||
+      hipMalloc((void**)&f->out, (size_t)d->nz * d->nshift * d->nphi * 4) != hipSuccess)
+    return WRT_E_NOMEM;
+  return slice_decisions(d, f->zi, f->zc, j->st);
 }
 
 static int ray_fast_load(wrt_job* j, const double* in) {
@@ -161,7 +174,8 @@ static int ray_fast_enqueue(wrt_job* j) {
 
 int wrt_fast_supported(const wrt_job_desc* d) {
   if (!d) return 0;
-  if (d->op == WRT_OP_RAY) return d->prec == WRT_F32 && d->nz <= 65535;
+  if (d->op == WRT_OP_RAY || d->op == WRT_OP_RAY_T) return d->prec == WRT_F32 && d->nz <= 65535;
+  if (d->op == WRT_OP_RADON_T) return d->prec == WRT_F32 && d->n <= 4096;
   /* the brick kernel packs a row's first and last lattice column (|i2| <= 2n) in 16 bits */
   if (d->op == WRT_OP_RADON) return (d->prec == WRT_F32 || d->prec == WRT_BF16) && d->n <= 4096;
   if (d->op == WRT_OP_ADJOINT) {
@@ -450,42 +464,282 @@ static int radon_fast_enqueue(wrt_job* j) {
   return wrtk_acc_to_f32(f->acc, f->out, nout, st);
 }
 
+/* ------------------------------------------------------------------ RADON_T --- */
+typedef struct {
+  int nb;
+  size_t nbricks;
+  int dgroup, ngroups;
+  int n_interior, n_boundary;
+  int* tasks;      /* device, interior then boundary */
+  double* staging; /* FP64 plane values [nshift][ndir] */
+  float* g32;      /* FP32 plane values */
+  float* part;     /* [ngroups][nbricks][WRT_BRICK^3] */
+  float* out;      /* n^3 [x][y][z] */
+} radon_T_fast;
+
+#define RT_PART_BUDGET ((size_t)1 << 30) /* bytes of group partials at most */
+
+/* groups: about 4096 tasks, at least 16 directions per group, partials within the budget */
+static void radon_T_groups(const wrt_job_desc* d, size_t nbricks, int* ngroups, int* dgroup) {
+  const size_t ndir = (size_t)d->nphi * d->ntheta;
+  const size_t slot = nbricks * WRT_BRICK * WRT_BRICK * WRT_BRICK * sizeof(float);
+  size_t g = (4096 + nbricks - 1) / nbricks;
+  const size_t by_dirs = ndir / 16 ? ndir / 16 : 1;
+  const size_t by_mem = RT_PART_BUDGET / slot ? RT_PART_BUDGET / slot : 1;
+  if (g > by_dirs) g = by_dirs;
+  if (g > by_mem) g = by_mem;
+  if (g < 1) g = 1;
+  const size_t dg = (ndir + g - 1) / g;
+  *dgroup = (int)dg;
+  *ngroups = (int)((ndir + dg - 1) / dg);
+}
+
+static size_t radon_T_fast_bytes(const wrt_job_desc* d) {
+  const size_t nb = (size_t)nb_of(d->n);
+  const size_t nbr = nb * nb * nb;
+  int ng, dg;
+  radon_T_groups(d, nbr, &ng, &dg);
+  const size_t nplanes = (size_t)d->nshift * d->nphi * d->ntheta;
+  return nplanes * 12 + (size_t)ng * nbr * WRT_BRICK * WRT_BRICK * WRT_BRICK * 4 +
+         2 * nbr * (size_t)ng * sizeof(int) + (size_t)d->n * d->n * d->n * 4;
+}
+
+static void radon_T_fast_destroy(wrt_job* j) {
+  radon_T_fast* f = (radon_T_fast*)j->fast;
+  if (!f) return;
+  hipFree(f->tasks);
+  hipFree(f->staging);
+  hipFree(f->g32);
+  hipFree(f->part);
+  hipFree(f->out);
+  free(f);
+  j->fast = NULL;
+}
+
+static int radon_T_fast_create(wrt_job* j) {
+  const wrt_job_desc* d = &j->d;
+  radon_T_fast* f = (radon_T_fast*)calloc(1, sizeof(radon_T_fast));
+  if (!f) return WRT_E_NOMEM;
+  j->fast = f;
+  f->nb = nb_of(d->n);
+  f->nbricks = (size_t)f->nb * f->nb * f->nb;
+  radon_T_groups(d, f->nbricks, &f->ngroups, &f->dgroup);
+  const size_t nplanes = (size_t)d->nshift * d->nphi * d->ntheta;
+  const size_t ntasks = f->nbricks * (size_t)f->ngroups;
+  if (ntasks > 0x7fffffff) return WRT_E_ARG;
+  if (hipMalloc((void**)&f->tasks, 2 * ntasks * sizeof(int)) != hipSuccess ||
+      hipMalloc((void**)&f->staging, nplanes * 8) != hipSuccess ||
+      hipMalloc((void**)&f->g32, nplanes * 4) != hipSuccess ||
+      hipMalloc((void**)&f->part, (size_t)f->ngroups * f->nbricks * WRT_BRICK * WRT_BRICK * WRT_BRICK * 4) != hipSuccess ||
+      hipMalloc((void**)&f->out, (size_t)d->n * d->n * d->n * 4) != hipSuccess)
+    return WRT_E_NOMEM;
+  /* bricks nearest the centre first (planes with |s| <= 1 cut them from every direction, so
+   * their tasks are the longest), interior bricks then bricks on the cube's faces */
+  int* order = (int*)malloc(f->nbricks * sizeof(int));
+  double* dist = (double*)malloc(f->nbricks * sizeof(double));
+  int* t = (int*)malloc(2 * ntasks * sizeof(int));
+  if (!order || !dist || !t) {
+    free(order);
+    free(dist);
+    free(t);
+    return WRT_E_NOMEM;
+  }
+  const int nb = f->nb;
+  for (size_t b = 0; b < f->nbricks; ++b) {
+    const int bx = (int)(b / ((size_t)nb * nb)), by = (int)((b / nb) % nb), bz = (int)(b % nb);
+    const double c[3] = {bx + 0.5, by + 0.5, bz + 0.5};
+    double r2 = 0.0;
+    for (int k = 0; k < 3; ++k) {
+      const double x = -1.0 + 2.0 * WRT_BRICK * c[k] / d->n;
+      r2 += x * x;
+    }
+    order[b] = (int)b;
+    dist[b] = r2;
+  }
+  for (size_t i = 1; i < f->nbricks; ++i) { /* insertion sort by distance */
+    const int v = order[i];
+    long q = (long)i - 1;
+    while (q >= 0 && dist[order[q]] > dist[v]) { order[q + 1] = order[q]; --q; }
+    order[q + 1] = v;
+  }
+  size_t k = 0;
+  for (int pass = 0; pass < 2; ++pass) {
+    for (size_t i = 0; i < f->nbricks; ++i) {
+      const int b = order[i];
+      const int bx = b / (nb * nb), by = (b / nb) % nb, bz = b % nb;
+      const int bnd = bx == 0 || by == 0 || bz == 0 || (bx + 1) * WRT_BRICK >= d->n ||
+                      (by + 1) * WRT_BRICK >= d->n || (bz + 1) * WRT_BRICK >= d->n;
+      if (bnd != pass) continue;
+      for (int g = 0; g < f->ngroups; ++g) {
+        t[2 * k] = b;
+        t[2 * k + 1] = g;
+        ++k;
+      }
+    }
+    if (pass == 0) f->n_interior = (int)k;
+  }
+  f->n_boundary = (int)k - f->n_interior;
+  int rc = WRT_OK;
+  if (hipMemcpyAsync(f->tasks, t, 2 * k * sizeof(int), hipMemcpyHostToDevice, j->st) != hipSuccess ||
+      hipStreamSynchronize(j->st) != hipSuccess)
+    rc = WRT_E_HIP;
+  free(order);
+  free(dist);
+  free(t);
+  return rc;
+}
+
+static int radon_T_fast_load(wrt_job* j, const double* in) {
+  radon_T_fast* f = (radon_T_fast*)j->fast;
+  const size_t nplanes = (size_t)j->d.nshift * j->d.nphi * j->d.ntheta;
+  HIPCHK(hipMemcpyAsync(f->staging, in, nplanes * 8, hipMemcpyHostToDevice, j->st));
+  const int rc = wrtk_f64_to_f32(f->staging, f->g32, nplanes, j->st);
+  if (rc != WRT_OK) return rc;
+  HIPCHK(hipStreamSynchronize(j->st));
+  return WRT_OK;
+}
+
+static int radon_T_fast_enqueue(wrt_job* j) {
+  radon_T_fast* f = (radon_T_fast*)j->fast;
+  const wrt_job_desc* d = &j->d;
+  hipStream_t st = j->st;
+  wrt_radon_fast_args a;
+  memset(&a, 0, sizeof(a));
+  a.n = d->n;
+  a.nb = f->nb;
+  a.dirs = j->d_dirs;
+  a.ndir = d->nphi * d->ntheta;
+  a.dgroup = f->dgroup;
+  a.shift = j->d_shift;
+  a.nshift = d->nshift;
+  a.rho2 = 4.0; /* no support clipping: every cell of the cube receives */
+  a.g = f->g32;
+  a.part = f->part;
+  HIPCHK(hipEventRecord(j->e1, st));
+  a.tasks = f->tasks;
+  a.ntasks = f->n_interior;
+  int rc = wrtk_radon_T_fast(&a, 0, st);
+  if (rc != WRT_OK) return rc;
+  a.tasks = f->tasks + 2 * (size_t)f->n_interior;
+  a.ntasks = f->n_boundary;
+  rc = wrtk_radon_T_fast(&a, 1, st);
+  if (rc != WRT_OK) return rc;
+  HIPCHK(hipEventRecord(j->e3, st));
+  const double delta = (1.0 / (d->n));
+  return wrtk_radon_T_reduce(f->part, f->ngroups, d->n, f->nb, delta * delta, f->out, st);
+}
+
+/* -------------------------------------------------------------------- RAY_T --- */
+typedef struct {
+  double* staging; /* FP64 ray values [z][s][phi] */
+  float* g32;
+  int* zi;         /* [nz] slice index (-1: |z| > 1) */
+  double* zc;      /* [nz] slice fraction */
+  double* acc;     /* n^3 FP64 [x][y][z] */
+  float* out;      /* n^3 FP32 */
+} ray_T_fast;
+
+static size_t ray_T_fast_bytes(const wrt_job_desc* d) {
+  const size_t n3 = (size_t)d->n * d->n * d->n;
+  return (size_t)d->nz * d->nshift * d->nphi * 12 + n3 * 12 + (size_t)d->nz * 12;
+}
+
+static void ray_T_fast_destroy(wrt_job* j) {
+  ray_T_fast* f = (ray_T_fast*)j->fast;
+  if (!f) return;
+  hipFree(f->staging);
+  hipFree(f->g32);
+  hipFree(f->zi);
+  hipFree(f->zc);
+  hipFree(f->acc);
+  hipFree(f->out);
+  free(f);
+  j->fast = NULL;
+}
+
+static int ray_T_fast_create(wrt_job* j) {
+  const wrt_job_desc* d = &j->d;
+  ray_T_fast* f = (ray_T_fast*)calloc(1, sizeof(ray_T_fast));
+  if (!f) return WRT_E_NOMEM;
+  j->fast = f;
+  const size_t n3 = (size_t)d->n * d->n * d->n;
+  const size_t nr = (size_t)d->nz * d->nshift * d->nphi;
+  if (hipMalloc((void**)&f->staging, nr * 8) != hipSuccess ||
+      hipMalloc((void**)&f->g32, nr * 4) != hipSuccess ||
+      hipMalloc((void**)&f->zi, (size_t)d->nz * sizeof(int)) != hipSuccess ||
+      hipMalloc((void**)&f->zc, (size_t)d->nz * sizeof(double)) != hipSuccess ||
+      hipMalloc((void**)&f->acc, n3 * 8) != hipSuccess ||
+      hipMalloc((void**)&f->out, n3 * 4) != hipSuccess)
+    return WRT_E_NOMEM;
+  return slice_decisions(d, f->zi, f->zc, j->st);
+}
+
+static int ray_T_fast_load(wrt_job* j, const double* in) {
+  ray_T_fast* f = (ray_T_fast*)j->fast;
+  const size_t nr = (size_t)j->d.nz * j->d.nshift * j->d.nphi;
+  HIPCHK(hipMemcpyAsync(f->staging, in, nr * 8, hipMemcpyHostToDevice, j->st));
+  const int rc = wrtk_f64_to_f32(f->staging, f->g32, nr, j->st);
+  if (rc != WRT_OK) return rc;
+  HIPCHK(hipStreamSynchronize(j->st));
+  return WRT_OK;
+}
+
+static int ray_T_fast_enqueue(wrt_job* j) {
+  ray_T_fast* f = (ray_T_fast*)j->fast;
+  const wrt_job_desc* d = &j->d;
+  const size_t n3 = (size_t)d->n * d->n * d->n;
+  int rc = wrtk_fill_f64(f->acc, 0.0, n3, j->st);
+  if (rc != WRT_OK) return rc;
+  HIPCHK(hipEventRecord(j->e1, j->st));
+  rc = wrtk_ray_T_f32(f->g32, d->n, f->zi, f->zc, d->nz, j->d_shift, d->nshift, j->d_dirs,
+                      d->nphi, f->acc, j->st);
+  if (rc != WRT_OK) return rc;
+  HIPCHK(hipEventRecord(j->e3, j->st));
+  return wrtk_acc_to_f32(f->acc, f->out, n3, j->st);
+}
+
 /* dispatch by operation; each job owns one fast state */
 size_t wrt_fast_bytes(const wrt_job_desc* d) {
   if (!wrt_fast_supported(d)) return 0;
   switch (d->op) {
     case WRT_OP_RADON: return radon_fast_bytes(d);
     case WRT_OP_RAY: return ray_fast_bytes(d);
+    case WRT_OP_RADON_T: return radon_T_fast_bytes(d);
+    case WRT_OP_RAY_T: return ray_T_fast_bytes(d);
     default: return adjoint_fast_bytes(d);
   }
 }
 
-int wrt_fast_create(wrt_job* j) {
-  if (j->d.op == WRT_OP_RADON) return radon_fast_create(j);
-  if (j->d.op == WRT_OP_RAY) {
-    const int rc = ray_fast_create(j);
-    if (rc != WRT_OK && j->fast) ray_fast_destroy(j);
-    return rc;
-  }
-  const int rc = adjoint_fast_create(j);
-  if (rc != WRT_OK && j->fast) adjoint_fast_destroy(j);
-  return rc;
-}
-
 void wrt_fast_destroy(wrt_job* j) {
   if (!j->fast) return;
-  if (j->d.op == WRT_OP_RADON)
-    radon_fast_destroy(j);
-  else if (j->d.op == WRT_OP_RAY)
-    ray_fast_destroy(j);
-  else
-    adjoint_fast_destroy(j);
+  switch (j->d.op) {
+    case WRT_OP_RADON: radon_fast_destroy(j); break;
+    case WRT_OP_RAY: ray_fast_destroy(j); break;
+    case WRT_OP_RADON_T: radon_T_fast_destroy(j); break;
+    case WRT_OP_RAY_T: ray_T_fast_destroy(j); break;
+    default: adjoint_fast_destroy(j); break;
+  }
+}
+
+int wrt_fast_create(wrt_job* j) {
+  int rc;
+  switch (j->d.op) {
+    case WRT_OP_RADON: rc = radon_fast_create(j); break;
+    case WRT_OP_RAY: rc = ray_fast_create(j); break;
+    case WRT_OP_RADON_T: rc = radon_T_fast_create(j); break;
+    case WRT_OP_RAY_T: rc = ray_T_fast_create(j); break;
+    default: rc = adjoint_fast_create(j); break;
+  }
+  if (rc != WRT_OK && j->fast) wrt_fast_destroy(j);
+  return rc;
 }
 
 int wrt_fast_load(wrt_job* j, const double* in) {
   switch (j->d.op) {
     case WRT_OP_RADON: return radon_fast_load(j, in);
     case WRT_OP_RAY: return ray_fast_load(j, in);
+    case WRT_OP_RADON_T: return radon_T_fast_load(j, in);
+    case WRT_OP_RAY_T: return ray_T_fast_load(j, in);
     default: return adjoint_fast_load(j, in);
   }
 }
@@ -494,17 +748,24 @@ int wrt_fast_enqueue(wrt_job* j) {
   switch (j->d.op) {
     case WRT_OP_RADON: return radon_fast_enqueue(j);
     case WRT_OP_RAY: return ray_fast_enqueue(j);
+    case WRT_OP_RADON_T: return radon_T_fast_enqueue(j);
+    case WRT_OP_RAY_T: return ray_T_fast_enqueue(j);
     default: return adjoint_fast_enqueue(j);
   }
 }
 
 int wrt_fast_fetch(wrt_job* j, double* out) {
   const wrt_job_desc* d = &j->d;
-  if (d->op == WRT_OP_RADON)
-    return fetch_f32(j, ((radon_fast*)j->fast)->out, (size_t)d->nshift * d->nphi * d->ntheta, out);
-  if (d->op == WRT_OP_RAY)
-    return fetch_f32(j, ((ray_fast*)j->fast)->out, (size_t)d->nz * d->nshift * d->nphi, out);
-  return fetch_f32(j, ((adjoint_fast*)j->fast)->out, (size_t)d->n * d->n * d->n, out);
+  const size_t n3 = (size_t)d->n * d->n * d->n;
+  switch (d->op) {
+    case WRT_OP_RADON:
+      return fetch_f32(j, ((radon_fast*)j->fast)->out, (size_t)d->nshift * d->nphi * d->ntheta, out);
+    case WRT_OP_RAY:
+      return fetch_f32(j, ((ray_fast*)j->fast)->out, (size_t)d->nz * d->nshift * d->nphi, out);
+    case WRT_OP_RADON_T: return fetch_f32(j, ((radon_T_fast*)j->fast)->out, n3, out);
+    case WRT_OP_RAY_T: return fetch_f32(j, ((ray_T_fast*)j->fast)->out, n3, out);
+    default: return fetch_f32(j, ((adjoint_fast*)j->fast)->out, n3, out);
+  }
 }
 
 /* a loaded fast job as an item of the persistent batch launch (ra.dgroup left to the
