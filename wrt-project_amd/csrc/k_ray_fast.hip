@@ -178,7 +178,7 @@ extern "C" int wrtk_ray_f32(const float* vol32, int n, const int* zi, const doub
 // and cz, times delta.  Index and fraction decisions are the forward's (the reference's).
 // Work layout: blockIdx.y = slice, blockIdx.x = one of the slice's ray blocks; a thread
 // walks rays t = blockIdx.x * RAY_THREADS + threadIdx.x + k * gridDim.x * RAY_THREADS.
-// For n <= 181 the plane is an LDS FP32 accumulator (LDS atomics), added to the FP64
+// For n <= 181 the plane is an LDS FP32 accumulator (lds_add_f32), added to the FP64
 // output accumulator once per workgroup; larger volumes add every point's weights to the
 // FP64 accumulator directly.  acc: n^3 FP64 [x][y][z], zeroed by the caller.
 template <bool LDS, bool POW2>
@@ -227,10 +227,10 @@ __global__ __launch_bounds__(RAY_THREADS) void k_ray_T_f32(const float* __restri
       const float w00 = gy0 * (1.f - fx), w10 = gy0 * fx, w01 = gy1 * (1.f - fx), w11 = gy1 * fx;
       const int o = ix * n + iy;
       if (LDS) {
-        atomicAdd(&plane[o], w00);
-        atomicAdd(&plane[o + n], w10);
-        atomicAdd(&plane[o + 1], w01);
-        atomicAdd(&plane[o + n + 1], w11);
+        lds_add_f32(&plane[o], w00);
+        lds_add_f32(&plane[o + n], w10);
+        lds_add_f32(&plane[o + 1], w01);
+        lds_add_f32(&plane[o + n + 1], w11);
       } else {
         double* p = acc + (size_t)o * n + kz;  // [x][y][z]
         const float wv[4] = {w00, w10, w01, w11};

@@ -89,6 +89,23 @@ __device__ __forceinline__ double wave_sum_dpp(double v) {
   return __hiloint2double(hi, lo);
 }
 
+// v added to the LDS float at p by compare-and-swap.  gfx950 executes ds_add_f32 lane by
+// lane: measured 193 clocks per wave instruction per CU, against 4.6 for ds_add_u32 and 11
+// for this read + ds_cmpst_rtn_b32 loop.  The loop repeats only for lanes whose cell
+// another lane or wave changed in between.
+__device__ __forceinline__ void lds_add_f32(float* p, float v) {
+  unsigned* const q = (unsigned*)p;
+  unsigned old = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  for (;;) {
+    unsigned expect = old;
+    const unsigned nv = __float_as_uint(__uint_as_float(old) + v);
+    if (__hip_atomic_compare_exchange_strong(q, &expect, nv, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_WORKGROUP))
+      break;
+    old = expect;
+  }
+}
+
 __device__ __forceinline__ float bf16_to_f32(unsigned short v) {
   return __uint_as_float(((unsigned)v) << 16);
 }
@@ -157,6 +174,7 @@ struct radon_lds {
 // every sample of a section whose cell lies in the brick adds the section's value a.g[plane]
 // to that cell (an LDS FP32 atomic), so that after the task the brick holds
 // sum_planes g_p * (number of the plane's samples in the cell); the caller stores it.
+// (The additions are compare-and-swap loops, see lds_add_f32.)
 // Called by every thread of a FR_WAVES*64-thread workgroup; returns with the workgroup's
 // updates issued (no barrier at the end: a caller that reads or reuses L waits first).
 template <bool BND, bool BF16, bool POW2, bool T = false>
@@ -383,15 +401,23 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
               for (int c = 0; c < 3; ++c)
                 if (flo[c] || fhi[c]) cube_ends(S[c] + av[c] * f1, bv[c], flo[c], fhi[c], rl, rh);
             }
-            nst = rh >= rl ? ((rh - rl + 8) >> 3) : 0;
+            // steps of 8 columns (forward) or of 32 columns (transpose walk, below)
+            nst = rh >= rl ? (T ? ((rh - rl + 32) >> 5) : ((rh - rl + 8) >> 3)) : 0;
           }
         }
         // first and last column, 16 bits each (|i2| <= 2n < 2^15)
         rowlo[wv][t] = (int)(((unsigned)rl & 0xffffu) | ((unsigned)rh << 16));
-        // max over the 8 consecutive rows of a band (lanes 8b .. 8b+7)
-        nst = max(nst, __shfl_xor(nst, 1, 64));
-        nst = max(nst, __shfl_xor(nst, 2, 64));
-        nst = max(nst, __shfl_xor(nst, 4, 64));
+        if (T) {
+          // max over the 8 rows 32B + dr + 4r (r = 0..7) of a transpose phase
+          nst = max(nst, __shfl_xor(nst, 4, 64));
+          nst = max(nst, __shfl_xor(nst, 8, 64));
+          nst = max(nst, __shfl_xor(nst, 16, 64));
+        } else {
+          // max over the 8 consecutive rows of a band (lanes 8b .. 8b+7)
+          nst = max(nst, __shfl_xor(nst, 1, 64));
+          nst = max(nst, __shfl_xor(nst, 2, 64));
+          nst = max(nst, __shfl_xor(nst, 4, 64));
+        }
         bsteps[half] = nst;
       }
       if (__ballot((bsteps[0] | bsteps[1]) != 0) == 0)
@@ -399,62 +425,90 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      // One sample at column f2 of the row whose terms are (Rx, Ry, Rz); frh is the row's
+      // last column (face bricks).  Forward: adds the cell's value to bs; transpose: adds the
+      // section's value to the cell.
+      auto sample = [&](double Rx, double Ry, double Rz, double f2, double frh, float& bs) {
+        // per-column products b*i2, rounded as the reference rounds delta*v2*i2
+        const double x = Rx + bv[0] * f2, y = Ry + bv[1] * f2, z = Rz + bv[2] * f2;
+        double qx, qy, qz;
+        if (POW2) {
+          // RN(x + 1) * h == RN(x*h + h) for h = n/2 a power of two: scaling by 2^k
+          // commutes with rounding, so one fma gives the reference's quotient exactly
+          qx = fma(x, h, h); qy = fma(y, h, h); qz = fma(z, h, h);
+        } else {
+          qx = div_rn(x + 1.0, dc, inv_dc);
+          qy = div_rn(y + 1.0, dc, inv_dc);
+          qz = div_rn(z + 1.0, dc, inv_dc);
+        }
+        // truncation as in zero_interp; for an interior brick a negative quotient (x < -1)
+        // truncates to 0 <= b0 - 32 and a quotient above n lands past the brick, so the
+        // brick test below also is the cube test.  Face bricks: the row ends hold the cube
+        // test, and x == 1 (index n) maps to n - 1.
+        int ix = (int)qx, iy = (int)qy, iz = (int)qz;
+        if (BND) {
+          ix = min(ix, cxh);
+          iy = min(iy, cyh);
+          iz = min(iz, czh);
+        }
+        const int lx = ix - b0x, ly = iy - b0y, lz = iz - b0z;
+        // all three offsets in [0, 32): a negative offset sets high bits of the OR
+        bool ok = ((unsigned)lx | (unsigned)ly | (unsigned)lz) < (unsigned)FB;
+        if (BND) ok = ok && f2 <= frh;
+        // byte offset of the cell: slot 0 holds the zero that rejected samples read (an
+        // inline constant in the select), the brick starts at slot 1
+        const unsigned off = mad_u24((unsigned)lx, 4u * FR_PX,
+                                     mad_u24((unsigned)ly, 4u * FR_PY, (unsigned)lz * 4u + 4u));
+        if (T) {
+          if (ok) lds_add_f32((float*)((char*)sb + off), gv);
+        } else {
+          bs += *(const float*)((const char*)sb + (ok ? off : 0u));
+        }
+      };
+      // Rows past R1 and columns past the lattice end need no test: a plane's cube section
+      // lies within sqrt(3) < 2 of the plane centre, inside the [-2n, 2n) lattice, so only
+      // the brick test can reject a sample.
       float ks = 0.f, kc = 0.f;  // Kahan-compensated lane sum of the section
-      int rlo_next = rowlo[wv][r_lane];
-      for (int k = 0; k < nbands; ++k) {
-        const int t = 8 * k + r_lane;  // this lane's row within the section
-        const int rlo = (rlo_next << 16) >> 16;
-        const double frh = (double)(rlo_next >> 16);  // the row's last column
-        if (k + 1 < nbands) rlo_next = rowlo[wv][t + 8];
-        const int steps = (k < 8) ? __builtin_amdgcn_readlane(bsteps[0], 8 * k)
-                                  : __builtin_amdgcn_readlane(bsteps[1], 8 * (k - 8));
-        if (steps == 0) continue;
-        const int i1 = SR0 + t;
-        // Rows past R1 and columns past the lattice end need no test: a plane's cube section
-        // lies within sqrt(3) < 2 of the plane centre, inside the [-2n, 2n) lattice, so only
-        // the brick test below can reject a sample.
-        const double f1 = (double)i1;
-        const double Rx = S[0] + av[0] * f1, Ry = S[1] + av[1] * f1, Rz = S[2] + av[2] * f1;
-        float bs = 0.f;
-        double f2 = (double)(rlo + k_lane);  // column i2, stepped exactly in binary64
-        for (int m = 0; m < steps; ++m, f2 += 8.0) {
-          // per-column products b*i2, rounded as the reference rounds delta*v2*i2
-          const double x = Rx + bv[0] * f2, y = Ry + bv[1] * f2, z = Rz + bv[2] * f2;
-          double qx, qy, qz;
-          if (POW2) {
-            // RN(x + 1) * h == RN(x*h + h) for h = n/2 a power of two: scaling by 2^k
-            // commutes with rounding, so one fma gives the reference's quotient exactly
-            qx = fma(x, h, h); qy = fma(y, h, h); qz = fma(z, h, h);
-          } else {
-            qx = div_rn(x + 1.0, dc, inv_dc);
-            qy = div_rn(y + 1.0, dc, inv_dc);
-            qz = div_rn(z + 1.0, dc, inv_dc);
-          }
-          // truncation as in zero_interp; for an interior brick a negative quotient (x < -1)
-          // truncates to 0 <= b0 - 32 and a quotient above n lands past the brick, so the
-          // brick test below also is the cube test.  Face bricks: the row ends hold the cube
-          // test, and x == 1 (index n) maps to n - 1.
-          int ix = (int)qx, iy = (int)qy, iz = (int)qz;
-          if (BND) {
-            ix = min(ix, cxh);
-            iy = min(iy, cyh);
-            iz = min(iz, czh);
-          }
-          const int lx = ix - b0x, ly = iy - b0y, lz = iz - b0z;
-          // all three offsets in [0, 32): a negative offset sets high bits of the OR
-          bool ok = ((unsigned)lx | (unsigned)ly | (unsigned)lz) < (unsigned)FB;
-          if (BND) ok = ok && f2 <= frh;
-          // byte offset of the cell: slot 0 holds the zero that rejected samples read (an
-          // inline constant in the select), the brick starts at slot 1
-          const unsigned off = mad_u24((unsigned)lx, 4u * FR_PX,
-                                       mad_u24((unsigned)ly, 4u * FR_PY, (unsigned)lz * 4u + 4u));
-          if (T) {
-            if (ok) atomicAdd((float*)((char*)sb + off), gv);
-          } else {
-            bs += *(const float*)((const char*)sb + (ok ? off : 0u));
+      if (T) {
+        // Transpose walk: the 64 samples of one step lie 4 rows or 4 columns apart, i.e. at
+        // least 4 delta = 4/n apart in the plane, more than a cell's diagonal 2 sqrt(3)/n, so
+        // no two lanes add to the same cell at once (LDS atomics to one address serialise).
+        // Band B of 32 rows in 4 phases dr: lane (r, k) takes row 32B + 4r + dr and the
+        // columns rl + dc + 4k + 32m, dc = 0..3.
+        float unused = 0.f;
+        const int nb32 = (nrows + 31) >> 5;
+        for (int B = 0; B < nb32; ++B) {
+          for (int dr = 0; dr < 4; ++dr) {
+            const int t = 32 * B + 4 * r_lane + dr;  // this lane's row within the section
+            const int steps = __builtin_amdgcn_readlane(B < 2 ? bsteps[0] : bsteps[1],
+                                                        (32 * B + dr) & 63);
+            if (steps == 0) continue;
+            const int rw = rowlo[wv][t];
+            const int rlo = (rw << 16) >> 16;
+            const double frh = (double)(rw >> 16);
+            const double f1 = (double)(SR0 + t);
+            const double Rx = S[0] + av[0] * f1, Ry = S[1] + av[1] * f1, Rz = S[2] + av[2] * f1;
+            for (int dcol = 0; dcol < 4; ++dcol) {
+              double f2 = (double)(rlo + dcol + 4 * k_lane);  // column i2, exact in binary64
+              for (int m = 0; m < steps; ++m, f2 += 32.0) sample(Rx, Ry, Rz, f2, frh, unused);
+            }
           }
         }
-        if (!T) {
+      } else {
+        int rlo_next = rowlo[wv][r_lane];
+        for (int k = 0; k < nbands; ++k) {
+          const int t = 8 * k + r_lane;  // this lane's row within the section
+          const int rlo = (rlo_next << 16) >> 16;
+          const double frh = (double)(rlo_next >> 16);  // the row's last column
+          if (k + 1 < nbands) rlo_next = rowlo[wv][t + 8];
+          const int steps = (k < 8) ? __builtin_amdgcn_readlane(bsteps[0], 8 * k)
+                                    : __builtin_amdgcn_readlane(bsteps[1], 8 * (k - 8));
+          if (steps == 0) continue;
+          const double f1 = (double)(SR0 + t);
+          const double Rx = S[0] + av[0] * f1, Ry = S[1] + av[1] * f1, Rz = S[2] + av[2] * f1;
+          float bs = 0.f;
+          double f2 = (double)(rlo + k_lane);  // column i2, stepped exactly in binary64
+          for (int m = 0; m < steps; ++m, f2 += 8.0) sample(Rx, Ry, Rz, f2, frh, bs);
           const float y = bs - kc;
           const float tt = ks + y;
           kc = (tt - ks) - y;
