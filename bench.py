@@ -7,7 +7,9 @@ per step).  A step is one pass of the FP32 form over that grid with the volume r
 HBM.  Printed: one JSON line with plane samples/s, the dominant kernel's roofline
 (algorithmic bytes from the exact in-cube sample count), the FP32 form's relative L2 and
 max error against the plain FP64 form, the plain FP64 form's rate on a subset of the same
-grid, and a CPU baseline (the oracle on the host cores, a bounded sample of planes).
+grid, and a CPU baseline (the reference's own plane_integral, compiled from its sources, on
+the host cores over a bounded sample of planes; the oracle's restatement when that build is
+absent).
 
 The "serve" object is the served form (C request loop): p50/p99 latency of a seeded
 Poisson stream (8 requests/s, 64 requests, mix 50% plane integrals FP32 / 30% backprojection
@@ -21,6 +23,12 @@ x (e+1) shifts) and a backprojection (FP32) of a seeded Gaussian sinogram on the
 all in one persistent graph-captured launch (direction groups autotuned on the first call);
 the replay's time is compared with the same 96 jobs launched one by one, and with their
 plain FP64 forms.
+
+The "transpose" object is the FP32 exact transpose of the plane integrals on the same grid
+(rate, error against the plain FP64 transpose on a phi subset, and the dot-product test of
+the FP32 pair at full size); "rays" is the line-integral workload at 128^3 (128 z x 180 phi
+x 257 shifts: FP32 against FP64, the FP32 transpose with its dot-product test, and the FP64
+ray -> Radon reduction chain).
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
     python bench.py --traffic-probe     (run under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)
@@ -167,6 +175,99 @@ def batch_leg(ctx, wrt, nvol, seed=3):
             "speedup_vs_plain": plain / st["kernel_seconds"]}
 
 
+def transpose_leg(ctx, wrt, vol, fast, phi, th, s, stride):
+    """The FP32 exact transpose of the plane integrals at the same grid: its rate, its error
+    against the plain FP64 transpose on every stride-th phi, and the dot-product test
+    <A f, g> = <f, A^T g> of the FP32 pair at full size (f = the phantom, A f = the FP32
+    pass above, g = seeded Gaussian plane values)."""
+    n = vol.shape[0]
+    g = wrt.gaussian((len(s), len(phi), len(th)), seed=11)
+    j = ctx.job("radon_T", "f32", n=n, phi=phi, theta=th, shift=s)
+    j.load(g)
+    j.run()
+    st = j.run()
+    ATg = j.fetch().reshape(n, n, n)
+    j.close()
+    lhs = float(np.dot(fast.ravel().astype(np.float64), g.ravel()))
+    rhs = float(np.dot(vol.ravel(), ATg.ravel()))
+    dot_rel = abs(lhs - rhs) / (np.linalg.norm(fast) * np.linalg.norm(g))
+    sub = phi[::stride]
+    gs = np.ascontiguousarray(g[:, ::stride, :])
+    out = {}
+    for prec in ("f32", "f64"):
+        js = ctx.job("radon_T", prec, n=n, phi=sub, theta=th, shift=s)
+        js.load(gs)
+        sst = js.run()
+        out[prec] = (js.fetch(), sst["kernel_seconds"])
+        js.close()
+    return {"seconds": st["kernel_seconds"],
+            "samples_per_s": st["logical_samples"] / st["kernel_seconds"],
+            "dot_lhs": lhs, "dot_rhs": rhs, "dot_rel": dot_rel,
+            "subset": f"every {stride}th phi",
+            "rel_l2_vs_f64": rel_l2(out["f32"][0], out["f64"][0]),
+            "plain_f64_subset_s": out["f64"][1], "f32_subset_s": out["f32"][1]}
+
+
+def rays_leg(ctx, wrt):
+    """The line-integral workload at 128^3: 128 z-slices x 180 phi in [0, pi) x 257 shifts,
+    FP32 against the plain FP64 form; the FP32 exact transpose and the dot-product test of
+    the FP32 pair on seeded Gaussian f and g; and the FP64 reduction chain (rays on 128
+    slices x 64 phi in [0, 2 pi) x 129 shifts -> Radon 64 phi x 32 theta x 129 shifts)."""
+    n = 128
+    z = wrt.zslice_grid(128)
+    s, phi2 = wrt.ray_grid_init(257, 360)
+    phi = np.ascontiguousarray(phi2[:180])
+    vol = wrt.phantom(n, False, 10, 0, 0.0)
+    res = {}
+    for prec in ("f32", "f64"):
+        j = ctx.job("ray", prec, n=n, z=z, shift=s, phi=phi)
+        j.load(vol)
+        j.run()
+        st = j.run()
+        res[prec] = (j.fetch(), st)
+        j.close()
+    f = wrt.gaussian((n, n, n), seed=21)
+    g = wrt.gaussian((len(z), len(s), len(phi)), seed=22)
+    j = ctx.job("ray", "f32", n=n, z=z, shift=s, phi=phi)
+    j.load(f)
+    j.run()
+    Af = j.fetch()
+    j.close()
+    j = ctx.job("ray_T", "f32", n=n, z=z, shift=s, phi=phi)
+    j.load(g)
+    j.run()
+    tst = j.run()
+    ATg = j.fetch()
+    j.close()
+    lhs, rhs = float(np.dot(Af, g.ravel())), float(np.dot(f.ravel(), ATg))
+    # reduction chain (FP64 by design): rays on the Radon output's phi and shift grids
+    rs, rphi = wrt.ray_grid_init(129, 64)
+    rz = wrt.zslice_grid(128)
+    rj = ctx.job("ray", "f64", n=n, z=rz, shift=rs, phi=rphi)
+    rj.load(vol)
+    rst = rj.run()
+    rays = rj.fetch()
+    rj.close()
+    _, th, _, _ = ctx.radon_grid_init(64, 32, 129)
+    dj = ctx.job("reduction", "f64", phi=rphi, theta=th, shift=rs, z=rz, nz=len(rz))
+    dj.load(rays)
+    dst = dj.run()
+    dj.close()
+    f32, st32 = res["f32"]
+    f64, st64 = res["f64"]
+    return {"workload": "ray_grid 128^3, 128 z x 180 phi in [0,pi) x 257 shifts",
+            "f32_s": st32["kernel_seconds"], "f64_s": st64["kernel_seconds"],
+            "f32_samples_per_s": st32["logical_samples"] / st32["kernel_seconds"],
+            "f64_samples_per_s": st64["logical_samples"] / st64["kernel_seconds"],
+            "rel_l2_vs_f64": rel_l2(f32, f64),
+            "max_err_vs_f64": float(np.max(np.abs(f32 - f64)) / np.max(np.abs(f64))),
+            "transpose_f32_s": tst["kernel_seconds"],
+            "dot_rel": abs(lhs - rhs) / (np.linalg.norm(Af) * np.linalg.norm(g)),
+            "reduction": {"grid": "rays 128 z x 64 phi x 129 s -> 64 phi x 32 theta x 129 s",
+                          "rays_f64_s": rst["kernel_seconds"], "reduction_f64_s": dst["kernel_seconds"],
+                          "tau_steps_per_s": dst["logical_samples"] / dst["kernel_seconds"]}}
+
+
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "radon_fast_256_traffic.md")
 CALIB_COUNT = 1 << 27  # elements of the calibration conversion: 1 GiB read, 512 MiB written
 
@@ -230,6 +331,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=64)
     ap.add_argument("--serve-burst", type=int, default=32)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-transpose", action="store_true")
+    ap.add_argument("--no-rays", action="store_true")
     ap.add_argument("--batch-volumes", type=int, default=48)
     ap.add_argument("--traffic-probe", action="store_true")
     args = ap.parse_args()
@@ -288,6 +391,18 @@ def main():
     ms = wall / args.steps * 1e3
     value = logical / (wall / args.steps)
 
+    transpose = None
+    if not args.no_transpose:
+        log("bench: transpose")
+        transpose = transpose_leg(ctx, wrt, vol, fast, phi, th, s, args.plain_phi_stride)
+        log(f"bench: transpose {transpose['seconds']:.3f} s, dot rel {transpose['dot_rel']:.2e}, "
+            f"relL2 {transpose['rel_l2_vs_f64']:.2e}")
+    rays = None
+    if not args.no_rays:
+        log("bench: rays")
+        rays = rays_leg(ctx, wrt)
+        log(f"bench: rays f32 {rays['f32_s'] * 1e3:.2f} ms, f64 {rays['f64_s'] * 1e3:.2f} ms, "
+            f"dot rel {rays['dot_rel']:.2e}")
     cpu = None
     if not args.no_cpu:
         log("bench: cpu baseline")
@@ -349,6 +464,8 @@ def main():
         "cpu_baseline": cpu,
         "serve": serve,
         "batch": batch,
+        "transpose": transpose,
+        "rays": rays,
     }
     print(json.dumps(out), flush=True)
 
