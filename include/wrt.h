@@ -139,6 +139,55 @@ int wrt_radon_count_incube(wrt_ctx* ctx, int n, const double* phi, int nphi,
                            const double* theta, int ntheta, const double* shift, int nshift,
                            double* count);
 
+/* ------------------------------------------------------ analytic test functions --- */
+/* The analytic programs (radon_analytic, ray_analytic) integrate a test function that the
+ * reference compiles in from test_function.c.  Here the test function is a sum of terms
+ * evaluated on the GPU; every term vanishes outside a known ball, so the lattice points
+ * where the sum is 0 are skipped (they add exactly nothing to the reference's sums). */
+typedef enum {
+  /* exp(-1/(r2max - |x-c|^2)) where |x-c|^2 < r2max - eps, else 0; p[0] = r2max, p[1] = eps
+   * (radon_analytic/examples-test-functions/exponential_bump.c:5-16) */
+  WRT_FN_EXP_BUMP = 0,
+  /* sine_bump_template (radon_analytic/examples-test-functions/sinusoidal_bump.c:3-19);
+   * p[0] = amplitude, p[1] = rin, p[2] = rout */
+  WRT_FN_SINE_BUMP = 1,
+  /* solid ellipsoid (no reference counterpart; the phantom's ellipsoid): p[0] = density,
+   * p[1..3] = semi-axes, p[4..12] = rotation R row-major (columns are the principal axes);
+   * inside when |diag(1/a) R^T (x - c)| <= 1 */
+  WRT_FN_ELLIPSOID = 2
+} wrt_fn_kind;
+
+typedef struct wrt_fn_term {
+  int kind;    /* wrt_fn_kind */
+  double c[3]; /* centre */
+  double p[13];
+} wrt_fn_term;
+
+#define WRT_FN_MAX_TERMS 256
+
+/* The reference's example test functions as terms: "exponential_bump" (1 term) and
+ * "sinusoidal_bump" (3 terms).  Returns the number of terms (<= max_terms) or WRT_E_ARG. */
+int wrt_fn_example(const char* name, wrt_fn_term* terms, int max_terms);
+/* the ellipsoids of wrt_phantom(n, ., nell, seed, 0) as nell WRT_FN_ELLIPSOID terms */
+int wrt_fn_phantom(int nell, unsigned long long seed, wrt_fn_term* terms);
+/* z_i = (-1.0) + i*2.0/(ngrid-1): the analytic ray program's slices
+ * (ray_analytic/ray_analytic.c:255; rounds differently from wrt_zslice_grid) */
+int wrt_analytic_zslice_grid(int ngrid, double* z);
+
+/* Plane integrals of a test function over every (shift, phi, theta), FP64: replaces
+ * plane_integral (radon_analytic/src/integral.c:25-56) called from
+ * radon_analytic/src/radon_analytic.c:253-266.  delta = (2/(ngrid-1))/2 and the lattice
+ * i1, i2 in [-2 ngrid, 2 ngrid).  out[s][phi][theta]. */
+int wrt_radon_analytic(wrt_ctx* ctx, const wrt_fn_term* terms, int nterms, int ngrid,
+                       const double* phi, int nphi, const double* theta, int ntheta,
+                       const double* shift, int nshift, double* out, wrt_stats* st);
+/* Line integrals of a test function in z-slices, FP64: replaces ray_integral
+ * (ray_analytic/integral.c:10-47) called from ray_analytic/ray_analytic.c:253-272.
+ * i in [-2 ngrid, 2 ngrid].  out[z][s][phi]. */
+int wrt_ray_analytic(wrt_ctx* ctx, const wrt_fn_term* terms, int nterms, int ngrid,
+                     const double* z, int nz, const double* shift, int nshift,
+                     const double* phi, int nphi, double* out, wrt_stats* st);
+
 /* ---------------------------------------------- device-resident jobs (batch/serve) --- */
 /* A job owns its device input, output and tables; load/run/fetch may repeat.  The grids
  * passed at creation are copied.  Unused grid fields may be NULL/0 for an op:

@@ -701,6 +701,131 @@ double orc_ellipsoid_line(const double c[3], const double axes[3], const double 
 }
 
 /* ------------------------------------------------------------------------------------ */
+/* analytic programs                                                                    */
+/* ------------------------------------------------------------------------------------ */
+
+/* The example functions square with pow(., 2).  The reference builds them at -O0, where that
+ * is a call of the C library's pow, which is not correctly rounded everywhere; at -O2 gcc
+ * would fold it to x*x.  Calling through a volatile pointer keeps the library call. */
+static double (*volatile lib_pow)(double, double) = pow;
+#define pow2(v) lib_pow((v), 2)
+
+/* exponential_bump.c:5-16 with the centre and the radius terms as parameters */
+static double fn_exp_bump(const orc_fn_term* t, double x, double y, double z) {
+  const double xx = x - t->c[0], yy = y - t->c[1], zz = z - t->c[2];
+  double value;
+  if ((pow2(xx) + pow2(yy) + pow2(zz)) < (t->p[0] - t->p[1])) {
+    value = exp((-1) / (t->p[0] - pow2(xx) - pow2(yy) - pow2(zz)));
+  } else {
+    value = 0;
+  }
+  return value;
+}
+
+/* sine_bump_template : sinusoidal_bump.c:3-19 */
+static double fn_sine_bump(const orc_fn_term* t, double x, double y, double z) {
+  const double amplitude = t->p[0], rin = t->p[1], rout = t->p[2];
+  double r = sqrt(pow2(x - t->c[0]) + pow2(y - t->c[1]) + pow2(z - t->c[2]));
+  if (r > rout) return 0;
+  if ((r < rout) && (r > rin)) {
+    const double tt = M_PI * (r - rout) / (rout - rin);
+    return ((amplitude / (20 * M_PI)) * (-sin(6 * tt) / 3 + 3 * sin(4 * tt) - 15 * sin(2 * tt) + 20 * tt));
+  }
+  if (r < rin) return amplitude;
+  return 0;
+}
+
+/* the phantom's solid ellipsoid (build-owned): inside when |diag(1/a) R^T (x - c)| <= 1 */
+static double fn_ellipsoid(const orc_fn_term* t, double x, double y, double z) {
+  const double dx = x - t->c[0], dy = y - t->c[1], dz = z - t->c[2];
+  const double* R = t->p + 4;
+  const double u = (R[0] * dx + R[3] * dy + R[6] * dz) / t->p[1];
+  const double w = (R[1] * dx + R[4] * dy + R[7] * dz) / t->p[2];
+  const double v = (R[2] * dx + R[5] * dy + R[8] * dz) / t->p[3];
+  return (u * u + w * w + v * v <= 1.0) ? t->p[0] : 0.0;
+}
+
+/* the terms summed left to right, as sinusoidal_bump.c:32-34 sums its three bumps */
+double orc_fn_eval(const orc_fn_term* t, int nt, double x, double y, double z) {
+  double v = 0;
+  for (int k = 0; k < nt; ++k) {
+    double f = 0;
+    switch (t[k].kind) {
+      case 0: f = fn_exp_bump(t + k, x, y, z); break;
+      case 1: f = fn_sine_bump(t + k, x, y, z); break;
+      case 2: f = fn_ellipsoid(t + k, x, y, z); break;
+    }
+    v = k == 0 ? f : v + f;
+  }
+  return v;
+}
+
+/* plane_integral : radon_analytic/src/integral.c:25-56 (basis as radon_grid's, :9-22) */
+double orc_analytic_plane(const orc_fn_term* t, int nt, int ngrid, double phi, double theta,
+                          double shift) {
+  double nr[3], v1[3], v2[3];
+  orc_plane_basis(phi, theta, nr, v1, v2);
+  const double delta = (2.0 / (ngrid - 1)) / 2.0;
+  const int nsize = 4 * ngrid;
+  double integral = 0;
+  for (int i1 = -(nsize / 2); i1 < (nsize / 2); ++i1) {
+    for (int i2 = -(nsize / 2); i2 < (nsize / 2); ++i2) {
+      const double x = shift * nr[0] + delta * v1[0] * i1 + delta * v2[0] * i2;
+      const double y = shift * nr[1] + delta * v1[1] * i1 + delta * v2[1] * i2;
+      const double z = shift * nr[2] + delta * v1[2] * i1 + delta * v2[2] * i2;
+      const double value = orc_fn_eval(t, nt, x, y, z);
+      integral += value * delta * delta;
+    }
+  }
+  return integral;
+}
+
+/* ray_integral : ray_analytic/integral.c:10-47 */
+double orc_analytic_ray(const orc_fn_term* t, int nt, double zslice, double shift, double phi,
+                        int ngrid) {
+  const double nx = cos(phi), ny = sin(phi);
+  const double dx = cos(phi + M_PI / 2), dy = sin(phi + M_PI / 2), dz = 0;
+  const double px = shift * nx, py = shift * ny, pz = zslice;
+  const double delta = (2.0 / (ngrid - 1)) / 2.0;
+  const int nsize = 4 * ngrid;
+  double integral = 0;
+  for (int i = -(nsize / 2); i < (nsize / 2) + 1; ++i) {
+    const double x = px + delta * dx * i;
+    const double y = py + delta * dy * i;
+    const double z = pz + delta * dz * i;
+    const double value = orc_fn_eval(t, nt, x, y, z);
+    integral += value * delta;
+  }
+  return integral;
+}
+
+void orc_analytic_radon_grid(const orc_fn_term* t, int nt, int ngrid, const double* phi, int nphi,
+                             const double* theta, int ntheta, const double* shift, int nshift,
+                             double* out) {
+  const long long total = (long long)nshift * nphi * ntheta;
+#pragma omp parallel for schedule(dynamic, 4)
+  for (long long k = 0; k < total; ++k) {
+    const int it = (int)(k % ntheta), ip = (int)((k / ntheta) % nphi), is = (int)(k / ((long long)ntheta * nphi));
+    out[k] = orc_analytic_plane(t, nt, ngrid, phi[ip], theta[it], shift[is]);
+  }
+}
+
+void orc_analytic_ray_grid(const orc_fn_term* t, int nt, int ngrid, const double* z, int nz,
+                           const double* shift, int nshift, const double* phi, int nphi,
+                           double* out) {
+  const long long total = (long long)nz * nshift * nphi;
+#pragma omp parallel for schedule(dynamic, 16)
+  for (long long k = 0; k < total; ++k) {
+    const int ip = (int)(k % nphi), is = (int)((k / nphi) % nshift), iz = (int)(k / ((long long)nphi * nshift));
+    out[k] = orc_analytic_ray(t, nt, z[iz], shift[is], phi[ip], ngrid);
+  }
+}
+
+void orc_analytic_zslices(int ngrid, double* z) {
+  for (int i = 0; i < ngrid; ++i) z[i] = (-1.0) + i * 2.0 / (ngrid - 1);
+}
+
+/* ------------------------------------------------------------------------------------ */
 /* arithmetic property used by the fast kernels                                          */
 /* ------------------------------------------------------------------------------------ */
 

@@ -88,6 +88,37 @@ double orc_ellipsoid_plane(const double c[3], const double axes[3], const double
 double orc_ellipsoid_line(const double c[3], const double axes[3], const double rot[9],
                           double rho, const double p0[3], const double dir[3]);
 
+/* ---- analytic programs (radon_analytic, ray_analytic) ---- */
+/* A test function as a sum of terms; the layout equals the product's wrt_fn_term.
+ *   kind 0, exponential bump  (radon_analytic/examples-test-functions/exponential_bump.c:5-16):
+ *           p[0] = r2max, p[1] = eps: exp(-1/(r2max - |x-c|^2)) where |x-c|^2 < r2max - eps
+ *   kind 1, sine bump         (…/sinusoidal_bump.c:3-19 sine_bump_template):
+ *           p[0] = amplitude, p[1] = rin, p[2] = rout
+ *   kind 2, solid ellipsoid   (build-owned, the phantom's ellipsoid):
+ *           p[0] = density, p[1..3] = semi-axes, p[4..12] = rotation, row-major */
+typedef struct {
+  int kind;
+  double c[3];
+  double p[13];
+} orc_fn_term;
+double orc_fn_eval(const orc_fn_term* t, int nt, double x, double y, double z);
+/* plane_integral (radon_analytic/src/integral.c:25-56): delta = (2/(ngrid-1))/2, lattice
+ * i in [-2 ngrid, 2 ngrid), every sample evaluated, serial sum */
+double orc_analytic_plane(const orc_fn_term* t, int nt, int ngrid, double phi, double theta,
+                          double shift);
+/* ray_integral (ray_analytic/integral.c:10-47): i in [-2 ngrid, 2 ngrid] */
+double orc_analytic_ray(const orc_fn_term* t, int nt, double zslice, double shift, double phi,
+                        int ngrid);
+/* whole grids, out[s][phi][theta] and out[z][s][phi] */
+void orc_analytic_radon_grid(const orc_fn_term* t, int nt, int ngrid, const double* phi, int nphi,
+                             const double* theta, int ntheta, const double* shift, int nshift,
+                             double* out);
+void orc_analytic_ray_grid(const orc_fn_term* t, int nt, int ngrid, const double* z, int nz,
+                           const double* shift, int nshift, const double* phi, int nphi,
+                           double* out);
+/* z_i = (-1.0) + i*2.0/(ngrid-1) (ray_analytic/ray_analytic.c:255) */
+void orc_analytic_zslices(int ngrid, double* z);
+
 /* ---- arithmetic property of the fast kernels' division ---- */
 long long orc_div_rn_mismatches(int n, long long count, unsigned long long seed);
 

@@ -7,6 +7,9 @@ Sources of truth (the reference has no tests of its own):
                       radon_grid/src/{integral,interp}.c, compiled by oracle/Makefile into
                       oracle/_ref (never copied into this repository)
   ref_ray_grid.npz    ray_transform / cube_trilinear_interp from ray_grid/src/{integral,interp}.c
+  ref_analytic.npz    plane_integral / ray_integral / test_function of the analytic programs,
+                      from radon_analytic/src/integral.c and ray_analytic/integral.c compiled
+                      with each example test function (examples-test-functions/*.c)
   gl.npz              Gauss-Legendre nodes/weights, 50-digit mpmath, correctly rounded
   spline.npz          scipy CubicSpline(bc_type='natural') values
 Volumes are integer-formula volumes, rebuilt exactly by volume_formula() in the tests, so
@@ -83,6 +86,40 @@ def make_ref():
     np.savez_compressed(os.path.join(HERE, "ref_ray_grid.npz"), **out)
 
 
+def make_analytic():
+    """plane_integral / ray_integral / test_function of the analytic programs, from the
+    reference's radon_analytic/src/integral.c and ray_analytic/integral.c compiled with each
+    of its example test functions (oracle/_ref/libref_{radon,ray}_analytic_<example>.so)."""
+    from golden_inputs import EXAMPLES, analytic_plane_cases, analytic_points, analytic_ray_cases
+
+    out = {}
+    for name in EXAMPLES:
+        rl = C.CDLL(os.path.join(ROOT, "oracle", "_ref", f"libref_radon_analytic_{name}.so"))
+        yl = C.CDLL(os.path.join(ROOT, "oracle", "_ref", f"libref_ray_analytic_{name}.so"))
+        rl.plane_integral.restype = D
+        rl.plane_integral.argtypes = [C.c_void_p, C.c_int, D, D, D]
+        yl.ray_integral.restype = D
+        yl.ray_integral.argtypes = [C.c_void_p, D, D, D, C.c_int]
+        rl.test_function.restype = D
+        rl.test_function.argtypes = [D, D, D]
+        fr = C.cast(rl.test_function, C.c_void_p)
+        fy = C.cast(yl.test_function, C.c_void_p)
+        for key, (g, phi, theta, shift) in analytic_plane_cases().items():
+            out[f"{name}_plane_{key}"] = np.array(
+                [rl.plane_integral(fr, g, p, t, s) for p, t, s in zip(phi, theta, shift)])
+            out[f"{name}_plane_{key}_in"] = np.stack([phi, theta, shift])
+            out[f"{name}_plane_{key}_g"] = np.array(g)
+        for key, (g, z, s, phi) in analytic_ray_cases().items():
+            out[f"{name}_ray_{key}"] = np.array(
+                [yl.ray_integral(fy, a, b, c, g) for a, b, c in zip(z, s, phi)])
+            out[f"{name}_ray_{key}_in"] = np.stack([z, s, phi])
+            out[f"{name}_ray_{key}_g"] = np.array(g)
+        pts = analytic_points(name)
+        out[f"{name}_fn"] = np.array([rl.test_function(*q) for q in pts])
+        out[f"{name}_fn_in"] = np.array(pts)
+    np.savez_compressed(os.path.join(HERE, "ref_analytic.npz"), **out)
+
+
 def make_gl():
     import mpmath as mp
 
@@ -129,6 +166,7 @@ def make_spline():
 
 if __name__ == "__main__":
     make_ref()
+    make_analytic()
     make_gl()
     make_spline()
     for f in sorted(os.listdir(HERE)):

@@ -88,3 +88,66 @@ def point_cases(kind):
             pts.append((0.2, np.nextafter(a - 1.0, -2.0), np.nextafter(a - 1.0, 2.0)))
         cases[f"n{n}"] = (n, [tuple(float(c) for c in p) for p in pts])
     return cases
+
+
+# ------------------------------------------------------------------ analytic programs ---
+def example_terms(name):
+    """The reference's example test functions as term tables (kind, centre, parameters):
+    radon_analytic/examples-test-functions/exponential_bump.c:5-16 (kind 0: r2max, eps) and
+    sinusoidal_bump.c:22-35 (kind 1: amplitude, rin, rout, three bumps summed in order)."""
+    if name == "exponential_bump":
+        return [(0, (0.2, 0.0, 0.0), (0.5, 1e-2))]
+    if name == "sinusoidal_bump":
+        return [(1, (0.0, 0.2, 0.0), (1.0, 0.4, 0.6)),
+                (1, (-0.5, 0.0, 0.0), (0.6, 0.3, 0.45)),
+                (1, (0.0, 0.2, 0.5), (0.3, 0.4, 0.6))]
+    raise ValueError(name)
+
+
+EXAMPLES = ("exponential_bump", "sinusoidal_bump")
+
+
+def analytic_plane_cases():
+    """name -> (ngrid, phi[], theta[], shift[]) for the analytic plane_integral"""
+    rng = np.random.default_rng(21)
+    cases = {"g17": (17,) + _grid(8, 4, 9)}
+    p, t, s = _grid(16, 8, 33)
+    idx = rng.choice(len(p), 40, replace=False)
+    m = 24
+    cases["g33"] = (33, np.concatenate([p[idx], rng.uniform(0, 2 * math.pi, m)]),
+                    np.concatenate([t[idx], np.arccos(rng.uniform(-1, 1, m))]),
+                    np.concatenate([s[idx], rng.uniform(-1, 1, m)]))
+    return cases
+
+
+def analytic_zslices(ngrid):
+    """z_i = (-1.0) + i*2.0/(ngrid-1), as ray_analytic/ray_analytic.c:255 forms it"""
+    return np.array([(-1.0) + i * 2.0 / (ngrid - 1) for i in range(ngrid)])
+
+
+def analytic_ray_cases():
+    """name -> (ngrid, z[], shift[], phi[]) for the analytic ray_integral"""
+    rng = np.random.default_rng(22)
+    z = analytic_zslices(17)
+    s = -1.0 + np.arange(9) * (2.0 / 8)
+    ph = (2 * math.pi / 8) * np.arange(8)
+    Z, S, PH = np.meshgrid(z, s, ph, indexing="ij")
+    cases = {"g17": (17, Z.ravel(), S.ravel(), PH.ravel())}
+    m = 64
+    cases["g64r"] = (64, rng.uniform(-1, 1, m), rng.uniform(-1, 1, m), rng.uniform(0, 2 * math.pi, m))
+    return cases
+
+
+def analytic_points(name):
+    """points for test_function: random, near the centres and on the terms' radii"""
+    rng = np.random.default_rng(23)
+    pts = [tuple(rng.uniform(-1.2, 1.2, 3)) for _ in range(300)]
+    for kind, c, p in example_terms(name):
+        radii = [math.sqrt(p[0] - p[1]), math.sqrt(p[0])] if kind == 0 else [p[1], p[2]]
+        for r in radii + [0.0, 0.1]:
+            for _ in range(6):
+                d = rng.standard_normal(3)
+                d /= np.linalg.norm(d)
+                pts.append(tuple(float(c[k] + r * d[k]) for k in range(3)))
+        pts.append(tuple(float(v) for v in c))
+    return [tuple(float(v) for v in q) for q in pts]

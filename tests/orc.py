@@ -212,6 +212,76 @@ def ellipsoid_line(c, axes, rot, rho, p0, d):
     return L().orc_ellipsoid_line(_p(c), _p(axes), _p(rot), rho, _p(p0), _p(d))
 
 
+# ------------------------------------------------------------------ analytic programs ---
+class FnTerm(C.Structure):
+    """orc_fn_term (same layout as the product's wrt_fn_term)"""
+    _fields_ = [("kind", C.c_int), ("c", C.c_double * 3), ("p", C.c_double * 13)]
+
+
+def fn_terms(terms):
+    """[(kind, centre[3], params[...]), ...] -> ctypes array of FnTerm"""
+    arr = (FnTerm * max(1, len(terms)))()
+    for k, (kind, c, p) in enumerate(terms):
+        arr[k].kind = kind
+        for i in range(3):
+            arr[k].c[i] = c[i]
+        for i, v in enumerate(p):
+            arr[k].p[i] = v
+    return arr
+
+
+def _fn_lib():
+    l = L()
+    if not getattr(l, "_fn_ready", False):
+        l.orc_fn_eval.restype = D
+        l.orc_fn_eval.argtypes = [P, I, D, D, D]
+        l.orc_analytic_plane.restype = D
+        l.orc_analytic_plane.argtypes = [P, I, I, D, D, D]
+        l.orc_analytic_ray.restype = D
+        l.orc_analytic_ray.argtypes = [P, I, D, D, D, I]
+        l._fn_ready = True
+    return l
+
+
+def fn_eval(terms, x, y, z):
+    t = fn_terms(terms)
+    return _fn_lib().orc_fn_eval(C.cast(t, P), len(terms), x, y, z)
+
+
+def analytic_plane(terms, ngrid, phi, theta, shift):
+    t = fn_terms(terms)
+    return _fn_lib().orc_analytic_plane(C.cast(t, P), len(terms), ngrid, phi, theta, shift)
+
+
+def analytic_ray(terms, zslice, shift, phi, ngrid):
+    t = fn_terms(terms)
+    return _fn_lib().orc_analytic_ray(C.cast(t, P), len(terms), zslice, shift, phi, ngrid)
+
+
+def analytic_radon_grid(terms, ngrid, phi, theta, shift):
+    phi, theta, shift = map(_f, (phi, theta, shift))
+    t = fn_terms(terms)
+    out = np.empty((len(shift), len(phi), len(theta)))
+    _fn_lib().orc_analytic_radon_grid(C.cast(t, P), len(terms), ngrid, _p(phi), len(phi),
+                                      _p(theta), len(theta), _p(shift), len(shift), _p(out))
+    return out
+
+
+def analytic_ray_grid(terms, ngrid, z, shift, phi):
+    z, shift, phi = map(_f, (z, shift, phi))
+    t = fn_terms(terms)
+    out = np.empty((len(z), len(shift), len(phi)))
+    _fn_lib().orc_analytic_ray_grid(C.cast(t, P), len(terms), ngrid, _p(z), len(z), _p(shift),
+                                    len(shift), _p(phi), len(phi), _p(out))
+    return out
+
+
+def analytic_zslices(ngrid):
+    o = np.empty(ngrid)
+    L().orc_analytic_zslices(ngrid, _p(o))
+    return o
+
+
 def rel_l2(a, b):
     a = np.asarray(a, dtype=np.float64).ravel()
     b = np.asarray(b, dtype=np.float64).ravel()

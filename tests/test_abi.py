@@ -87,3 +87,45 @@ def test_init_without_device_reports_nodev():
         return
     assert rc == wrt.WRT_E_NODEV
     assert "device" in wrt.strerror(rc)
+
+
+def test_fn_examples_are_the_reference_example_functions():
+    from golden_inputs import EXAMPLES, example_terms
+
+    for name in EXAMPLES:
+        got = wrt.fn_example(name)
+        want = example_terms(name)
+        assert len(got) == len(want)
+        for (k, c, p), (k2, c2, p2) in zip(got, want):
+            assert k == k2 and tuple(c) == tuple(c2)
+            assert tuple(p[:len(p2)]) == tuple(p2) and not any(p[len(p2):])
+    try:
+        wrt.fn_example("no_such_function")
+        assert False
+    except wrt.WrtError as e:
+        assert e.code == wrt.WRT_E_ARG
+
+
+def test_fn_phantom_terms_are_the_phantom_ellipsoids():
+    P = wrt.phantom_params(10, seed=4)
+    T = wrt.fn_phantom(10, seed=4)
+    for e in range(10):
+        kind, c, p = T[e]
+        assert kind == wrt.FN["ellipsoid"]
+        assert np.array_equal(c, P[e, 0:3]) and p[0] == P[e, 15]
+        assert np.array_equal(p[1:4], P[e, 3:6]) and np.array_equal(p[4:13], P[e, 6:15])
+    # the same evaluation as the voxelised phantom at its cell centres
+    n = 24
+    vol = wrt.phantom(n, True, 10, 4, 0.0)
+    xs = -1.0 + (2.0 * np.arange(n) + 1.0) / n
+    rng = np.random.default_rng(0)
+    for _ in range(300):
+        i, j, k = rng.integers(n, size=3)
+        want = vol[i, j, k]
+        got = orc.fn_eval(T, xs[i], xs[j], xs[k]) if xs[i] ** 2 + xs[j] ** 2 + xs[k] ** 2 <= 1 else 0.0
+        assert got == want
+
+
+def test_analytic_zslices_match_oracle():
+    for g in (6, 17, 129):
+        assert np.array_equal(wrt.analytic_zslice_grid(g), orc.analytic_zslices(g))

@@ -63,3 +63,42 @@ def test_readers_report_short_and_malformed_inputs(tmp_path):
         f.write("0.1, 0.2, 0.3\n")
     r = run("radon_reduction", ["-p", "rcfg.txt", "-i", "rays.txt", "-o", "out.txt"], tmp_path)
     assert r.returncode != 0 and "matching failure" in r.stderr
+
+
+AN_PROGS = ["radon_analytic", "ray_analytic"]
+
+
+@pytest.mark.parametrize("prog", AN_PROGS)
+def test_analytic_programs_options(prog, tmp_path):
+    assert os.access(os.path.join(BIN, prog), os.X_OK)
+    r = run(prog, ["-h"], tmp_path)
+    assert r.returncode == 0 and "exponential_bump" in r.stdout
+    r = run(prog, [], tmp_path)
+    assert r.returncode != 0 and "arguments required" in r.stderr
+    # the test function: exactly one of -F / -f
+    r = run(prog, ["-p", "cfg.txt", "-o", "out.txt"], tmp_path)
+    assert r.returncode != 0 and "-F" in r.stderr
+    r = run(prog, ["-p", "cfg.txt", "-o", "out.txt", "-F", "exponential_bump", "-f", "t.txt"], tmp_path)
+    assert r.returncode != 0
+    r = run(prog, ["-p", "cfg.txt", "-o", "out.txt", "-F", "no_such_bump"], tmp_path)
+    assert r.returncode != 0 and "no_such_bump" in r.stderr
+
+
+@pytest.mark.parametrize("prog", AN_PROGS)
+def test_analytic_term_files_and_configs(prog, tmp_path):
+    with open(tmp_path / "bad.txt", "w") as f:
+        f.write("# comment\nexp_bump 0.2 0 0 0.5\n")  # one number short
+    r = run(prog, ["-p", "cfg.txt", "-o", "out.txt", "-f", "bad.txt"], tmp_path)
+    assert r.returncode != 0 and "takes 5 numbers" in r.stderr
+    with open(tmp_path / "unk.txt", "w") as f:
+        f.write("cube 0 0 0 1\n")
+    r = run(prog, ["-p", "cfg.txt", "-o", "out.txt", "-f", "unk.txt"], tmp_path)
+    assert r.returncode != 0 and "unknown term" in r.stderr
+    with open(tmp_path / "ok.txt", "w") as f:
+        f.write("exp_bump 0.2 0 0 0.5 0.01  # the reference's example\n\n"
+                "sine_bump 0 0.2 0 1.0 0.4 0.6\n")
+    # a config with too few lines fails after the test function was read
+    with open(tmp_path / "short.txt", "w") as f:
+        f.write("8 a\n")
+    r = run(prog, ["-p", "short.txt", "-o", "out.txt", "-f", "ok.txt"], tmp_path)
+    assert r.returncode != 0 and "2 terms" in r.stdout and "end of file" in r.stderr

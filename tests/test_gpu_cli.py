@@ -105,3 +105,44 @@ def test_serve_program(tmp_path):
     import json
     rep = json.loads(r.stdout.strip().split("\n")[-1])
     assert rep["requests"] == 12 and rep["p50_ms"] <= rep["p99_ms"] and rep["samples"] > 0
+
+
+def test_radon_analytic_program(tmp_path):
+    from golden_inputs import example_terms
+
+    nphi, nth, ns, g = 8, 4, 9, 17
+    write_config(tmp_path / "cfg.txt", [nphi, nth, ns, g])
+    r = run("radon_analytic", ["-p", "cfg.txt", "-o", "r.txt", "-F", "sinusoidal_bump"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    t = read_table(tmp_path / "r.txt", 4)
+    phi = orc.phi_grid(nphi)
+    th, _ = orc.theta_grid(nth)
+    s = orc.shift_grid(ns)
+    S, P, T = np.meshgrid(s, phi, th, indexing="ij")
+    assert np.allclose(t[:, 0], S.ravel(), atol=5e-7) and np.allclose(t[:, 2], T.ravel(), atol=5e-7)
+    want = orc.analytic_radon_grid(example_terms("sinusoidal_bump"), g, phi, th, s)
+    _same_text(_col(tmp_path / "r.txt", 4), lf(want))
+    # the same function from a term file
+    with open(tmp_path / "terms.txt", "w") as f:
+        for kind, c, p in example_terms("sinusoidal_bump"):
+            f.write("sine_bump " + " ".join(repr(v) for v in list(c) + list(p)) + "\n")
+    r = run("radon_analytic", ["-p", "cfg.txt", "-o", "r2.txt", "-f", "terms.txt"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    assert open(tmp_path / "r2.txt").read() == open(tmp_path / "r.txt").read()
+
+
+def test_ray_analytic_program(tmp_path):
+    from golden_inputs import analytic_zslices, example_terms
+
+    ns, nphi, g = 9, 8, 17
+    write_config(tmp_path / "cfg.txt", [ns, nphi, g])
+    r = run("ray_analytic", ["-p", "cfg.txt", "-o", "p.txt", "-F", "exponential_bump"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    t = read_table(tmp_path / "p.txt", 4)
+    z = analytic_zslices(g)
+    s, phi = orc.shift_grid(ns), orc.phi_grid(nphi)
+    Z, S, P = np.meshgrid(z, s, phi, indexing="ij")
+    assert np.allclose(t[:, 0], Z.ravel(), atol=5e-7) and np.allclose(t[:, 2], P.ravel(), atol=5e-7)
+    want = orc.analytic_ray_grid(example_terms("exponential_bump"), g, z, s, phi)
+    _same_text(_col(tmp_path / "p.txt", 4), lf(want))
+    assert "[z], [s], [phi], [Pf]" in open(tmp_path / "readme_p.txt").read()

@@ -292,3 +292,113 @@ def test_corrected_reciprocal_division_is_ieee():
     for n in list(range(3, 1025)):
         if n & (n - 1):
             assert f(n, 2000, n) == 0, n
+
+
+# ----------------------------------------------------------------- analytic programs ---
+from golden_inputs import EXAMPLES, analytic_zslices, example_terms  # noqa: E402
+
+
+def test_analytic_test_functions_match_reference_bits():
+    f = _npz("ref_analytic.npz")
+    for name in EXAMPLES:
+        terms = example_terms(name)
+        pts = f[name + "_fn_in"]
+        got = np.array([orc.fn_eval(terms, *p) for p in pts])
+        assert np.array_equal(got, f[name + "_fn"]), name
+        assert np.count_nonzero(got) > 20
+
+
+def test_analytic_plane_integral_matches_reference_bits():
+    f = _npz("ref_analytic.npz")
+    for name in EXAMPLES:
+        terms = example_terms(name)
+        keys = [k for k in f.files if k.startswith(name + "_plane_") and not k.endswith(("_in", "_g"))]
+        assert len(keys) == 2
+        for k in keys:
+            g = int(f[k + "_g"])
+            phi, theta, shift = f[k + "_in"]
+            got = np.array([orc.analytic_plane(terms, g, p, t, s) for p, t, s in zip(phi, theta, shift)])
+            assert np.array_equal(got, f[k]), k
+            assert np.count_nonzero(got) > len(got) // 4
+
+
+def test_analytic_ray_integral_matches_reference_bits():
+    f = _npz("ref_analytic.npz")
+    for name in EXAMPLES:
+        terms = example_terms(name)
+        keys = [k for k in f.files if k.startswith(name + "_ray_") and not k.endswith(("_in", "_g"))]
+        assert len(keys) == 2
+        for k in keys:
+            g = int(f[k + "_g"])
+            z, s, phi = f[k + "_in"]
+            got = np.array([orc.analytic_ray(terms, a, b, c, g) for a, b, c in zip(z, s, phi)])
+            assert np.array_equal(got, f[k]), k
+
+
+def test_analytic_zslices_follow_their_own_rounding():
+    # ray_analytic forms z = -1 + (i*2.0)/(ngrid-1), ray_grid z = -1 + i*(2.0/(n-1)): the two
+    # grids differ in the last bit for some i, and each program keeps its own
+    for g in (6, 17, 33, 100, 129, 257):
+        assert np.array_equal(orc.analytic_zslices(g), analytic_zslices(g))
+    diffs = sum(np.count_nonzero(orc.analytic_zslices(g) != orc.zslice_grid(g)) for g in (6, 7, 11, 100))
+    assert diffs > 0
+
+
+def _exp_bump_plane(r2max, eps, d):
+    """pi * int_eps^A exp(-1/t) dt, A = r2max - d^2: the bump's plane integral at distance d
+    (t e^(-1/t) - E1(1/t) is an antiderivative of e^(-1/t))"""
+    from scipy.special import exp1
+
+    a = r2max - d * d
+    if a <= eps:
+        return 0.0
+    F = lambda t: t * math.exp(-1.0 / t) - exp1(1.0 / t)  # noqa: E731
+    return math.pi * (F(a) - F(eps))
+
+
+def _exp_bump_line(r2max, eps, d):
+    from scipy.integrate import quad
+
+    a = r2max - d * d
+    if a <= eps:
+        return 0.0
+    L = math.sqrt(a - eps)
+    return quad(lambda u: math.exp(-1.0 / (a - u * u)), -L, L, epsabs=1e-16, epsrel=1e-13,
+                limit=200)[0]
+
+
+def test_analytic_exponential_bump_closed_forms():
+    # The bump is smooth and compactly supported, so the lattice sums converge faster than
+    # any power of delta: at ngrid = 65 they equal the exact integrals to ~1e-11 (plane,
+    # via the exponential integral E1) and ~1e-11 (line, via quadrature).
+    terms = example_terms("exponential_bump")
+    c = np.array(terms[0][1])
+    r2max, eps = terms[0][2]
+    rng = np.random.default_rng(5)
+    peak_p, peak_l = _exp_bump_plane(r2max, eps, 0.0), _exp_bump_line(r2max, eps, 0.0)
+    for _ in range(8):
+        phi, th, s = rng.uniform(0, 2 * math.pi), math.acos(rng.uniform(-1, 1)), rng.uniform(-0.6, 0.6)
+        n = np.array([math.sin(th) * math.cos(phi), math.sin(th) * math.sin(phi), math.cos(th)])
+        got = orc.analytic_plane(terms, 65, phi, th, s)
+        assert abs(got - _exp_bump_plane(r2max, eps, abs(c @ n - s))) <= 1e-10 * peak_p
+        z, s, phi = rng.uniform(-0.5, 0.5), rng.uniform(-0.5, 0.5), rng.uniform(0, 2 * math.pi)
+        p0 = np.array([s * math.cos(phi), s * math.sin(phi), z])
+        dv = np.array([math.cos(phi + math.pi / 2), math.sin(phi + math.pi / 2), 0.0])
+        w = c - p0
+        d = math.sqrt(max(w @ w - (w @ dv) ** 2, 0.0))
+        got = orc.analytic_ray(terms, z, s, phi, 65)
+        assert abs(got - _exp_bump_line(r2max, eps, d)) <= 1e-10 * peak_l
+
+
+def test_analytic_ellipsoid_term_closed_form():
+    # the indicator has a jump, so the lattice sum converges like delta: 1% at ngrid = 129
+    rot = [0.36, -0.48, 0.8, 0.8, 0.6, 0.0, -0.48, 0.64, 0.6]
+    terms = [(2, (0.1, 0.05, -0.1), [0.7, 0.5, 0.3, 0.2] + rot)]
+    rng = np.random.default_rng(6)
+    scale = 0.7 * math.pi * 0.5 * 0.3
+    for _ in range(6):
+        phi, th, s = rng.uniform(0, 2 * math.pi), math.acos(rng.uniform(-1, 1)), rng.uniform(-0.3, 0.3)
+        n = np.array([math.sin(th) * math.cos(phi), math.sin(th) * math.sin(phi), math.cos(th)])
+        want = orc.ellipsoid_plane(np.array(terms[0][1]), np.array([0.5, 0.3, 0.2]),
+                                   np.array(rot), 0.7, n, s)
+        assert abs(orc.analytic_plane(terms, 129, phi, th, s) - want) <= 1e-2 * scale

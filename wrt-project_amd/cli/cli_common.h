@@ -108,15 +108,15 @@ static int cli_parse(int argc, char** argv, cli_opts* o, int with_small) {
   return 0;
 }
 
-/* 4 lines "%d %[^\n]\n": returns 0, errno (open failure) or -1 (parse failure) */
-static int cli_params(const char* fn, int p[4]) {
+/* count lines "%d %[^\n]\n": returns 0, errno (open failure) or -1 (parse failure) */
+static int cli_params_n(const char* fn, int* p, int count) {
   FILE* f = fopen(fn, "r");
   if (!f) {
     fprintf(stderr, "Recieved call %s while opening %s. \n", strerror(errno), fn);
     return errno;
   }
   char buf[CLI_LINE * 8];
-  for (int k = 0; k < 4; ++k) {
+  for (int k = 0; k < count; ++k) {
     int cnt = fscanf(f, "%d %1023[^\n]\n", &p[k], buf);
     if (cnt == EOF) {
       fprintf(stderr, "Error: fscanf parameters %d : reached end of file before it was expected. \n", k);
@@ -131,6 +131,9 @@ static int cli_params(const char* fn, int p[4]) {
   fclose(f);
   return 0;
 }
+
+/* the grid programs' 4-line config */
+static int cli_params(const char* fn, int p[4]) { return cli_params_n(fn, p, 4); }
 
 /* count values "%lf\n" (cols = 1) or "%lf, %lf, %lf, %lf\n" keeping the last (cols = 4) */
 static int cli_read_values(const char* fn, size_t count, int cols, double* out) {
@@ -192,6 +195,134 @@ static void cli_readme(const char* out, const char* what, const char* format, co
   fprintf(f, "  Parameters of the data:\n");
   for (int k = 0; k < nv; ++k) fprintf(f, "    %s: %d%s\n", names[k], vals[k], k + 1 < nv ? "," : ".");
   fclose(f);
+}
+
+/* ------------------------------------------------------------ analytic programs --- */
+/* Options of radon_analytic / ray_analytic: the reference's -h -p -o -n (no input file: the
+ * reference compiles its test function in), plus the test function, which here is chosen at
+ * run time: -F <example name> (the reference's example files) or -f <term file>. */
+typedef struct {
+  const char* params;
+  const char* output;
+  const char* example;
+  const char* terms_file;
+  int nthreads;
+  int device;
+} cli_an_opts;
+
+static void cli_an_usage(FILE* f, const char* prog) {
+  fprintf(f, "Usage: %s -p (filename) -o (filename) -n (number) (-F name | -f filename)\n", prog);
+  fprintf(f,
+          "   -h --help                 Display this usage information.\n"
+          "   -p --parameters filename  Read the grid configuration from file.\n"
+          "   -o --output filename      Write output data to the file.\n"
+          "   -n --nthreads number      Accepted for compatibility (work runs on the GPU).\n"
+          "   -F --function name        Test function: exponential_bump or sinusoidal_bump\n"
+          "                             (the reference's example test functions).\n"
+          "   -f --terms filename       Test function as a sum of terms, one per line:\n"
+          "                               exp_bump  cx cy cz r2max eps\n"
+          "                               sine_bump cx cy cz amplitude rin rout\n"
+          "                               ellipsoid cx cy cz density a1 a2 a3 r11 r12 ... r33\n"
+          "                             ('#' starts a comment).\n"
+          "   -d --device number        HIP device (default 0).\n");
+}
+
+static void cli_an_parse(int argc, char** argv, cli_an_opts* o) {
+  memset(o, 0, sizeof(*o));
+  o->nthreads = 1;
+  if (argc == 1) {
+    fprintf(stderr, "%s: arguments required. Try '-h' or '--help'.\n", argv[0]);
+    exit(EXIT_FAILURE);
+  }
+  const struct option longs[] = {{"help", no_argument, 0, 'h'},
+                                 {"parameters", required_argument, 0, 'p'},
+                                 {"output", required_argument, 0, 'o'},
+                                 {"nthreads", required_argument, 0, 'n'},
+                                 {"function", required_argument, 0, 'F'},
+                                 {"terms", required_argument, 0, 'f'},
+                                 {"device", required_argument, 0, 'd'},
+                                 {0, 0, 0, 0}};
+  int c;
+  while ((c = getopt_long(argc, argv, "hp:o:n:F:f:d:", longs, NULL)) != -1) {
+    switch (c) {
+      case 'h': cli_an_usage(stdout, argv[0]); exit(EXIT_SUCCESS);
+      case 'p': o->params = optarg; printf("Parameters file: %s\n", optarg); break;
+      case 'o': o->output = optarg; printf("Output file: %s\n", optarg); break;
+      case 'n':
+        o->nthreads = atoi(optarg);
+        if (o->nthreads <= 0) {
+          fprintf(stderr, "%s: the number of threads must be positive\n", argv[0]);
+          exit(EXIT_FAILURE);
+        }
+        printf("Number of threads: %d\n", o->nthreads);
+        break;
+      case 'F': o->example = optarg; break;
+      case 'f': o->terms_file = optarg; break;
+      case 'd': o->device = atoi(optarg); break;
+      default: cli_an_usage(stderr, argv[0]); exit(EXIT_FAILURE);
+    }
+  }
+  if (optind < argc) printf("Warning: non-option ARGV-arguments are omitted. See '-h' or '--help'.\n");
+  if (!o->params || !o->output || (!o->example == !o->terms_file)) {
+    fprintf(stderr, "%s: -p, -o and one of -F / -f are required\n", argv[0]);
+    exit(EXIT_FAILURE);
+  }
+}
+
+/* a term file (format in cli_an_usage): returns the number of terms or -1 */
+static int cli_read_terms(const char* fn, wrt_fn_term* t, int max) {
+  FILE* f = fopen(fn, "r");
+  if (!f) {
+    fprintf(stderr, "Recieved call %s while opening %s. \n", strerror(errno), fn);
+    return -1;
+  }
+  char line[1024];
+  int n = 0, lineno = 0;
+  while (fgets(line, sizeof(line), f)) {
+    ++lineno;
+    char* h = strchr(line, '#');
+    if (h) *h = 0;
+    char kind[32];
+    int used = 0;
+    if (sscanf(line, " %31s%n", kind, &used) != 1) continue; /* blank */
+    int want;
+    wrt_fn_term* q = t + n;
+    memset(q, 0, sizeof(*q));
+    if (!strcmp(kind, "exp_bump")) { q->kind = WRT_FN_EXP_BUMP; want = 5; }
+    else if (!strcmp(kind, "sine_bump")) { q->kind = WRT_FN_SINE_BUMP; want = 6; }
+    else if (!strcmp(kind, "ellipsoid")) { q->kind = WRT_FN_ELLIPSOID; want = 16; }
+    else { fprintf(stderr, "%s:%d: unknown term '%s'\n", fn, lineno, kind); fclose(f); return -1; }
+    if (n >= max) { fprintf(stderr, "%s: more than %d terms\n", fn, max); fclose(f); return -1; }
+    double v[16];
+    char* s = line + used;
+    int got = 0;
+    for (; got < want; ++got) {
+      int u = 0;
+      if (sscanf(s, " %lf%n", &v[got], &u) != 1) break;
+      s += u;
+    }
+    if (got != want) {
+      fprintf(stderr, "%s:%d: '%s' takes %d numbers\n", fn, lineno, kind, want);
+      fclose(f);
+      return -1;
+    }
+    for (int k = 0; k < 3; ++k) q->c[k] = v[k];
+    for (int k = 3; k < want; ++k) q->p[k - 3] = v[k];
+    ++n;
+  }
+  fclose(f);
+  return n;
+}
+
+static int cli_an_terms(const cli_an_opts* o, wrt_fn_term* t, int max) {
+  const int n = o->example ? wrt_fn_example(o->example, t, max) : cli_read_terms(o->terms_file, t, max);
+  if (n < 0) {
+    fprintf(stderr, "Aborting : unknown or malformed test function %s\n",
+            o->example ? o->example : o->terms_file);
+    exit(EXIT_FAILURE);
+  }
+  printf("Test function: %s (%d term%s)\n", o->example ? o->example : o->terms_file, n, n == 1 ? "" : "s");
+  return n;
 }
 
 #endif

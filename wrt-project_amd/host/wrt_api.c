@@ -144,7 +144,13 @@ int wrt_zslice_grid(int nslices, double* z) {
  * convention), then a = delta*v1, b = delta*v2: radon_grid/src/integral.c:10-36,47-49 */
 void wrt_host_radon_dirs(int n, const double* phi, int nphi, const double* theta, int ntheta,
                          double* dirs) {
-  const double delta = (1.0 / (n));
+  wrt_host_plane_dirs((1.0 / (n)), phi, nphi, theta, ntheta, dirs);
+}
+
+/* the same basis for a lattice step delta (radon_analytic/src/integral.c:9-22,36-45 forms
+ * it identically with delta = (2/(ngrid-1))/2) */
+void wrt_host_plane_dirs(double delta, const double* phi, int nphi, const double* theta,
+                         int ntheta, double* dirs) {
   for (int ip = 0; ip < nphi; ++ip)
     for (int it = 0; it < ntheta; ++it) {
       const double p = phi[ip], t = theta[it];

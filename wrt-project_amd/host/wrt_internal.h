@@ -70,6 +70,8 @@ int wrt_fast_describe(wrt_job* j, wrt_fast_desc* fd);
 /* host-side tables (C library sin/cos, the reference's bits) */
 void wrt_host_radon_dirs(int n, const double* phi, int nphi, const double* theta, int ntheta,
                          double* dirs);
+void wrt_host_plane_dirs(double delta, const double* phi, int nphi, const double* theta,
+                         int ntheta, double* dirs);
 void wrt_host_ray_trig(const double* phi, int nphi, double* trig);
 
 int wrt_hip(hipError_t e);

@@ -81,7 +81,54 @@ ABI_SYMBOLS = [
     "wrt_phantom_params", "wrt_gaussian", "wrt_radon_count_incube", "wrt_job_launch",
     "wrt_job_poll", "wrt_job_wait", "wrt_serve_describe", "wrt_serve_grids", "wrt_serve_desc",
     "wrt_serve_input", "wrt_serve_run", "wrt_batch_dgroup",
+    "wrt_fn_example", "wrt_fn_phantom", "wrt_analytic_zslice_grid", "wrt_radon_analytic",
+    "wrt_ray_analytic",
 ]
+
+FN = {"exp_bump": 0, "sine_bump": 1, "ellipsoid": 2}
+FN_MAX_TERMS = 256
+
+
+class FnTerm(C.Structure):
+    """wrt_fn_term: kind, centre, 13 parameters"""
+    _fields_ = [("kind", C.c_int), ("c", C.c_double * 3), ("p", C.c_double * 13)]
+
+
+def fn_terms(terms):
+    """[(kind, centre[3], params[...]), ...] (kind an int or a FN key) -> ctypes array"""
+    arr = (FnTerm * max(1, len(terms)))()
+    for k, (kind, c, p) in enumerate(terms):
+        arr[k].kind = FN[kind] if isinstance(kind, str) else int(kind)
+        for i in range(3):
+            arr[k].c[i] = c[i]
+        for i, v in enumerate(p):
+            arr[k].p[i] = v
+    return arr
+
+
+def _terms_list(arr, count):
+    return [(arr[k].kind, tuple(arr[k].c), tuple(arr[k].p)) for k in range(count)]
+
+
+def fn_example(name):
+    """the reference's example test function as terms"""
+    arr = (FnTerm * 8)()
+    rc = lib().wrt_fn_example(name.encode(), arr, 8)
+    if rc < 0:
+        raise WrtError(rc, "wrt_fn_example")
+    return _terms_list(arr, rc)
+
+
+def fn_phantom(nell, seed=0):
+    arr = (FnTerm * max(1, nell))()
+    _chk(lib().wrt_fn_phantom(nell, seed, arr), "wrt_fn_phantom")
+    return _terms_list(arr, nell)
+
+
+def analytic_zslice_grid(ngrid):
+    o = np.empty(ngrid)
+    _chk(lib().wrt_analytic_zslice_grid(ngrid, _p(o)), "wrt_analytic_zslice_grid")
+    return o
 
 
 def lib():
@@ -136,6 +183,11 @@ def lib():
         L.wrt_serve_desc.argtypes = [P, P, P, P, P, P, P]
         L.wrt_serve_input.argtypes = [P, P]
         L.wrt_serve_run.argtypes = [P, P, P, P, P]
+        L.wrt_fn_example.argtypes = [C.c_char_p, P, I]
+        L.wrt_fn_phantom.argtypes = [I, C.c_ulonglong, P]
+        L.wrt_analytic_zslice_grid.argtypes = [I, P]
+        L.wrt_radon_analytic.argtypes = [P, P, I, I, P, I, P, I, P, I, P, P]
+        L.wrt_ray_analytic.argtypes = [P, P, I, I, P, I, P, I, P, I, P, P]
         _lib = L
     return _lib
 
@@ -279,6 +331,27 @@ class Context:
         _chk(lib().wrt_ray_grid_T(self.h, _p(g), n, _p(z), len(z), _p(shift), len(shift),
                                   _p(phi), len(phi), PREC[prec], _p(out), C.byref(st)),
              "wrt_ray_grid_T")
+        return out, st.as_dict()
+
+    def radon_analytic(self, terms, ngrid, phi, theta, shift):
+        """plane integrals of the test function `terms` (list of (kind, c, p)), FP64"""
+        phi, theta, shift = map(_f64, (phi, theta, shift))
+        t = fn_terms(terms)
+        out = np.empty((len(shift), len(phi), len(theta)))
+        st = Stats()
+        _chk(lib().wrt_radon_analytic(self.h, t, len(terms), ngrid, _p(phi), len(phi), _p(theta),
+                                      len(theta), _p(shift), len(shift), _p(out), C.byref(st)),
+             "wrt_radon_analytic")
+        return out, st.as_dict()
+
+    def ray_analytic(self, terms, ngrid, z, shift, phi):
+        z, shift, phi = map(_f64, (z, shift, phi))
+        t = fn_terms(terms)
+        out = np.empty((len(z), len(shift), len(phi)))
+        st = Stats()
+        _chk(lib().wrt_ray_analytic(self.h, t, len(terms), ngrid, _p(z), len(z), _p(shift),
+                                    len(shift), _p(phi), len(phi), _p(out), C.byref(st)),
+             "wrt_ray_analytic")
         return out, st.as_dict()
 
     def radon_count_incube(self, n, phi, theta, shift):
