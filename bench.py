@@ -28,7 +28,9 @@ The "transpose" object is the FP32 exact transpose of the plane integrals on the
 (rate, error against the plain FP64 transpose on a phi subset, and the dot-product test of
 the FP32 pair at full size); "rays" is the line-integral workload at 128^3 (128 z x 180 phi
 x 257 shifts: FP32 against FP64, the FP32 transpose with its dot-product test, and the FP64
-ray -> Radon reduction chain).
+ray -> Radon reduction chain); "analytic" is radon_analytic on its own example configuration
+(256 phi x 128 theta x 129 shifts, ngrid 129, the exponential-bump example function) beside
+the reference's compiled plane_integral on the host cores.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
     python bench.py --traffic-probe     (run under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)
@@ -268,6 +270,62 @@ def rays_leg(ctx, wrt):
                           "tau_steps_per_s": dst["logical_samples"] / dst["kernel_seconds"]}}
 
 
+def analytic_leg(ctx, wrt, seconds=6.0):
+    """radon_analytic on the reference's example configuration (radon_analytic/binary/
+    config.txt: 256 phi x 128 theta x 129 shifts, ngrid 129) with its exponential-bump example
+    function, FP64 on the GPU; beside it the reference's own plane_integral compiled from its
+    sources with that example function (oracle/_ref), on seeded planes over the host cores,
+    whose values also give the GPU's error against the reference."""
+    import ctypes as C
+    from concurrent.futures import ThreadPoolExecutor
+
+    g = 129
+    phi, th, w, s = ctx.radon_grid_init(256, 128, 129)
+    terms = wrt.fn_example("exponential_bump")
+    ctx.radon_analytic(terms, g, phi, th, s)
+    out, st = ctx.radon_analytic(terms, g, phi, th, s)
+    res = {"workload": "radon_analytic, exponential bump, 256 phi x 128 theta x 129 s, ngrid 129",
+           "seconds": st["kernel_seconds"],
+           "samples_per_s": st["logical_samples"] / st["kernel_seconds"]}
+    lib_path = os.path.join(ROOT, "oracle", "_ref", "libref_radon_analytic_exponential_bump.so")
+    if not os.path.exists(lib_path):
+        return res
+    lib = C.CDLL(lib_path)
+    lib.plane_integral.restype = C.c_double
+    lib.plane_integral.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double]
+    fptr = C.cast(lib.test_function, C.c_void_p)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    rng = np.random.default_rng(77)
+
+    def run(idx):
+        return [lib.plane_integral(fptr, g, float(phi[i]), float(th[j]), float(s[k])) for k, i, j in idx]
+
+    m = 2 * threads
+    while True:
+        idx = list(zip(rng.integers(len(s), size=m), rng.integers(len(phi), size=m),
+                       rng.integers(len(th), size=m)))
+        parts = [idx[t::threads] for t in range(threads)]
+        with ThreadPoolExecutor(threads) as ex:
+            t0 = time.perf_counter()
+            vals = list(ex.map(run, parts))
+            dt = time.perf_counter() - t0
+        if dt >= 0.8 * seconds or m >= 1 << 20:
+            break
+        m = max(m + threads, int(m * min(16.0, 1.1 * seconds / max(dt, 1e-3))) // threads * threads)
+    ref = np.array([v for t in range(threads) for v in vals[t]])
+    got = np.array([out[k, i, j] for t in range(threads) for (k, i, j) in parts[t]])
+    cpu_rate = m * 16.0 * g * g / dt
+    res.update({"cpu_baseline": {"value": cpu_rate, "unit": "plane samples/s", "cores": threads,
+                                 "kind": "reference",
+                                 "sample": f"{m} seeded planes, {dt:.1f} s; the reference's "
+                                           "plane_integral with its example exponential_bump, "
+                                           "built from its sources at -O0"},
+                "speedup_vs_cpu": res["samples_per_s"] / cpu_rate,
+                "rel_l2_vs_reference": rel_l2(got, ref),
+                "max_err_vs_reference": float(np.max(np.abs(got - ref)) / np.max(np.abs(ref)))})
+    return res
+
+
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "radon_fast_256_traffic.md")
 CALIB_COUNT = 1 << 27  # elements of the calibration conversion: 1 GiB read, 512 MiB written
 
@@ -333,6 +391,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-transpose", action="store_true")
     ap.add_argument("--no-rays", action="store_true")
+    ap.add_argument("--no-analytic", action="store_true")
     ap.add_argument("--batch-volumes", type=int, default=48)
     ap.add_argument("--traffic-probe", action="store_true")
     args = ap.parse_args()
@@ -403,6 +462,11 @@ def main():
         rays = rays_leg(ctx, wrt)
         log(f"bench: rays f32 {rays['f32_s'] * 1e3:.2f} ms, f64 {rays['f64_s'] * 1e3:.2f} ms, "
             f"dot rel {rays['dot_rel']:.2e}")
+    analytic = None
+    if not args.no_analytic:
+        log("bench: analytic")
+        analytic = analytic_leg(ctx, wrt, seconds=0.0 if args.no_cpu else 6.0)
+        log(f"bench: analytic {analytic['seconds']:.3f} s")
     cpu = None
     if not args.no_cpu:
         log("bench: cpu baseline")
@@ -466,6 +530,7 @@ def main():
         "batch": batch,
         "transpose": transpose,
         "rays": rays,
+        "analytic": analytic,
     }
     print(json.dumps(out), flush=True)
 
