@@ -139,6 +139,16 @@ int wrt_radon_count_incube(wrt_ctx* ctx, int n, const double* phi, int nphi,
                            const double* theta, int ntheta, const double* shift, int nshift,
                            double* count);
 
+/* --------------------------------------------------------- spectral Laplacians --- */
+/* The step after the backprojection (the inversion is the Laplacian of the adjoint in 3D):
+ * output = ifftn(U .* fftn(u)), U = -(2 pi / period)^2 * |k|^2 with the signed integer
+ * frequencies k (index i <= n/2 -> i, else i - n).  FP64; u and out are n^3 (n^2) arrays in
+ * any index order (the operator is the same along every axis); the real part is returned.
+ * Replace laplace3dfft (radon_inversion/backprojection/laplace3dfft.m:1-25) and laplace2dfft
+ * (radon_inversion/backprojection/laplace2dfft.m:1-23). */
+int wrt_laplace3d(wrt_ctx* ctx, const double* u, int n, double period, double* out, wrt_stats* st);
+int wrt_laplace2d(wrt_ctx* ctx, const double* u, int n, double period, double* out, wrt_stats* st);
+
 /* ------------------------------------------------------ analytic test functions --- */
 /* The analytic programs (radon_analytic, ray_analytic) integrate a test function that the
  * reference compiles in from test_function.c.  Here the test function is a sum of terms

@@ -402,3 +402,52 @@ def test_analytic_ellipsoid_term_closed_form():
         want = orc.ellipsoid_plane(np.array(terms[0][1]), np.array([0.5, 0.3, 0.2]),
                                    np.array(rot), 0.7, n, s)
         assert abs(orc.analytic_plane(terms, 129, phi, th, s) - want) <= 1e-2 * scale
+
+
+# ------------------------------------------------------------- spectral Laplacians ---
+def _spectral():
+    import importlib.util
+
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle",
+                        "spectral.py")
+    spec = importlib.util.spec_from_file_location("spectral", path)
+    m = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(m)
+    return m
+
+
+@pytest.mark.parametrize("n", [8, 9, 16])
+def test_spectral_laplacian_on_trigonometric_polynomials(n):
+    # cos(2 pi (a.i)/n) is an eigenvector: Laplacian = -(2 pi / period)^2 |a|^2 times it, for
+    # every frequency up to and including the Nyquist one (kept as +n/2 by laplace3dfft.m)
+    sp = _spectral()
+    period = 2.5
+    i = np.arange(n)
+    I, J, K = np.meshgrid(i, i, i, indexing="ij")
+    for a in [(1, 0, 0), (0, 2, 1), (3, 1, 2), (n // 2, 0, 1), (n // 2, n // 2, n // 2)]:
+        u = np.cos(2 * np.pi * (a[0] * I + a[1] * J + a[2] * K) / n)
+        want = -(2 * np.pi / period) ** 2 * (a[0] ** 2 + a[1] ** 2 + a[2] ** 2) * u
+        assert np.max(np.abs(sp.laplace_fft(u, period) - want)) <= 1e-11 * max(1.0, np.max(np.abs(want)))
+    I2, J2 = np.meshgrid(i, i, indexing="ij")
+    u = np.sin(2 * np.pi * (2 * I2 + 3 * J2) / n)
+    want = -(2 * np.pi / period) ** 2 * 13 * u
+    assert np.max(np.abs(sp.laplace_fft(u, period) - want)) <= 1e-11 * np.max(np.abs(want))
+    f = sp.frequencies(n)
+    assert f[0] == 0 and f[n // 2] == n // 2 and (n % 2 == 0 or f[n // 2 + 1] == -(n // 2))
+
+
+def test_spectral_laplacian_is_axis_symmetric_and_smooth_limit():
+    sp = _spectral()
+    rng = np.random.default_rng(3)
+    u = rng.standard_normal((12, 12, 12))
+    assert np.allclose(sp.laplace_fft(u.transpose(2, 0, 1), 2.0),
+                       sp.laplace_fft(u, 2.0).transpose(2, 0, 1), atol=1e-12)
+    # a narrow Gaussian on a wide period (e^-36 at the edge): the spectral Laplacian is the
+    # analytic one, (4 r^2 - 6) e^(-r^2)
+    n, period = 128, 12.0
+    x = (np.arange(n) - n // 2) * (period / n)
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    r2 = X ** 2 + Y ** 2 + Z ** 2
+    u = np.exp(-r2)
+    want = (4 * r2 - 6) * u
+    assert np.max(np.abs(sp.laplace_fft(u, period) - want)) <= 1e-10

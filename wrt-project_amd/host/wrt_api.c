@@ -73,6 +73,7 @@ void wrt_free(wrt_ctx* c) {
   hipSetDevice(c->device);
   hipStreamSynchronize(c->stream);
   wrt_batch_release(c);
+  wrt_spectral_release(c);
   hipEventDestroy(c->ev0);
   hipEventDestroy(c->ev1);
   hipEventDestroy(c->ev2);

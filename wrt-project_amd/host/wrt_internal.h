@@ -19,7 +19,10 @@ struct wrt_ctx {
   int cu_count;
   /* batch graph cache */
   void* batch;
+  /* spectral Laplacian: FFT plans of the last size (host/wrt_spectral.c) */
+  void* spectral;
 };
+void wrt_spectral_release(wrt_ctx* c);
 
 struct wrt_job {
   wrt_ctx* ctx;

@@ -82,7 +82,7 @@ ABI_SYMBOLS = [
     "wrt_job_poll", "wrt_job_wait", "wrt_serve_describe", "wrt_serve_grids", "wrt_serve_desc",
     "wrt_serve_input", "wrt_serve_run", "wrt_batch_dgroup",
     "wrt_fn_example", "wrt_fn_phantom", "wrt_analytic_zslice_grid", "wrt_radon_analytic",
-    "wrt_ray_analytic",
+    "wrt_ray_analytic", "wrt_laplace3d", "wrt_laplace2d",
 ]
 
 FN = {"exp_bump": 0, "sine_bump": 1, "ellipsoid": 2}
@@ -183,6 +183,8 @@ def lib():
         L.wrt_serve_desc.argtypes = [P, P, P, P, P, P, P]
         L.wrt_serve_input.argtypes = [P, P]
         L.wrt_serve_run.argtypes = [P, P, P, P, P]
+        L.wrt_laplace3d.argtypes = [P, P, I, D, P, P]
+        L.wrt_laplace2d.argtypes = [P, P, I, D, P, P]
         L.wrt_fn_example.argtypes = [C.c_char_p, P, I]
         L.wrt_fn_phantom.argtypes = [I, C.c_ulonglong, P]
         L.wrt_analytic_zslice_grid.argtypes = [I, P]
@@ -352,6 +354,16 @@ class Context:
         _chk(lib().wrt_ray_analytic(self.h, t, len(terms), ngrid, _p(z), len(z), _p(shift),
                                     len(shift), _p(phi), len(phi), _p(out), C.byref(st)),
              "wrt_ray_analytic")
+        return out, st.as_dict()
+
+    def laplace(self, u, period):
+        """spectral Laplacian of a 2-D or 3-D FP64 array with equal sides"""
+        u = _f64(u)
+        n = u.shape[0]
+        out = np.empty_like(u)
+        st = Stats()
+        fn = lib().wrt_laplace3d if u.ndim == 3 else lib().wrt_laplace2d
+        _chk(fn(self.h, _p(u), n, float(period), _p(out), C.byref(st)), "wrt_laplace")
         return out, st.as_dict()
 
     def radon_count_incube(self, n, phi, theta, shift):
