@@ -139,6 +139,27 @@ int wrt_radon_count_incube(wrt_ctx* ctx, int n, const double* phi, int nphi,
                            const double* theta, int ntheta, const double* shift, int nshift,
                            double* count);
 
+/* --------------------------------------------------------------- file formats --- */
+/* The reference's text files, read and written in parallel with the same values and bytes
+ * as its fscanf / fprintf loops, and a binary format for chaining programs without text.
+ * wrt_read_text: count records "%lf\n" (cols = 1) or "%lf, %lf, %lf, %lf\n" keeping the
+ * last value (cols = 4), as tfunction_values_init (radon_grid/src/init.c:107-139),
+ * ray_values_init (radon_reduction/init.c:116-164) and radon_values_init
+ * (radon_inversion/backprojection/C/src/init.c:117-153) read them; WRT_E_IO on a short or
+ * malformed file (with the reference programs' messages on stderr). */
+int wrt_read_text(const char* path, size_t count, int cols, double* out);
+/* rows "%lf, %lf, %lf, %lf\n" = a[i], b[j], c[k], values[(i*nb + j)*nc + k] in the order
+ * i -> j -> k, as the drivers write them (radon_grid/src/radon_grid.c:248-258,
+ * ray_grid/src/ray_grid.c:244-255, the analytic programs) */
+int wrt_write_text_grid(const char* path, const double* a, int na, const double* b, int nb,
+                        const double* c, int nc, const double* values);
+/* rows "%lf\n" (radon_reduction/reduction.c:213-229, radon3d_ajoint.c:240-254) */
+int wrt_write_text_values(const char* path, const double* values, size_t count);
+/* binary (no reference counterpart): magic "WRTF64\0\1", uint64 count, little-endian
+ * binary64 values in the text files' element order (the value column only) */
+int wrt_write_f64(const char* path, const double* values, size_t count);
+int wrt_read_f64(const char* path, size_t count, double* out);
+
 /* --------------------------------------------------------- spectral Laplacians --- */
 /* The step after the backprojection (the inversion is the Laplacian of the adjoint in 3D):
  * output = ifftn(U .* fftn(u)), U = -(2 pi / period)^2 * |k|^2 with the signed integer

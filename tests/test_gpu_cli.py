@@ -146,3 +146,35 @@ def test_ray_analytic_program(tmp_path):
     want = orc.analytic_ray_grid(example_terms("exponential_bump"), g, z, s, phi)
     _same_text(_col(tmp_path / "p.txt", 4), lf(want))
     assert "[z], [s], [phi], [Pf]" in open(tmp_path / "readme_p.txt").read()
+
+
+def test_binary_chain_matches_library_calls(tmp_path):
+    # ray_grid -B -> radon_reduction -b -B -> radon3d_adjoint -b -B: full binary64 values
+    # between the programs (no %lf quantisation), equal to the library's own results
+    n, ns, nphi, nth = 17, 9, 8, 4
+    vol = wrt.phantom(n, False, 10, 6, 0.0)
+    wrt.write_f64(tmp_path / "vol.f64", vol)
+    write_config(tmp_path / "rcfg.txt", [n, n, ns, nphi])  # ngrid nslices nshift nphi
+    r = run("ray_grid", ["-p", "rcfg.txt", "-i", "vol.f64", "-b", "-o", "rays.f64", "-B"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    s, phi = wrt.ray_grid_init(ns, nphi)
+    z = wrt.zslice_grid(n)
+    with wrt.Context(0) as ctx:
+        rays, _ = ctx.ray_grid(vol, z, s, phi)
+        assert np.array_equal(wrt.read_f64(tmp_path / "rays.f64", rays.size), rays.ravel())
+        write_config(tmp_path / "dcfg.txt", [ns, nphi, nth, n])  # nshift nphi ntheta nslices
+        r = run("radon_reduction", ["-p", "dcfg.txt", "-i", "rays.f64", "-b", "-o", "radon.f64", "-B"],
+                tmp_path)
+        assert r.returncode == 0, r.stderr
+        _, th, w, _ = ctx.radon_grid_init(nphi, nth, ns)
+        radon, _ = ctx.radon_reduction(rays, s, phi, th)
+        assert np.array_equal(wrt.read_f64(tmp_path / "radon.f64", radon.size), radon.ravel())
+        write_config(tmp_path / "acfg.txt", [nphi, nth, ns, 12])  # nphi ntheta nshift ngrid
+        r = run("radon3d_adjoint", ["-p", "acfg.txt", "-i", "radon.f64", "-b", "-o", "adj.f64", "-B"],
+                tmp_path)
+        assert r.returncode == 0, r.stderr
+        adj, _ = ctx.radon_adjoint(radon, phi, th, w, s, 12)
+        assert np.array_equal(wrt.read_f64(tmp_path / "adj.f64", adj.size), adj.ravel())
+    # a binary file of the wrong size is refused
+    r = run("radon3d_adjoint", ["-p", "acfg.txt", "-i", "rays.f64", "-b", "-o", "x.f64"], tmp_path)
+    assert r.returncode != 0 and "values" in r.stderr

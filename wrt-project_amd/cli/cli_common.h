@@ -9,7 +9,10 @@
  *     (radon_grid/src/init.c:17-82), so a line without trailing text is an error;
  *   - text outputs printed with %lf in the reference's element order, plus readme_<output>.
  * Build-only options that never change the meaning of the reference's: -P f64|f32|bf16
- * (precision, default f64 = the plain form), -d <device>.
+ * (precision, default f64 = the plain form), -d <device>, -b / -B (input / output in the
+ * library's binary format, the value column only).  Text is read and written through the
+ * library's parallel codecs (wrt_read_text, wrt_write_text_*), with the same values and bytes
+ * as fscanf / fprintf.
  * Long options (--help, --parameters, --input, --output, --nthreads, --small) work here;
  * the reference declares them with a zero flag value, so they never reach its switch.
  */
@@ -34,6 +37,7 @@ typedef struct {
   int nthreads;
   int small;
   int device;
+  int bin_in, bin_out; /* binary input / output (wrt_read_f64 / wrt_write_f64 format) */
   wrt_prec prec;
 } cli_opts;
 
@@ -49,6 +53,8 @@ static void cli_usage(FILE* f, const char* prog, int with_small) {
   if (with_small) fprintf(f, "   -s --small                Input holds one value per line.\n");
   fprintf(f,
           "   -P --precision f64|f32|bf16  Arithmetic (default f64, the plain form).\n"
+          "   -b --binary-in            Input in the library's binary format (values only).\n"
+          "   -B --binary-out           Output in the library's binary format (values only).\n"
           "   -d --device number        HIP device (default 0).\n");
 }
 
@@ -68,8 +74,10 @@ static int cli_parse(int argc, char** argv, cli_opts* o, int with_small) {
                                  {"small", no_argument, 0, 's'},
                                  {"precision", required_argument, 0, 'P'},
                                  {"device", required_argument, 0, 'd'},
+                                 {"binary-in", no_argument, 0, 'b'},
+                                 {"binary-out", no_argument, 0, 'B'},
                                  {0, 0, 0, 0}};
-  const char* shorts = with_small ? "hp:i:so:n:P:d:" : "hp:i:o:n:P:d:";
+  const char* shorts = with_small ? "hp:i:so:n:P:d:bB" : "hp:i:o:n:P:d:bB";
   int c;
   while ((c = getopt_long(argc, argv, shorts, longs, NULL)) != -1) {
     switch (c) {
@@ -97,6 +105,8 @@ static int cli_parse(int argc, char** argv, cli_opts* o, int with_small) {
         else { fprintf(stderr, "%s: unknown precision %s\n", argv[0], optarg); exit(EXIT_FAILURE); }
         break;
       case 'd': o->device = atoi(optarg); break;
+      case 'b': o->bin_in = 1; break;
+      case 'B': o->bin_out = 1; break;
       default: cli_usage(stderr, argv[0], with_small); exit(EXIT_FAILURE);
     }
   }
@@ -135,33 +145,24 @@ static int cli_params_n(const char* fn, int* p, int count) {
 /* the grid programs' 4-line config */
 static int cli_params(const char* fn, int p[4]) { return cli_params_n(fn, p, 4); }
 
-/* count values "%lf\n" (cols = 1) or "%lf, %lf, %lf, %lf\n" keeping the last (cols = 4) */
-static int cli_read_values(const char* fn, size_t count, int cols, double* out) {
-  FILE* f = fopen(fn, "r");
-  if (!f) {
-    fprintf(stderr, "Recieved call %s while opening %s. \n", strerror(errno), fn);
-    return errno;
+/* count values "%lf\n" (cols = 1) or "%lf, %lf, %lf, %lf\n" keeping the last (cols = 4),
+ * or the binary format with -b; 0 or -1 */
+static int cli_read_input(const cli_opts* o, size_t count, int cols, double* out) {
+  const int rc = o->bin_in ? wrt_read_f64(o->input, count, out) : wrt_read_text(o->input, count, cols, out);
+  return rc == WRT_OK ? 0 : -1;
+}
+
+/* the output: 4-column text rows over the grid (a, b, c), 1-column text, or binary (-B) */
+static void cli_write_output(int bin, const char* path, const double* a, int na, const double* b,
+                             int nb, const double* c, int nc, const double* values, size_t count) {
+  int rc;
+  if (bin) rc = wrt_write_f64(path, values, count);
+  else if (a) rc = wrt_write_text_grid(path, a, na, b, nb, c, nc, values);
+  else rc = wrt_write_text_values(path, values, count);
+  if (rc != WRT_OK) {
+    fprintf(stderr, "Aborting : cannot write %s : %s\n", path, wrt_strerror(rc));
+    exit(EXIT_FAILURE);
   }
-  for (size_t i = 0; i < count; ++i) {
-    int cnt;
-    if (cols == 1) {
-      cnt = fscanf(f, "%lf\n", &out[i]);
-    } else {
-      double a, b, c;
-      cnt = fscanf(f, "%lf, %lf, %lf, %lf\n", &a, &b, &c, &out[i]);
-    }
-    if (cnt == EOF) {
-      fprintf(stderr, "Error : fscanf : reached the end of file before it was expected.\n");
-      fclose(f);
-      return -1;
-    } else if (cnt != cols) {
-      fprintf(stderr, "Error: fscanf : matching failure, expected %d floating-point value(s).\n", cols);
-      fclose(f);
-      return -1;
-    }
-  }
-  fclose(f);
-  return 0;
 }
 
 static double cli_now(void) {
@@ -208,6 +209,7 @@ typedef struct {
   const char* terms_file;
   int nthreads;
   int device;
+  int bin_out;
 } cli_an_opts;
 
 static void cli_an_usage(FILE* f, const char* prog) {
@@ -224,6 +226,7 @@ static void cli_an_usage(FILE* f, const char* prog) {
           "                               sine_bump cx cy cz amplitude rin rout\n"
           "                               ellipsoid cx cy cz density a1 a2 a3 r11 r12 ... r33\n"
           "                             ('#' starts a comment).\n"
+          "   -B --binary-out           Output in the library's binary format (values only).\n"
           "   -d --device number        HIP device (default 0).\n");
 }
 
@@ -241,9 +244,10 @@ static void cli_an_parse(int argc, char** argv, cli_an_opts* o) {
                                  {"function", required_argument, 0, 'F'},
                                  {"terms", required_argument, 0, 'f'},
                                  {"device", required_argument, 0, 'd'},
+                                 {"binary-out", no_argument, 0, 'B'},
                                  {0, 0, 0, 0}};
   int c;
-  while ((c = getopt_long(argc, argv, "hp:o:n:F:f:d:", longs, NULL)) != -1) {
+  while ((c = getopt_long(argc, argv, "hp:o:n:F:f:d:B", longs, NULL)) != -1) {
     switch (c) {
       case 'h': cli_an_usage(stdout, argv[0]); exit(EXIT_SUCCESS);
       case 'p': o->params = optarg; printf("Parameters file: %s\n", optarg); break;
@@ -259,6 +263,7 @@ static void cli_an_parse(int argc, char** argv, cli_an_opts* o) {
       case 'F': o->example = optarg; break;
       case 'f': o->terms_file = optarg; break;
       case 'd': o->device = atoi(optarg); break;
+      case 'B': o->bin_out = 1; break;
       default: cli_an_usage(stderr, argv[0]); exit(EXIT_FAILURE);
     }
   }

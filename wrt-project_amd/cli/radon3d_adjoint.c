@@ -28,7 +28,7 @@ int main(int argc, char** argv) {
   if (rc) cli_die("wrt_radon_grid_init", rc);
   const size_t nin = (size_t)nshift * nphi * ntheta;
   double* radon = (double*)cli_alloc(sizeof(double) * nin);
-  if (cli_read_values(o.input, nin, 4, radon) != 0) { perror("Aborting : radon_values_init.\n"); exit(EXIT_FAILURE); }
+  if (cli_read_input(&o, nin, 4, radon) != 0) { perror("Aborting : radon_values_init.\n"); exit(EXIT_FAILURE); }
   printf("Done.\n\n");
   const size_t tot = (size_t)n * n * n;
   double* out = (double*)cli_alloc(sizeof(double) * tot);
@@ -36,10 +36,7 @@ int main(int argc, char** argv) {
   wrt_stats st;
   rc = wrt_radon_adjoint(ctx, radon, phi, nphi, theta, w, ntheta, shift, nshift, n, o.prec, out, &st);
   if (rc) cli_die("wrt_radon_adjoint", rc);
-  FILE* f = fopen(o.output, "w");
-  if (!f) { fprintf(stderr, "Aborting : cannot open %s\n", o.output); exit(EXIT_FAILURE); }
-  for (size_t k = 0; k < tot; ++k) fprintf(f, "%lf\n", out[k]);
-  fclose(f);
+  cli_write_output(o.bin_out, o.output, NULL, 0, NULL, 0, NULL, 0, out, tot);
   printf("\n  Number of threads: %d, elapsed time: %3.1f sec (GPU kernel %.3f sec)\n", o.nthreads,
          cli_now() - t0, st.kernel_seconds);
   const char* names[] = {"number of grid points [ngrid]", "number of angles of [phi]    ",

@@ -35,14 +35,7 @@ int main(int argc, char** argv) {
   wrt_stats st;
   rc = wrt_radon_analytic(ctx, terms, nt, ngrid, phi, nphi, theta, ntheta, shift, nshift, out, &st);
   if (rc) cli_die("wrt_radon_analytic", rc);
-  FILE* f = fopen(o.output, "w");
-  if (!f) { fprintf(stderr, "Aborting : cannot open %s\n", o.output); exit(EXIT_FAILURE); }
-  size_t k = 0;
-  for (int is = 0; is < nshift; ++is)
-    for (int ip = 0; ip < nphi; ++ip)
-      for (int it = 0; it < ntheta; ++it, ++k)
-        fprintf(f, "%lf, %lf, %lf, %lf\n", shift[is], phi[ip], theta[it], out[k]);
-  fclose(f);
+  cli_write_output(o.bin_out, o.output, shift, nshift, phi, nphi, theta, ntheta, out, tot);
   printf("\n  Number of threads: %d, elapsed time: %3.1f sec (GPU kernel %.3f sec)\n", o.nthreads,
          cli_now() - t0, st.kernel_seconds);
   const char* names[] = {"number of angles of [phi]  ", "number of angles of [theta]",

@@ -17,7 +17,7 @@ int main(int argc, char** argv) {
   if (nphi < 1 || ntheta < 1 || nshift < 2 || n < 1) { fprintf(stderr, "Aborting : invalid grid sizes\n"); exit(EXIT_FAILURE); }
   printf("  Initializing data for the test-function...");
   double* vol = (double*)cli_alloc(sizeof(double) * (size_t)n * n * n);
-  if (cli_read_values(o.input, (size_t)n * n * n, 1, vol) != 0) { perror("Aborting : tfunction_values_init\n"); exit(EXIT_FAILURE); }
+  if (cli_read_input(&o, (size_t)n * n * n, 1, vol) != 0) { perror("Aborting : tfunction_values_init\n"); exit(EXIT_FAILURE); }
   printf("Done.\n");
   wrt_ctx* ctx = NULL;
   int rc = wrt_init(&ctx, o.device);
@@ -35,14 +35,7 @@ int main(int argc, char** argv) {
   wrt_stats st;
   rc = wrt_radon_grid(ctx, vol, n, phi, nphi, theta, ntheta, shift, nshift, o.prec, out, &st);
   if (rc) cli_die("wrt_radon_grid", rc);
-  FILE* f = fopen(o.output, "w");
-  if (!f) { fprintf(stderr, "Aborting : cannot open %s\n", o.output); exit(EXIT_FAILURE); }
-  size_t k = 0;
-  for (int is = 0; is < nshift; ++is)
-    for (int ip = 0; ip < nphi; ++ip)
-      for (int it = 0; it < ntheta; ++it, ++k)
-        fprintf(f, "%lf, %lf, %lf, %lf\n", shift[is], phi[ip], theta[it], out[k]);
-  fclose(f);
+  cli_write_output(o.bin_out, o.output, shift, nshift, phi, nphi, theta, ntheta, out, tot);
   printf("\n  Number of threads: %d, elapsed time: %3.1f sec (GPU kernel %.3f sec)\n", o.nthreads,
          cli_now() - t0, st.kernel_seconds);
   const char* names[] = {"number of grid ponts [ngrid]", "number of angles of [phi]  ",

@@ -20,7 +20,7 @@ int main(int argc, char** argv) {
   printf("  Initializing data for ray transforms...");
   const size_t nin = (size_t)nz * nshift * nphi;
   double* rays = (double*)cli_alloc(sizeof(double) * nin);
-  if (cli_read_values(o.input, nin, o.small ? 1 : 4, rays) != 0) { perror("Aborting : ray_values_init.\n"); exit(EXIT_FAILURE); }
+  if (cli_read_input(&o, nin, o.small ? 1 : 4, rays) != 0) { perror("Aborting : ray_values_init.\n"); exit(EXIT_FAILURE); }
   printf("Done.\n");
   wrt_ctx* ctx = NULL;
   int rc = wrt_init(&ctx, o.device);
@@ -38,10 +38,7 @@ int main(int argc, char** argv) {
   wrt_stats st;
   rc = wrt_radon_reduction(ctx, rays, nz, shift, nshift, phi, nphi, theta, ntheta, out, &st);
   if (rc) cli_die("wrt_radon_reduction", rc);
-  FILE* f = fopen(o.output, "w");
-  if (!f) { fprintf(stderr, "Aborting : cannot open %s\n", o.output); exit(EXIT_FAILURE); }
-  for (size_t k = 0; k < tot; ++k) fprintf(f, "%lf\n", out[k]);
-  fclose(f);
+  cli_write_output(o.bin_out, o.output, NULL, 0, NULL, 0, NULL, 0, out, tot);
   printf("\n  Number of threads: %d, elapsed time: %3.1f sec (GPU kernel %.3f sec)\n", o.nthreads,
          cli_now() - t0, st.kernel_seconds);
   const char* names[] = {"number of angles of [phi]  ", "number of angles of [theta]", "number of shifts of [s]    "};

@@ -34,14 +34,7 @@ int main(int argc, char** argv) {
   wrt_stats st;
   rc = wrt_ray_analytic(ctx, terms, nt, ngrid, z, ngrid, shift, nshift, phi, nphi, out, &st);
   if (rc) cli_die("wrt_ray_analytic", rc);
-  FILE* f = fopen(o.output, "w");
-  if (!f) { fprintf(stderr, "Aborting : cannot open %s\n", o.output); exit(EXIT_FAILURE); }
-  size_t k = 0;
-  for (int iz = 0; iz < ngrid; ++iz)
-    for (int is = 0; is < nshift; ++is)
-      for (int ip = 0; ip < nphi; ++ip, ++k)
-        fprintf(f, "%lf, %lf, %lf, %lf\n", z[iz], shift[is], phi[ip], out[k]);
-  fclose(f);
+  cli_write_output(o.bin_out, o.output, z, ngrid, shift, nshift, phi, nphi, out, tot);
   printf("\n  Number of threads: %d, elapsed time: %3.1f sec (GPU kernel %.3f sec)\n", o.nthreads,
          cli_now() - t0, st.kernel_seconds);
   const char* names[] = {"number of z-slices [nz]", "number of shifts   [s] ", "number of angles   [phi]"};

@@ -17,7 +17,7 @@ int main(int argc, char** argv) {
   if (n < 2 || nz < 2 || nshift < 2 || nphi < 1) { fprintf(stderr, "Aborting : invalid grid sizes\n"); exit(EXIT_FAILURE); }
   printf("  Initializing test-function data...");
   double* vol = (double*)cli_alloc(sizeof(double) * (size_t)n * n * n);
-  if (cli_read_values(o.input, (size_t)n * n * n, 1, vol) != 0) { perror("Aborting : tfunction_values_init.\n"); exit(EXIT_FAILURE); }
+  if (cli_read_input(&o, (size_t)n * n * n, 1, vol) != 0) { perror("Aborting : tfunction_values_init.\n"); exit(EXIT_FAILURE); }
   printf("Done.\n");
   double* phi = (double*)cli_alloc(sizeof(double) * nphi);
   double* shift = (double*)cli_alloc(sizeof(double) * nshift);
@@ -35,14 +35,7 @@ int main(int argc, char** argv) {
   wrt_stats st;
   rc = wrt_ray_grid(ctx, vol, n, z, nz, shift, nshift, phi, nphi, o.prec, out, &st);
   if (rc) cli_die("wrt_ray_grid", rc);
-  FILE* f = fopen(o.output, "w");
-  if (!f) { fprintf(stderr, "Aborting : cannot open %s\n", o.output); exit(EXIT_FAILURE); }
-  size_t k = 0;
-  for (int iz = 0; iz < nz; ++iz)
-    for (int is = 0; is < nshift; ++is)
-      for (int ip = 0; ip < nphi; ++ip, ++k)
-        fprintf(f, "%lf, %lf, %lf, %lf\n", z[iz], shift[is], phi[ip], out[k]);
-  fclose(f);
+  cli_write_output(o.bin_out, o.output, z, nz, shift, nshift, phi, nphi, out, tot);
   printf("\n  Number of threads: %d, elapsed time: %3.1f sec (GPU kernel %.3f sec)\n", o.nthreads,
          cli_now() - t0, st.kernel_seconds);
   const char* names[] = {"number of z-slices [z]", "number of shifts [s]  ", "number of angles [phi]"};

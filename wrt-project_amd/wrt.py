@@ -82,7 +82,8 @@ ABI_SYMBOLS = [
     "wrt_job_poll", "wrt_job_wait", "wrt_serve_describe", "wrt_serve_grids", "wrt_serve_desc",
     "wrt_serve_input", "wrt_serve_run", "wrt_batch_dgroup",
     "wrt_fn_example", "wrt_fn_phantom", "wrt_analytic_zslice_grid", "wrt_radon_analytic",
-    "wrt_ray_analytic", "wrt_laplace3d", "wrt_laplace2d",
+    "wrt_ray_analytic", "wrt_laplace3d", "wrt_laplace2d", "wrt_read_text",
+    "wrt_write_text_grid", "wrt_write_text_values", "wrt_write_f64", "wrt_read_f64",
 ]
 
 FN = {"exp_bump": 0, "sine_bump": 1, "ellipsoid": 2}
@@ -123,6 +124,36 @@ def fn_phantom(nell, seed=0):
     arr = (FnTerm * max(1, nell))()
     _chk(lib().wrt_fn_phantom(nell, seed, arr), "wrt_fn_phantom")
     return _terms_list(arr, nell)
+
+
+def read_text(path, count, cols):
+    out = np.empty(count)
+    _chk(lib().wrt_read_text(str(path).encode(), count, cols, _p(out)), "wrt_read_text")
+    return out
+
+
+def write_text_grid(path, a, b, c, values):
+    a, b, c, values = map(_f64, (a, b, c, values))
+    assert values.size == a.size * b.size * c.size
+    _chk(lib().wrt_write_text_grid(str(path).encode(), _p(a), a.size, _p(b), b.size, _p(c), c.size,
+                                   _p(values)), "wrt_write_text_grid")
+
+
+def write_text_values(path, values):
+    values = _f64(values).ravel()
+    _chk(lib().wrt_write_text_values(str(path).encode(), _p(values), values.size),
+         "wrt_write_text_values")
+
+
+def write_f64(path, values):
+    values = _f64(values).ravel()
+    _chk(lib().wrt_write_f64(str(path).encode(), _p(values), values.size), "wrt_write_f64")
+
+
+def read_f64(path, count):
+    out = np.empty(count)
+    _chk(lib().wrt_read_f64(str(path).encode(), count, _p(out)), "wrt_read_f64")
+    return out
 
 
 def analytic_zslice_grid(ngrid):
@@ -183,6 +214,11 @@ def lib():
         L.wrt_serve_desc.argtypes = [P, P, P, P, P, P, P]
         L.wrt_serve_input.argtypes = [P, P]
         L.wrt_serve_run.argtypes = [P, P, P, P, P]
+        L.wrt_read_text.argtypes = [C.c_char_p, C.c_size_t, I, P]
+        L.wrt_write_text_grid.argtypes = [C.c_char_p, P, I, P, I, P, I, P]
+        L.wrt_write_text_values.argtypes = [C.c_char_p, P, C.c_size_t]
+        L.wrt_write_f64.argtypes = [C.c_char_p, P, C.c_size_t]
+        L.wrt_read_f64.argtypes = [C.c_char_p, C.c_size_t, P]
         L.wrt_laplace3d.argtypes = [P, P, I, D, P, P]
         L.wrt_laplace2d.argtypes = [P, P, I, D, P, P]
         L.wrt_fn_example.argtypes = [C.c_char_p, P, I]
