@@ -30,7 +30,9 @@ the FP32 pair at full size); "rays" is the line-integral workload at 128^3 (128 
 x 257 shifts: FP32 against FP64, the FP32 transpose with its dot-product test, and the FP64
 ray -> Radon reduction chain); "analytic" is radon_analytic on its own example configuration
 (256 phi x 128 theta x 129 shifts, ngrid 129, the exponential-bump example function) beside
-the reference's compiled plane_integral on the host cores.
+the reference's compiled plane_integral on the host cores; "laplace" is the spectral
+Laplacian of a 256^3 volume; "programs" runs the radon_grid drop-in program end to end at
+256^3 with text files and with the binary format.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
     python bench.py --traffic-probe     (run under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)
@@ -326,6 +328,53 @@ def analytic_leg(ctx, wrt, seconds=6.0):
     return res
 
 
+def laplace_leg(ctx, wrt, n=256):
+    """the spectral Laplacian of an n^3 backprojection-sized volume (FP64): time, and the
+    HBM bytes its three passes must move (real volume in, spectrum out and in, volume out)"""
+    u = wrt.gaussian((n, n, n), seed=31)
+    ctx.laplace(u, 2.0)
+    ts = [ctx.laplace(u, 2.0)[1]["kernel_seconds"] for _ in range(3)]
+    t = float(np.median(ts))
+    spec = n * n * (n // 2 + 1) * 16
+    alg = 2 * n ** 3 * 8 + 2 * spec  # the two real-complex transforms, each read and written once
+    return {"workload": f"spectral Laplacian {n}^3 FP64 (hipFFT D2Z, factor, Z2D)", "seconds": t,
+            "alg_bytes": alg, "achieved_GBps": alg / t / 1e9, "frac_of_hbm": alg / t / HBM_PEAK}
+
+
+def cli_leg(tmp):
+    """end to end radon_grid at 256^3 (FP32 form), text files as the reference's programs read
+    and write them, and the binary format; the time of the program from start to exit"""
+    import subprocess
+
+    import wrt
+
+    bindir = os.path.join(ROOT, "wrt-project_amd", "bin")
+    n = 256
+    vol = wrt.phantom(n, True, 10, 0, 0.01)
+    cfg = os.path.join(tmp, "cfg.txt")
+    with open(cfg, "w") as f:
+        f.write("256 phi\n128 theta\n257 shifts\n256 ngrid\n")
+    wrt.write_text_values(os.path.join(tmp, "vol.txt"), vol)
+    wrt.write_f64(os.path.join(tmp, "vol.f64"), vol)
+    res = {"workload": "radon_grid program, 256^3, 256 phi x 128 theta x 257 s, -P f32"}
+    for name, args in (("text", ["-i", "vol.txt", "-o", "radon.txt"]),
+                       ("binary", ["-i", "vol.f64", "-b", "-o", "radon.f64", "-B"])):
+        t0 = time.perf_counter()
+        r = subprocess.run([os.path.join(bindir, "radon_grid"), "-p", cfg, "-P", "f32"] + args,
+                           cwd=tmp, capture_output=True, text=True, timeout=600)
+        dt = time.perf_counter() - t0
+        if r.returncode != 0:
+            res[name] = {"error": r.stderr[-300:]}
+            continue
+        res[name] = {"seconds": dt, "output_bytes": os.path.getsize(os.path.join(tmp, args[-2 if name == "binary" else -1]))}
+    for f in ("vol.txt", "vol.f64", "radon.txt", "radon.f64"):
+        try:
+            os.remove(os.path.join(tmp, f))
+        except OSError:
+            pass
+    return res
+
+
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "radon_fast_256_traffic.md")
 CALIB_COUNT = 1 << 27  # elements of the calibration conversion: 1 GiB read, 512 MiB written
 
@@ -392,6 +441,8 @@ def main():
     ap.add_argument("--no-transpose", action="store_true")
     ap.add_argument("--no-rays", action="store_true")
     ap.add_argument("--no-analytic", action="store_true")
+    ap.add_argument("--no-laplace", action="store_true")
+    ap.add_argument("--no-programs", action="store_true")
     ap.add_argument("--batch-volumes", type=int, default=48)
     ap.add_argument("--traffic-probe", action="store_true")
     args = ap.parse_args()
@@ -467,6 +518,11 @@ def main():
         log("bench: analytic")
         analytic = analytic_leg(ctx, wrt, seconds=0.0 if args.no_cpu else 6.0)
         log(f"bench: analytic {analytic['seconds']:.3f} s")
+    laplace = None
+    if not args.no_laplace:
+        log("bench: laplace")
+        laplace = laplace_leg(ctx, wrt)
+        log(f"bench: laplace {laplace['seconds'] * 1e3:.2f} ms")
     cpu = None
     if not args.no_cpu:
         log("bench: cpu baseline")
@@ -497,6 +553,15 @@ def main():
         log(f"bench: batch {batch['graph_s']:.3f} s, {batch['speedup_vs_per_job']:.2f}x per-job, "
             f"{batch['speedup_vs_plain']:.2f}x plain")
     ctx.close()
+    programs = None
+    if not args.no_programs:
+        import tempfile
+
+        log("bench: drop-in program end to end")
+        with tempfile.TemporaryDirectory(dir=os.path.join(ROOT, "gpurun_out") if os.path.isdir(
+                os.path.join(ROOT, "gpurun_out")) else None) as tmp:
+            programs = cli_leg(tmp)
+        log(f"bench: programs {programs}")
 
     out = {
         "metric": "plane samples/s at 256^3 FP32 (relL2<=1e-5 vs FP64) vs plain; p99 served latency",
@@ -531,6 +596,8 @@ def main():
         "transpose": transpose,
         "rays": rays,
         "analytic": analytic,
+        "laplace": laplace,
+        "programs": programs,
     }
     print(json.dumps(out), flush=True)
 
