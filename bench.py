@@ -31,7 +31,8 @@ x 257 shifts: FP32 against FP64, the FP32 transpose with its dot-product test, a
 ray -> Radon reduction chain); "analytic" is radon_analytic on its own example configuration
 (256 phi x 128 theta x 129 shifts, ngrid 129, the exponential-bump example function) beside
 the reference's compiled plane_integral on the host cores; "laplace" is the spectral
-Laplacian of a 256^3 volume; "programs" runs the radon_grid drop-in program end to end at
+Laplacian of a 256^3 volume; "fourier" is the Chang-type Fourier-slice inversion (64 phi x
+32 theta x 129 shifts -> 64^3, FP64) with its rate against the FP64 peak; "programs" runs the radon_grid drop-in program end to end at
 256^3 with text files and with the binary format.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
@@ -56,6 +57,7 @@ sys.path.insert(0, os.path.join(ROOT, "wrt-project_amd"))
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
+FP64_PEAK = 78.6e12  # MI355X dense FP64, flop/s
 
 
 def log(*a):
@@ -341,6 +343,35 @@ def laplace_leg(ctx, wrt, n=256):
             "alg_bytes": alg, "achieved_GBps": alg / t / 1e9, "frac_of_hbm": alg / t / HBM_PEAK}
 
 
+def fourier_leg(ctx, wrt, nphi=64, ntheta=32, nshift=129, ngrid=64):
+    """the Fourier-slice inversion (FP64) of the plane integrals of an off-centre Gaussian:
+    time of the two kernels, the reference's adjoint-sum terms (voxel x direction x frequency)
+    per second, and the FP64 rate of the exact sum as evaluated: 4 fused multiply-adds (8 flops)
+    per voxel, direction and folded frequency (Horner's scheme)"""
+    phi = (2 * np.pi / nphi) * np.arange(nphi)
+    x, w = np.polynomial.legendre.leggauss(ntheta)
+    theta = np.arccos(x)
+    s = -1.0 + np.arange(nshift) * (2.0 / (nshift - 1))
+    P, T = np.meshgrid(phi, theta, indexing="ij")
+    c, s2 = (0.2, -0.12, 0.15), 0.04  # centre and sigma^2: below 1e-5 outside the unit ball
+    cu = np.sin(T) * np.cos(P) * c[0] + np.sin(T) * np.sin(P) * c[1] + np.cos(T) * c[2]
+    radon = np.pi * s2 * np.exp(-((s[:, None, None] - cu[None]) ** 2) / s2)
+    out, st = ctx.fourier_inversion(radon, phi, theta, w, ngrid)
+    ts = [ctx.fourier_inversion(radon, phi, theta, w, ngrid)[1]["kernel_seconds"] for _ in range(3)]
+    t = float(np.median(ts))
+    g = (np.arange(ngrid) - ngrid // 2) * (2.0 / ngrid)
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    want = np.exp(-((X - c[0]) ** 2 + (Y - c[1]) ** 2 + (Z - c[2]) ** 2) / s2)
+    ntotal = 4 * ((nshift - 1) // 2)  # padding 4 * floor(1 / dshift)
+    # the sum folds the nodes at -rho and +rho: ntotal/2 + 1 Horner terms per voxel and direction
+    flops = 8.0 * ngrid ** 3 * nphi * ntheta * (ntotal // 2 + 1)
+    return {"workload": f"Chang-type inversion, {nphi} phi x {ntheta} theta x {nshift} s -> {ngrid}^3, FP64",
+            "seconds": t, "adjoint_terms": st["logical_samples"],
+            "terms_per_s": st["logical_samples"] / t, "fp64_TFLOPs": flops / t / 1e12,
+            "frac_of_fp64_peak": flops / t / FP64_PEAK,
+            "max_err_vs_gaussian": float(np.max(np.abs(out - want)))}
+
+
 def cli_leg(tmp):
     """end to end radon_grid at 256^3 (FP32 form), text files as the reference's programs read
     and write them, and the binary format; the time of the program from start to exit"""
@@ -442,6 +473,7 @@ def main():
     ap.add_argument("--no-rays", action="store_true")
     ap.add_argument("--no-analytic", action="store_true")
     ap.add_argument("--no-laplace", action="store_true")
+    ap.add_argument("--no-fourier", action="store_true")
     ap.add_argument("--no-programs", action="store_true")
     ap.add_argument("--batch-volumes", type=int, default=48)
     ap.add_argument("--traffic-probe", action="store_true")
@@ -523,6 +555,11 @@ def main():
         log("bench: laplace")
         laplace = laplace_leg(ctx, wrt)
         log(f"bench: laplace {laplace['seconds'] * 1e3:.2f} ms")
+    fourier = None
+    if not args.no_fourier:
+        log("bench: fourier")
+        fourier = fourier_leg(ctx, wrt)
+        log(f"bench: fourier {fourier['seconds'] * 1e3:.2f} ms")
     cpu = None
     if not args.no_cpu:
         log("bench: cpu baseline")
@@ -597,6 +634,7 @@ def main():
         "rays": rays,
         "analytic": analytic,
         "laplace": laplace,
+        "fourier": fourier,
         "programs": programs,
     }
     print(json.dumps(out), flush=True)

@@ -170,6 +170,23 @@ int wrt_read_f64(const char* path, size_t count, double* out);
 int wrt_laplace3d(wrt_ctx* ctx, const double* u, int n, double period, double* out, wrt_stats* st);
 int wrt_laplace2d(wrt_ctx* ctx, const double* u, int n, double period, double* out, wrt_stats* st);
 
+/* ------------------------------------------------- Fourier-slice inversion --- */
+/* Chang-type inversion through the Fourier slice theorem (radon_inversion/chang3d.m:3-30):
+ * rtft3d (radon_inversion/rtft3d.m:3-137) Fourier transforms every direction's data along s,
+ * zero-padded to ntotal = padding * floor(1/dshift) points (dshift = 2 rsupp/(nshift-1)),
+ * with the shift correction exp(2 pi i rsupp k dfreq) and the jacobian weights
+ * 0.5 dshift dfreq dphi w_theta rho^2; nfft_reconstruct_3d (radon_inversion/
+ * nfft_reconstruct_3d.m:19-53) sums the weighted values back over the nodes rho * u(phi,
+ * theta) at the points p = k * 2/ngrid, k in [-ngrid/2, ngrid/2)^3 (the adjoint NFFT; here
+ * evaluated exactly), real part, divided by w0.  radon[s][phi][theta] on the grids given
+ * (uniform phi and s, Gauss-Legendre theta with weights theta_w); ngrid even.
+ * out[ix][iy][iz] = f((ix, iy, iz) * 2/ngrid - 1), i.e. the reference's
+ * test_function(iy, ix, iz).  FP64.  ntotal must be an even integer (WRT_E_ARG otherwise). */
+int wrt_fourier_inversion(wrt_ctx* ctx, const double* radon, const double* phi, int nphi,
+                          const double* theta, const double* theta_w, int ntheta, int nshift,
+                          double rsupp, double padding, double w0, int ngrid, double* out,
+                          wrt_stats* st);
+
 /* ------------------------------------------------------ analytic test functions --- */
 /* The analytic programs (radon_analytic, ray_analytic) integrate a test function that the
  * reference compiles in from test_function.c.  Here the test function is a sum of terms

@@ -826,6 +826,32 @@ void orc_analytic_zslices(int ngrid, double* z) {
 }
 
 /* ------------------------------------------------------------------------------------ */
+/* Fourier-slice inversion                                                              */
+/* ------------------------------------------------------------------------------------ */
+
+/* the adjoint NFFT of nfft_reconstruct_3d.m:19-53 as its exact sum */
+void orc_fourier_adjoint(const double* coef, const double* rho, int n, const double* dirs,
+                         int ndir, int ngrid, double* out) {
+  const double dx = 2.0 / ngrid;
+  const long long total = (long long)ngrid * ngrid * ngrid;
+#pragma omp parallel for schedule(dynamic, 64)
+  for (long long v = 0; v < total; ++v) {
+    const int iz = (int)(v % ngrid), iy = (int)((v / ngrid) % ngrid), ix = (int)(v / ((long long)ngrid * ngrid));
+    const double px = (ix - ngrid / 2) * dx, py = (iy - ngrid / 2) * dx, pz = (iz - ngrid / 2) * dx;
+    double acc = 0.0;
+    for (int d = 0; d < ndir; ++d) {
+      const double t = px * dirs[3 * d] + py * dirs[3 * d + 1] + pz * dirs[3 * d + 2];
+      const double* c = coef + 2 * (size_t)d * n;
+      for (int m = 0; m < n; ++m) {
+        const double a = 2 * M_PI * rho[m] * t;
+        acc += c[2 * m] * cos(a) - c[2 * m + 1] * sin(a);
+      }
+    }
+    out[v] = acc;
+  }
+}
+
+/* ------------------------------------------------------------------------------------ */
 /* arithmetic property used by the fast kernels                                          */
 /* ------------------------------------------------------------------------------------ */
 

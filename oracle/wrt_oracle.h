@@ -119,6 +119,13 @@ void orc_analytic_ray_grid(const orc_fn_term* t, int nt, int ngrid, const double
 /* z_i = (-1.0) + i*2.0/(ngrid-1) (ray_analytic/ray_analytic.c:255) */
 void orc_analytic_zslices(int ngrid, double* z);
 
+/* ---- Fourier-slice inversion (radon_inversion/nfft_reconstruct_3d.m:19-53) ---- */
+/* out[ix][iy][iz] = Re sum_d sum_m coef[d][m] exp(2 pi i rho_m (p . u_d)) at
+ * p = ((ix, iy, iz) - ngrid/2) * 2/ngrid: the adjoint NFFT's sum, evaluated exactly with one
+ * cosine and sine per term.  coef [ndir][n] complex (re, im), dirs [ndir][3]. */
+void orc_fourier_adjoint(const double* coef, const double* rho, int n, const double* dirs,
+                         int ndir, int ngrid, double* out);
+
 /* ---- arithmetic property of the fast kernels' division ---- */
 long long orc_div_rn_mismatches(int n, long long count, unsigned long long seed);
 

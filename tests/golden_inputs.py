@@ -151,3 +151,31 @@ def analytic_points(name):
                 pts.append(tuple(float(c[k] + r * d[k]) for k in range(3)))
         pts.append(tuple(float(v) for v in c))
     return [tuple(float(v) for v in q) for q in pts]
+
+
+# the off-centre Gaussian of the inversion tests: below 1e-5 of its peak outside the unit ball,
+# so that the plane integrals on s in [-1, 1] hold all of it
+GAUSS_C = (0.2, -0.12, 0.15)
+GAUSS_SIGMA = 0.2
+
+
+def fourier_case(nphi, ntheta, nshift, c=GAUSS_C, sigma=GAUSS_SIGMA):
+    """the grids of rtft3d.m (uniform phi on [0, 2 pi), theta = arccos of the Gauss-Legendre
+    nodes with their weights, uniform shifts on [-1, 1]) and radon[s][phi][theta] of
+    f(x) = exp(-|x - c|^2 / sigma^2), whose plane integrals are pi sigma^2 exp(-(s - c.u)^2 /
+    sigma^2)"""
+    phi = (2 * math.pi / nphi) * np.arange(nphi)
+    x, w = np.polynomial.legendre.leggauss(ntheta)
+    theta = np.arccos(x)
+    s = -1.0 + np.arange(nshift) * (2.0 / (nshift - 1))
+    P, T = np.meshgrid(phi, theta, indexing="ij")
+    cu = (np.sin(T) * np.cos(P) * c[0] + np.sin(T) * np.sin(P) * c[1] + np.cos(T) * c[2])
+    radon = math.pi * sigma ** 2 * np.exp(-((s[:, None, None] - cu[None]) ** 2) / sigma ** 2)
+    return radon, phi, theta, w
+
+
+def gauss_on_grid(ngrid, c=GAUSS_C, sigma=GAUSS_SIGMA):
+    """f at ((ix, iy, iz) - ngrid/2) * 2/ngrid, array [ix][iy][iz]"""
+    x = (np.arange(ngrid) - ngrid // 2) * (2.0 / ngrid)
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    return np.exp(-((X - c[0]) ** 2 + (Y - c[1]) ** 2 + (Z - c[2]) ** 2) / sigma ** 2)

@@ -282,6 +282,31 @@ def analytic_zslices(ngrid):
     return o
 
 
+def fourier_adjoint(rho, dirs, coef, ngrid):
+    """exact adjoint sum of the Fourier-slice inversion (C, OpenMP); coef complex [ndir][n]"""
+    rho, dirs = _f(rho), _f(dirs)
+    c = np.ascontiguousarray(np.stack([coef.real, coef.imag], axis=-1), dtype=np.float64)
+    out = np.empty((ngrid, ngrid, ngrid))
+    L().orc_fourier_adjoint(_p(c), _p(rho), len(rho), _p(dirs), dirs.shape[0], ngrid, _p(out))
+    return out
+
+
+def fourier_module():
+    """oracle/fourier.py (numpy restatement of rtft3d.m / chang3d.m)"""
+    import importlib.util
+
+    spec = importlib.util.spec_from_file_location("fourier", os.path.join(ROOT, "oracle", "fourier.py"))
+    m = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(m)
+    return m
+
+
+def chang3d(radon, phi, theta, theta_w, ngrid, w0=1.0, rsupp=1.0, padding=4.0):
+    """the oracle's Fourier-slice inversion: numpy rtft3d, C exact adjoint sum"""
+    rho, dirs, coef = fourier_module().rtft3d(radon, phi, theta, theta_w, rsupp, padding)
+    return fourier_adjoint(rho, dirs, coef, ngrid) / w0
+
+
 def rel_l2(a, b):
     a = np.asarray(a, dtype=np.float64).ravel()
     b = np.asarray(b, dtype=np.float64).ravel()

@@ -84,6 +84,7 @@ ABI_SYMBOLS = [
     "wrt_fn_example", "wrt_fn_phantom", "wrt_analytic_zslice_grid", "wrt_radon_analytic",
     "wrt_ray_analytic", "wrt_laplace3d", "wrt_laplace2d", "wrt_read_text",
     "wrt_write_text_grid", "wrt_write_text_values", "wrt_write_f64", "wrt_read_f64",
+    "wrt_fourier_inversion",
 ]
 
 FN = {"exp_bump": 0, "sine_bump": 1, "ellipsoid": 2}
@@ -219,6 +220,7 @@ def lib():
         L.wrt_write_text_values.argtypes = [C.c_char_p, P, C.c_size_t]
         L.wrt_write_f64.argtypes = [C.c_char_p, P, C.c_size_t]
         L.wrt_read_f64.argtypes = [C.c_char_p, C.c_size_t, P]
+        L.wrt_fourier_inversion.argtypes = [P, P, P, I, P, P, I, I, D, D, D, I, P, P]
         L.wrt_laplace3d.argtypes = [P, P, I, D, P, P]
         L.wrt_laplace2d.argtypes = [P, P, I, D, P, P]
         L.wrt_fn_example.argtypes = [C.c_char_p, P, I]
@@ -391,6 +393,31 @@ class Context:
                                     len(shift), _p(phi), len(phi), _p(out), C.byref(st)),
              "wrt_ray_analytic")
         return out, st.as_dict()
+
+    def fourier_inversion(self, radon, phi, theta, theta_w, ngrid, w0=1.0, rsupp=1.0,
+                          padding=4.0):
+        """Chang-type Fourier-slice inversion of radon[s][phi][theta] to ngrid^3, FP64"""
+        radon, phi, theta, theta_w = map(_f64, (radon, phi, theta, theta_w))
+        out = np.empty((ngrid, ngrid, ngrid))
+        st = Stats()
+        _chk(lib().wrt_fourier_inversion(self.h, _p(radon), _p(phi), len(phi), _p(theta),
+                                         _p(theta_w), len(theta), radon.shape[0], rsupp, padding,
+                                         w0, ngrid, _p(out), C.byref(st)), "wrt_fourier_inversion")
+        return out, st.as_dict()
+
+    def chang3d(self, ngrid, w0, input_filename, nphi, ntheta, nshift, rsupp=1.0, padd=4.0):
+        """the reference's chang3d(ngrid, w0, input_filename, nphi, ntheta, nshift, rsupp,
+        padd) (radon_inversion/chang3d.m:3-30): reads the last column of the Radon data file
+        (s -> phi -> theta), with phi = k 2 pi / nphi and theta = acos of the Gauss-Legendre
+        nodes in descending order (lgwt's order, rtft3d.m:32-33), and returns the
+        ngrid^3 reconstruction [x][y][z] divided by w0"""
+        radon = read_text(input_filename, nshift * nphi * ntheta, 4)
+        _, theta, w, _ = self.radon_grid_init(nphi, ntheta, nshift)
+        theta, w = theta[::-1].copy(), w[::-1].copy()  # nodes cos(theta) descending
+        phi = np.arange(nphi) * (2 * np.pi / nphi)
+        out, _ = self.fourier_inversion(radon.reshape(nshift, nphi, ntheta), phi, theta, w,
+                                        ngrid, w0=w0, rsupp=rsupp, padding=padd)
+        return out
 
     def laplace(self, u, period):
         """spectral Laplacian of a 2-D or 3-D FP64 array with equal sides"""
