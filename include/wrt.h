@@ -145,7 +145,9 @@ int wrt_radon_count_incube(wrt_ctx* ctx, int n, const double* phi, int nphi,
  * wrt_read_text: count records "%lf\n" (cols = 1) or "%lf, %lf, %lf, %lf\n" keeping the
  * last value (cols = 4), as tfunction_values_init (radon_grid/src/init.c:107-139),
  * ray_values_init (radon_reduction/init.c:116-164) and radon_values_init
- * (radon_inversion/backprojection/C/src/init.c:117-153) read them; WRT_E_IO on a short or
+ * (radon_inversion/backprojection/C/src/init.c:117-153) read them; cols = 2 or 3 reads the
+ * shorter comma-separated records the same way (the 2D Radon data "s, phi, value" that
+ * rtft2d.m:16-17 takes the last column of); WRT_E_IO on a short or
  * malformed file (with the reference programs' messages on stderr). */
 int wrt_read_text(const char* path, size_t count, int cols, double* out);
 /* rows "%lf, %lf, %lf, %lf\n" = a[i], b[j], c[k], values[(i*nb + j)*nc + k] in the order
@@ -186,6 +188,14 @@ int wrt_fourier_inversion(wrt_ctx* ctx, const double* radon, const double* phi, 
                           const double* theta, const double* theta_w, int ntheta, int nshift,
                           double rsupp, double padding, double w0, int ngrid, double* out,
                           wrt_stats* st);
+
+/* The same in 2D (radon_inversion/chang2d.m:3-26 = rtft2d.m:3-62 then
+ * nfft_reconstruct_2d.m:19-45): radon[s][phi] on uniform phi and s, directions
+ * (cos phi, sin phi), weights 0.5 dshift dfreq dphi |rho|; out[ix][iy] = f((ix, iy) * 2/ngrid
+ * - 1), i.e. the reference's test_function(iy, ix).  ngrid even.  FP64. */
+int wrt_fourier_inversion2d(wrt_ctx* ctx, const double* radon, const double* phi, int nphi,
+                            int nshift, double rsupp, double padding, double w0, int ngrid,
+                            double* out, wrt_stats* st);
 
 /* ------------------------------------------------------ analytic test functions --- */
 /* The analytic programs (radon_analytic, ray_analytic) integrate a test function that the

@@ -179,3 +179,19 @@ def gauss_on_grid(ngrid, c=GAUSS_C, sigma=GAUSS_SIGMA):
     x = (np.arange(ngrid) - ngrid // 2) * (2.0 / ngrid)
     X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
     return np.exp(-((X - c[0]) ** 2 + (Y - c[1]) ** 2 + (Z - c[2]) ** 2) / sigma ** 2)
+
+
+def fourier_case2d(nphi, nshift, c=GAUSS_C[:2], sigma=GAUSS_SIGMA):
+    """rtft2d.m's grids (uniform phi on [0, 2 pi), uniform shifts on [-1, 1]) and radon[s][phi]
+    of f(x) = exp(-|x - c|^2 / sigma^2) in 2D: sqrt(pi) sigma exp(-(s - c.u)^2 / sigma^2)"""
+    phi = (2 * math.pi / nphi) * np.arange(nphi)
+    s = -1.0 + np.arange(nshift) * (2.0 / (nshift - 1))
+    cu = np.cos(phi) * c[0] + np.sin(phi) * c[1]
+    radon = math.sqrt(math.pi) * sigma * np.exp(-((s[:, None] - cu[None]) ** 2) / sigma ** 2)
+    return radon, phi
+
+
+def gauss2d_on_grid(ngrid, c=GAUSS_C[:2], sigma=GAUSS_SIGMA):
+    x = (np.arange(ngrid) - ngrid // 2) * (2.0 / ngrid)
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    return np.exp(-((X - c[0]) ** 2 + (Y - c[1]) ** 2) / sigma ** 2)

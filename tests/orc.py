@@ -307,6 +307,11 @@ def chang3d(radon, phi, theta, theta_w, ngrid, w0=1.0, rsupp=1.0, padding=4.0):
     return fourier_adjoint(rho, dirs, coef, ngrid) / w0
 
 
+def chang2d(radon, phi, ngrid, w0=1.0, rsupp=1.0, padding=4.0):
+    """the oracle's 2D Fourier-slice inversion (numpy rtft2d and exact adjoint sum)"""
+    return fourier_module().chang2d(radon, phi, ngrid, w0, rsupp, padding)
+
+
 def rel_l2(a, b):
     a = np.asarray(a, dtype=np.float64).ravel()
     b = np.asarray(b, dtype=np.float64).ravel()

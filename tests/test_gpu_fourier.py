@@ -1,5 +1,5 @@
-"""GPU parity of the Fourier-slice (Chang-type) inversion (wrt_fourier_inversion, FP64,
-radon_inversion/chang3d.m:3-30) against the oracle's restatement (numpy rtft3d + the exact
+"""GPU parity of the Fourier-slice (Chang-type) inversions (wrt_fourier_inversion and
+wrt_fourier_inversion2d, FP64, radon_inversion/chang3d.m:3-30, chang2d.m:3-26) against the oracle's restatement (numpy rtft3d + the exact
 adjoint sum in C, pinned in test_oracle.py by a brute-force DFT and a closed-form Gaussian):
 relative L2 <= 1e-11 and max error <= 1e-10 of the output's scale.  The product evaluates the
 adjoint sum by Horner's scheme, the oracle term by term, so the two differ by rounding only."""
@@ -10,7 +10,8 @@ import pytest
 
 import orc
 import wrt
-from golden_inputs import example_terms, fourier_case, gauss_on_grid
+from golden_inputs import (example_terms, fourier_case, fourier_case2d, gauss2d_on_grid,
+                           gauss_on_grid)
 
 pytestmark = pytest.mark.gpu
 
@@ -104,3 +105,63 @@ def test_chang3d_reads_the_reference_file_format(ctx, tmp_path):
     want, _ = ctx.fourier_inversion(text, phi, theta, w, 16, w0=2.0)
     assert orc.rel_l2(got, want) <= 1e-13
     assert np.max(np.abs(2.0 * got - gauss_on_grid(16))) <= 0.01
+
+
+CASES2D = [
+    (16, 17, 8, 1.0, 4.0),      # ntotal 32
+    (12, 33, 12, 1.0, 2.0),     # ntotal 32
+    (64, 65, 32, 1.0, 4.0),     # ntotal 128
+    (7, 33, 6, 1.0, 0.5),       # ntotal 8 < nshift
+    (20, 13, 10, 0.75, 8.0),    # rsupp 0.75, ntotal 64
+    (1, 9, 4, 1.0, 2.0),        # one direction
+]
+
+
+@pytest.mark.parametrize("case", CASES2D, ids=[str(c) for c in CASES2D])
+def test_inversion2d_matches_oracle_on_random_data(ctx, case):
+    nphi, nshift, ngrid, rsupp, padding = case
+    print("fourier_inversion2d random", case, flush=True)
+    rng = np.random.default_rng(nphi * 1000 + nshift + 7)
+    radon = rng.standard_normal((nshift, nphi))
+    phi = (2 * math.pi / nphi) * np.arange(nphi)
+    got, st = ctx.fourier_inversion2d(radon, phi, ngrid, w0=0.75, rsupp=rsupp, padding=padding)
+    want = orc.chang2d(radon, phi, ngrid, w0=0.75, rsupp=rsupp, padding=padding)
+    assert orc.rel_l2(got, want) <= 1e-11, orc.rel_l2(got, want)
+    assert _max_scale_err(got, want) <= 1e-10, _max_scale_err(got, want)
+    assert st["outputs"] == ngrid ** 2
+
+
+def test_inversion2d_reconstructs_an_off_centre_gaussian(ctx):
+    print("fourier_inversion2d gaussian 256 phi x 257 s -> 128^2, padding 16", flush=True)
+    radon, phi = fourier_case2d(256, 257)
+    got, _ = ctx.fourier_inversion2d(radon, phi, 128, padding=16.0)
+    want = gauss2d_on_grid(128)
+    assert np.max(np.abs(got - want)) <= 4e-4
+    assert orc.rel_l2(got, want) <= 3e-3
+
+
+def test_chang2d_reads_the_reference_file_format(ctx, tmp_path):
+    nphi, nshift = 64, 129
+    print("chang2d from a text file", (nphi, nshift), flush=True)
+    radon, phi = fourier_case2d(nphi, nshift)
+    s = -1.0 + np.arange(nshift) * (2.0 / (nshift - 1))
+    S, P = np.meshgrid(s, phi, indexing="ij")
+    path = tmp_path / "radon2d.txt"
+    with open(path, "w") as f:
+        for a, b, v in zip(S.ravel(), P.ravel(), radon.ravel()):
+            f.write("%f, %f, %f\n" % (a, b, v))
+    got = ctx.chang2d(32, 2.0, str(path), nphi, nshift, 1.0, 16.0)
+    text = wrt.read_text(path, radon.size, 3).reshape(radon.shape)
+    assert np.max(np.abs(text - radon)) <= 5e-7
+    want, _ = ctx.fourier_inversion2d(text, phi, 32, w0=2.0, padding=16.0)
+    assert orc.rel_l2(got, want) <= 1e-13
+    assert np.max(np.abs(2.0 * got - gauss2d_on_grid(32))) <= 1e-3
+
+
+def test_inversion2d_rejects_bad_arguments(ctx):
+    radon = np.zeros((17, 4))
+    phi = np.arange(4) * (math.pi / 2)
+    for kw in [dict(ngrid=7), dict(ngrid=8, padding=0.3), dict(ngrid=8, w0=0.0),
+               dict(ngrid=8, rsupp=0.0), dict(ngrid=8, padding=1.125)]:
+        with pytest.raises(wrt.WrtError):
+            ctx.fourier_inversion2d(radon, phi, **kw)

@@ -519,3 +519,38 @@ def test_chang3d_reconstructs_an_off_centre_gaussian():
     radon, phi, theta, w = fourier_case(32, 16, 65)
     got = orc.chang3d(radon, phi, theta, w, 16)
     assert 1e-3 <= np.max(np.abs(got - gauss_on_grid(16))) <= 0.01
+
+
+def test_rtft2d_is_the_padded_dft_with_shift_correction_and_weights():
+    fo = orc.fourier_module()
+    rng = np.random.default_rng(7)
+    nshift, nphi, rsupp, padding = 9, 3, 1.0, 2.0
+    radon = rng.standard_normal((nshift, nphi))
+    phi = np.array([0.3, 2.5, 5.0])
+    rho, dirs, coef = fo.rtft2d(radon, phi, rsupp, padding)
+    dshift = 2 * rsupp / (nshift - 1)
+    n = int(padding * math.floor(1 / dshift))
+    dfreq = 1 / (n * dshift)
+    for ip in range(nphi):
+        assert np.allclose(dirs[ip], [math.cos(phi[ip]), math.sin(phi[ip])], atol=1e-15)
+        for m in range(n):
+            k = (m + n // 2) % n
+            x = sum(radon[j, ip] * np.exp(-2j * math.pi * j * k / n) for j in range(min(nshift, n)))
+            want = (0.5 * dfreq * (2 * math.pi / nphi) * abs(rho[m]) * dshift
+                    * np.exp(2j * math.pi * rsupp * k * dfreq) * x)
+            assert abs(coef[ip, m] - want) <= 1e-13 * (1 + abs(want))
+
+
+def test_chang2d_reconstructs_an_off_centre_gaussian():
+    # the 2D weight |rho| has a kink at 0, so the rule over rho is second order in dfreq =
+    # 1/(ntotal dshift), which the padding sets: the error falls about 4x per doubling of the
+    # padding (a wrong weight or sign would not converge), and is 2.6e-4 at padding 16
+    from golden_inputs import fourier_case2d, gauss2d_on_grid
+
+    radon, phi = fourier_case2d(64, 129)
+    want = gauss2d_on_grid(32)
+    errs = [np.max(np.abs(orc.chang2d(radon, phi, 32, padding=p) - want)) for p in (4.0, 8.0, 16.0)]
+    assert errs[2] <= 4e-4 and orc.rel_l2(orc.chang2d(radon, phi, 32, padding=16.0), want) <= 3e-3
+    assert 3.0 <= errs[0] / errs[1] <= 6.0 and 3.0 <= errs[1] / errs[2] <= 6.0
+    got = orc.chang2d(radon, phi, 32, w0=2.0, padding=16.0)
+    assert np.unravel_index(np.argmax(got), got.shape) == (16 + 3, 16 - 2)

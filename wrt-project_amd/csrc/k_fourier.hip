@@ -1,4 +1,4 @@
-// k_fourier.hip -- Fourier-slice (Chang-type) inversion of Radon data, FP64.
+// k_fourier.hip -- Fourier-slice (Chang-type) inversion of Radon data in 3D and 2D, FP64.
 //
 // rtft3d.m (radon_inversion/rtft3d.m:3-137) Fourier transforms every direction's Radon data
 // along s (zero-padded to ntotal = n points), multiplies by exp(2 pi i rsupp k dfreq) and
@@ -7,6 +7,9 @@
 // (:19-53) then evaluates, by an adjoint NFFT,
 //   f(p) = Re sum_d sum_m a[d][m] exp(2 pi i rho_m (p . u_d))
 // at p = k * 2/ngrid, k in [-ngrid/2, ngrid/2)^3, and chang3d.m (:3-30) divides by w0.
+// In 2D (rtft2d.m:3-62, nfft_reconstruct_2d.m:19-45, chang2d.m:3-26) the directions are
+// (cos phi, sin phi), the weight 0.5 dshift dfreq dphi |rho_m| and the grid ngrid^2; the
+// kernels take the radial factor (rho^2 or |rho|) as a table and the grid as nx x ny x nz.
 //
 // Only the real part is kept, and Re(a e^{-i psi}) = Re(conj(a) e^{i psi}), so the nodes at
 // -rho and +rho fold into one coefficient: with z = exp(2 pi i dfreq t), t = p . u_d,
@@ -27,7 +30,7 @@
 __device__ __forceinline__ double2 rtft_value(const double* __restrict__ sig, int ns, int m, int n,
                                               const double2* __restrict__ tw,
                                               const double2* __restrict__ corr,
-                                              const double* __restrict__ rho, double wd) {
+                                              const double* __restrict__ rw, double wd) {
   const int k = (m + n / 2) % n;
   double re = 0.0, im = 0.0;
   int q = 0;  // (j * k) mod n, stepped exactly
@@ -39,17 +42,18 @@ __device__ __forceinline__ double2 rtft_value(const double* __restrict__ sig, in
     if (q >= n) q -= n;
   }
   const double2 c = corr[k];
-  const double g = wd * rho[m] * rho[m];
+  const double g = wd * rw[m];
   return make_double2(g * (re * c.x - im * c.y), g * (re * c.y + im * c.x));
 }
 
 // radon [nshift][ndir]; tw [n] = exp(-2 pi i q / n); corr [n] = exp(2 pi i rsupp k dfreq);
-// jw [ndir] = 0.5 dshift dfreq dphi w_theta; rho [n]; out coef [ndir][n/2 + 1] (folded)
+// jw [ndir] = 0.5 dshift dfreq dphi (w_theta); rw [n] = rho_m^2 (3D) or |rho_m| (2D);
+// out coef [ndir][n/2 + 1] (folded)
 __global__ __launch_bounds__(256) void k_rtft(const double* __restrict__ radon, int nshift, int ndir,
                                               const double2* __restrict__ tw,
                                               const double2* __restrict__ corr,
                                               const double* __restrict__ jw,
-                                              const double* __restrict__ rho, int n,
+                                              const double* __restrict__ rw, int n,
                                               double2* __restrict__ coef) {
   extern __shared__ double sig[];
   const int d = blockIdx.x;
@@ -61,12 +65,12 @@ __global__ __launch_bounds__(256) void k_rtft(const double* __restrict__ radon, 
   for (int j = threadIdx.x; j <= h; j += blockDim.x) {
     double2 c;
     if (j == 0) {
-      c = rtft_value(sig, ns, h, n, tw, corr, rho, wd);
+      c = rtft_value(sig, ns, h, n, tw, corr, rw, wd);
     } else {
-      const double2 lo = rtft_value(sig, ns, h - j, n, tw, corr, rho, wd);
+      const double2 lo = rtft_value(sig, ns, h - j, n, tw, corr, rw, wd);
       c = make_double2(lo.x, -lo.y);
       if (j < h) {
-        const double2 hi = rtft_value(sig, ns, h + j, n, tw, corr, rho, wd);
+        const double2 hi = rtft_value(sig, ns, h + j, n, tw, corr, rw, wd);
         c.x += hi.x;
         c.y += hi.y;
       }
@@ -76,12 +80,12 @@ __global__ __launch_bounds__(256) void k_rtft(const double* __restrict__ radon, 
 }
 
 extern "C" int wrtk_rtft(const double* radon, int nshift, int ndir, const void* tw, const void* corr,
-                         const double* jw, const double* rho, int n, void* coef, hipStream_t st) {
+                         const double* jw, const double* rw, int n, void* coef, hipStream_t st) {
   if (nshift < 1 || ndir < 1 || n < 2 || (n & 1)) return WRT_E_ARG;
   const size_t lds = (size_t)(nshift < n ? nshift : n) * sizeof(double);
   if (lds > 65536) return WRT_E_ARG;
   k_rtft<<<ndir, 256, lds, st>>>(radon, nshift, ndir, (const double2*)tw, (const double2*)corr, jw,
-                                 rho, n, (double2*)coef);
+                                 rw, n, (double2*)coef);
   WRT_LAUNCH_CHECK();
   return WRT_OK;
 }
@@ -90,25 +94,27 @@ extern "C" int wrtk_rtft(const double* radon, int nshift, int ndir, const void* 
 #define FA_UNROLL 8
 
 // coef [ndir][nc] folded; dirs [ndir][4] = u (3) and pad; directions [d0, d1) of the slice;
-// out [ngrid^3] in [x][y][z] order: scale * the slice's sum (partial sums when scale = 1)
+// out [nx][ny][nz] at p = ((ix, iy, iz) - (nx, ny, nz)/2) * 2/ngrid (2D: nx = 1, u.x = 0):
+// scale * the slice's sum (partial sums when scale = 1)
 __global__ __launch_bounds__(256) void k_fourier_adjoint(const double2* __restrict__ coef, int nc,
                                                          double dfreq,
                                                          const double4* __restrict__ dirs,
-                                                         int ndir, int nslice, int ngrid,
-                                                         double scale, double* __restrict__ out) {
-  const int nzq = (ngrid + FA_VPT - 1) / FA_VPT;
+                                                         int ndir, int nslice, int nx, int ny,
+                                                         int nz, int ngrid, double scale,
+                                                         double* __restrict__ out) {
+  const int nzq = (nz + FA_VPT - 1) / FA_VPT;
   const long long tq = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long long nq = (long long)ngrid * ngrid * nzq;
+  const long long nq = (long long)nx * ny * nzq;
   if (tq >= nq) return;
   const int s = blockIdx.y;
   const int d0 = (int)((long long)ndir * s / nslice), d1 = (int)((long long)ndir * (s + 1) / nslice);
-  const int izq = (int)(tq % nzq), iy = (int)((tq / nzq) % ngrid), ix = (int)(tq / ((long long)nzq * ngrid));
+  const int izq = (int)(tq % nzq), iy = (int)((tq / nzq) % ny), ix = (int)(tq / ((long long)nzq * ny));
   const double dx = 2.0 / ngrid;
-  const double px = (ix - ngrid / 2) * dx, py = (iy - ngrid / 2) * dx;
+  const double px = (ix - nx / 2) * dx, py = (iy - ny / 2) * dx;
   double pz[FA_VPT], acc[FA_VPT];
 #pragma unroll
   for (int k = 0; k < FA_VPT; ++k) {
-    pz[k] = (izq * FA_VPT + k - ngrid / 2) * dx;
+    pz[k] = (izq * FA_VPT + k - nz / 2) * dx;
     acc[k] = 0.0;
   }
   const double w1 = 2.0 * M_PI * dfreq;  // z = exp(i w1 t)
@@ -150,10 +156,10 @@ __global__ __launch_bounds__(256) void k_fourier_adjoint(const double2* __restri
 #pragma unroll
     for (int k = 0; k < FA_VPT; ++k) acc[k] += hr[k];
   }
-  double* o = out + (size_t)s * ngrid * ngrid * ngrid + ((size_t)ix * ngrid + iy) * ngrid;
+  double* o = out + (size_t)s * nx * ny * nz + ((size_t)ix * ny + iy) * nz;
 #pragma unroll
   for (int k = 0; k < FA_VPT; ++k)
-    if (izq * FA_VPT + k < ngrid) o[izq * FA_VPT + k] = acc[k] * scale;
+    if (izq * FA_VPT + k < nz) o[izq * FA_VPT + k] = acc[k] * scale;
 }
 
 // out[v] = scale * sum_s part[s][v], slices in order
@@ -167,20 +173,20 @@ __global__ __launch_bounds__(256) void k_fourier_sum(const double* __restrict__ 
   out[v] = acc * scale;
 }
 
-// part: nslice * ngrid^3 doubles when nslice > 1 (unused otherwise)
+// part: nslice * nx * ny * nz doubles when nslice > 1 (unused otherwise)
 extern "C" int wrtk_fourier_adjoint(const void* coef, int n, double dfreq, const void* dirs, int ndir,
-                                    int ngrid, int nslice, double inv_w0, double* part, double* out,
-                                    hipStream_t st) {
-  const long long nvox = (long long)ngrid * ngrid * ngrid;
-  if (nvox <= 0 || n < 2 || (n & 1) || ndir < 1 || nslice < 1 || nslice > ndir || nslice > 65535 ||
-      (nslice > 1 && !part))
+                                    int nx, int ny, int nz, int ngrid, int nslice, double inv_w0,
+                                    double* part, double* out, hipStream_t st) {
+  const long long nvox = (long long)nx * ny * nz;
+  if (nx < 1 || ny < 1 || nz < 1 || ngrid < 1 || n < 2 || (n & 1) || ndir < 1 || nslice < 1 ||
+      nslice > ndir || nslice > 65535 || (nslice > 1 && !part))
     return WRT_E_ARG;
-  const long long nq = (long long)ngrid * ngrid * ((ngrid + FA_VPT - 1) / FA_VPT);
+  const long long nq = (long long)nx * ny * ((nz + FA_VPT - 1) / FA_VPT);
   const long long blocks = (nq + 255) / 256;
   if (blocks >= (1LL << 31)) return WRT_E_ARG;
   const dim3 grid((unsigned)blocks, (unsigned)nslice);
   k_fourier_adjoint<<<grid, 256, 0, st>>>((const double2*)coef, n / 2 + 1, dfreq,
-                                          (const double4*)dirs, ndir, nslice, ngrid,
+                                          (const double4*)dirs, ndir, nslice, nx, ny, nz, ngrid,
                                           nslice > 1 ? 1.0 : inv_w0, nslice > 1 ? part : out);
   WRT_LAUNCH_CHECK();
   if (nslice > 1) {

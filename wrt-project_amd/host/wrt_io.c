@@ -172,7 +172,7 @@ static int is_space(char ch) {
 }
 
 /* One record at *pp with fscanf's rules: "%lf" skips white space and converts; in the
- * 4-column format each "," must follow its number directly and ", " skips any white space;
+ * multi-column formats each "," must follow its number directly and ", " skips any white space;
  * the trailing "\n" skips any white space.  Returns the number of values matched (cols on
  * success), or -1 at the end of the data before the first value. */
 static int scan_record(const char** pp, const char* end, int cols, double* last) {
@@ -226,7 +226,7 @@ static int read_sequential(const char* data, size_t len, size_t count, int cols,
 }
 
 int wrt_read_text(const char* path, size_t count, int cols, double* out) {
-  if (!path || (cols != 1 && cols != 4) || (!out && count)) return WRT_E_ARG;
+  if (!path || cols < 1 || cols > 4 || (!out && count)) return WRT_E_ARG;
   FILE* f = fopen(path, "rb");
   if (!f) {
     fprintf(stderr, "Recieved call %s while opening %s. \n", strerror(errno), path);

@@ -84,7 +84,7 @@ ABI_SYMBOLS = [
     "wrt_fn_example", "wrt_fn_phantom", "wrt_analytic_zslice_grid", "wrt_radon_analytic",
     "wrt_ray_analytic", "wrt_laplace3d", "wrt_laplace2d", "wrt_read_text",
     "wrt_write_text_grid", "wrt_write_text_values", "wrt_write_f64", "wrt_read_f64",
-    "wrt_fourier_inversion",
+    "wrt_fourier_inversion", "wrt_fourier_inversion2d",
 ]
 
 FN = {"exp_bump": 0, "sine_bump": 1, "ellipsoid": 2}
@@ -221,6 +221,7 @@ def lib():
         L.wrt_write_f64.argtypes = [C.c_char_p, P, C.c_size_t]
         L.wrt_read_f64.argtypes = [C.c_char_p, C.c_size_t, P]
         L.wrt_fourier_inversion.argtypes = [P, P, P, I, P, P, I, I, D, D, D, I, P, P]
+        L.wrt_fourier_inversion2d.argtypes = [P, P, P, I, I, D, D, D, I, P, P]
         L.wrt_laplace3d.argtypes = [P, P, I, D, P, P]
         L.wrt_laplace2d.argtypes = [P, P, I, D, P, P]
         L.wrt_fn_example.argtypes = [C.c_char_p, P, I]
@@ -404,6 +405,27 @@ class Context:
                                          _p(theta_w), len(theta), radon.shape[0], rsupp, padding,
                                          w0, ngrid, _p(out), C.byref(st)), "wrt_fourier_inversion")
         return out, st.as_dict()
+
+    def fourier_inversion2d(self, radon, phi, ngrid, w0=1.0, rsupp=1.0, padding=4.0):
+        """Chang-type Fourier-slice inversion of radon[s][phi] to ngrid^2 [x][y], FP64"""
+        radon, phi = _f64(radon), _f64(phi)
+        out = np.empty((ngrid, ngrid))
+        st = Stats()
+        _chk(lib().wrt_fourier_inversion2d(self.h, _p(radon), _p(phi), len(phi), radon.shape[0],
+                                           rsupp, padding, w0, ngrid, _p(out), C.byref(st)),
+             "wrt_fourier_inversion2d")
+        return out, st.as_dict()
+
+    def chang2d(self, ngrid, w0, input_filename, nphi, nshift, rsupp=1.0, padd=4.0):
+        """the reference's chang2d(ngrid, w0, input_filename, nphi, nshift, rsupp, padd)
+        (radon_inversion/chang2d.m:3-26): reads the last column of the "s, phi, value" file
+        (s -> phi), phi = k 2 pi / nphi, and returns the ngrid^2 reconstruction [x][y] divided
+        by w0"""
+        radon = read_text(input_filename, nshift * nphi, 3)
+        phi = np.arange(nphi) * (2 * np.pi / nphi)
+        out, _ = self.fourier_inversion2d(radon.reshape(nshift, nphi), phi, ngrid, w0=w0,
+                                          rsupp=rsupp, padding=padd)
+        return out
 
     def chang3d(self, ngrid, w0, input_filename, nphi, ntheta, nshift, rsupp=1.0, padd=4.0):
         """the reference's chang3d(ngrid, w0, input_filename, nphi, ntheta, nshift, rsupp,
