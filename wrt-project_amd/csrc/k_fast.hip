@@ -33,23 +33,24 @@ extern "C" int wrtk_radon_fast(const wrt_radon_fast_args* a, int boundary, hipSt
 }
 
 // ------------------------------------------------- transpose of the plane integrals ---
-// One workgroup per (brick, direction group): the brick task in transpose mode scatters
-// every section's value into the LDS brick, which is then stored whole (coalesced, in the
-// brick's own [x][y][z] order) to the group's slot.  No global atomics: the groups' slots
-// are summed afterwards in FP64 in group order.  (Within a task the LDS additions run in
-// whatever order the waves reach them, so repeated runs agree to FP32 rounding, not bits.)
+// One workgroup per (brick, half brick, direction group): the brick task in transpose mode
+// adds every section's fixed-point value into the half brick's 64-bit LDS accumulators,
+// which are then stored whole to the group's slot in the brick's own [x][y][z] order.  No
+// global atomics, and every sum is an exact integer sum: the result is the same bits for
+// any schedule.
 template <bool BND, bool POW2>
 __global__ __launch_bounds__(FR_WAVES * 64) void k_radon_T_fast(wrt_radon_fast_args a, double h,
                                                                 double dc, double inv_dc) {
-  __shared__ radon_lds L;
-  const int brick = a.tasks[2 * blockIdx.x], grp = a.tasks[2 * blockIdx.x + 1];
-  radon_brick_task<BND, false, POW2, true>(a, brick, grp, h, dc, inv_dc, L);
+  __shared__ radon_T_lds L;
+  const int bh = a.tasks[2 * blockIdx.x], grp = a.tasks[2 * blockIdx.x + 1];
+  const int brick = bh >> 1, zh = bh & 1;
+  radon_brick_task<BND, false, POW2, true>(a, brick, grp, h, dc, inv_dc, L, zh);
   __syncthreads();
   const size_t nbr = (size_t)a.nb * a.nb * a.nb;
-  float* __restrict__ dst = a.part + ((size_t)grp * nbr + brick) * (FB * FB * FB);
-  for (int i = threadIdx.x; i < FB * FB * FB; i += blockDim.x) {
-    const int lx = i >> 10, ly = (i >> 5) & 31, lz = i & 31;
-    dst[i] = L.sb[1 + lx * FR_PX + ly * FR_PY + lz];
+  long long* __restrict__ dst = a.part + ((size_t)grp * nbr + brick) * (FB * FB * FB) + FT_Z * zh;
+  for (int i = threadIdx.x; i < FB * FB * FT_Z; i += blockDim.x) {
+    const int lx = i >> 9, ly = (i >> 4) & 31, lz = i & 15;
+    dst[(lx * FB + ly) * FB + lz] = (long long)L.acc[lx * FT_PX + ly * FT_PY + lz];
   }
 }
 
@@ -66,12 +67,12 @@ static int launch_T(const wrt_radon_fast_args* a, bool pow2, hipStream_t st) {
 
 extern "C" int wrtk_radon_T_fast(const wrt_radon_fast_args* a, int boundary, hipStream_t st) {
   if (a->ntasks <= 0) return WRT_OK;
-  if (!a->g || !a->part || a->rho2 < 3.0) return WRT_E_ARG;
+  if (!a->g || !a->part || a->rho2 < 3.0 || !(a->gscale > 0.0)) return WRT_E_ARG;
   const bool pow2 = (a->n & (a->n - 1)) == 0;
   return boundary ? launch_T<true>(a, pow2, st) : launch_T<false>(a, pow2, st);
 }
 
-__global__ __launch_bounds__(256) void k_radon_T_reduce(const float* __restrict__ part,
+__global__ __launch_bounds__(256) void k_radon_T_reduce(const long long* __restrict__ part,
                                                         int ngroups, int n, int nb,
                                                         double scale, float* __restrict__ out) {
   const size_t total = (size_t)n * n * n;
@@ -81,14 +82,14 @@ __global__ __launch_bounds__(256) void k_radon_T_reduce(const float* __restrict_
     const int z = (int)(k % n), y = (int)((k / n) % n), x = (int)(k / ((size_t)n * n));
     const size_t brick = ((size_t)(x / FB) * nb + y / FB) * nb + z / FB;
     const size_t cell = brick * (FB * FB * FB) + ((x % FB) * FB + y % FB) * FB + z % FB;
-    double acc = 0.0;
-    for (int g = 0; g < ngroups; ++g) acc += (double)part[(size_t)g * slot + cell];
-    out[k] = (float)(acc * scale);
+    unsigned long long acc = 0ull;  // two's complement: wrapping adds give the exact sum
+    for (int g = 0; g < ngroups; ++g) acc += (unsigned long long)part[(size_t)g * slot + cell];
+    out[k] = (float)((double)(long long)acc * scale);
   }
 }
 
-extern "C" int wrtk_radon_T_reduce(const float* part, int ngroups, int n, int nb, double scale,
-                                   float* out, hipStream_t st) {
+extern "C" int wrtk_radon_T_reduce(const long long* part, int ngroups, int n, int nb,
+                                   double scale, float* out, hipStream_t st) {
   const size_t total = (size_t)n * n * n;
   size_t blocks = (total + 255) / 256;
   if (blocks > 65536) blocks = 65536;

@@ -15,6 +15,12 @@
 // brick add its FP32 value to a per-band partial, folded into a Kahan-compensated FP32
 // lane sum; a section's lane sums are reduced over the wavefront in FP64 and added to the
 // plane's FP64 accumulator (one atomic per section and brick).
+//
+// Transpose mode works on half bricks (32 x 32 x 16 cells) of 64-bit integer accumulators:
+// every sample whose cell lies in the half brick adds the section's value, quantized to a
+// fixed point q = rint(g * scale), to the cell with an LDS 64-bit integer atomic.  Integer
+// sums are exact and associative, so the result does not depend on the order in which waves
+// reach a cell; the scale keeps every cell's total within 2^62 (the caller's bound).
 #ifndef WRT_RADON_BRICK_HPP
 #define WRT_RADON_BRICK_HPP
 #include "wrt_common.hpp"
@@ -90,9 +96,10 @@ __device__ __forceinline__ double wave_sum_dpp(double v) {
 }
 
 // v added to the LDS float at p by compare-and-swap.  gfx950 executes ds_add_f32 lane by
-// lane: measured 193 clocks per wave instruction per CU, against 4.6 for ds_add_u32 and 11
-// for this read + ds_cmpst_rtn_b32 loop.  The loop repeats only for lanes whose cell
-// another lane or wave changed in between.
+// lane: measured 193 clocks per wave instruction per CU, against 5 for ds_add_u32, 7 for
+// ds_add_u64, 9 for ds_add_f64 and 15 for this read + ds_cmpst_rtn_b32 loop (distinct
+// addresses).  The loop repeats only for lanes whose cell another lane or wave changed in
+// between.
 __device__ __forceinline__ void lds_add_f32(float* p, float v) {
   unsigned* const q = (unsigned*)p;
   unsigned old = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -168,32 +175,44 @@ struct radon_lds {
   int next_dir;
 };
 
+// Transpose mode: the half brick's 64-bit accumulators, cell (lx, ly, lz) at
+// lx * FT_PX + ly * FT_PY + lz (odd pitches spread a plane patch over the banks).
+#define FT_Z 16
+#define FT_PY 17
+#define FT_PX (17 * 32 + 1)
+struct radon_T_lds {
+  unsigned long long acc[FB * FT_PX];
+  int rowlo[FR_WAVES][136];
+  int next_dir;
+};
+
 // One task: brick `brick` against directions [grp*dgroup, (grp+1)*dgroup).
-// Forward (T = false): the brick of a.bricks is staged and every section's sum is added to
-// its plane's FP64 accumulator.  Transpose (T = true): the LDS brick starts at zero and
-// every sample of a section whose cell lies in the brick adds the section's value a.g[plane]
-// to that cell (an LDS FP32 atomic), so that after the task the brick holds
-// sum_planes g_p * (number of the plane's samples in the cell); the caller stores it.
-// (The additions are compare-and-swap loops, see lds_add_f32.)
+// Forward (T = false, L a radon_lds): the brick of a.bricks is staged and every section's
+// sum is added to its plane's FP64 accumulator.  Transpose (T = true, L a radon_T_lds): the
+// half brick zhalf (cells with z in [16*zhalf, 16*zhalf + 16) of the brick) starts at zero
+// and every sample of a section whose cell lies in it adds the section's quantized value
+// rint(a.g[plane] * a.gscale) to that cell, so that after the task the accumulators hold
+// sum_planes q_p * (number of the plane's samples in the cell); the caller stores them.
 // Called by every thread of a FR_WAVES*64-thread workgroup; returns with the workgroup's
 // updates issued (no barrier at the end: a caller that reads or reuses L waits first).
-template <bool BND, bool BF16, bool POW2, bool T = false>
+template <bool BND, bool BF16, bool POW2, bool T = false, class LDS = radon_lds>
 __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, int brick, int grp,
-                                                 double h, double dc, double inv_dc,
-                                                 radon_lds& L) {
-  float* const sb = L.sb;
+                                                 double h, double dc, double inv_dc, LDS& L,
+                                                 int zhalf = 0) {
+  constexpr int ZN = T ? FT_Z : FB;  // the box's extent in z
   int (*const rowlo)[136] = L.rowlo;
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int nb = a.nb, n = a.n;
   const int bxi = brick / (nb * nb), byi = (brick / nb) % nb, bzi = brick % nb;
-  const int b0x = bxi * FB, b0y = byi * FB, b0z = bzi * FB;
+  const int b0x = bxi * FB, b0y = byi * FB, b0z = bzi * FB + (T ? FT_Z * zhalf : 0);
 
-  // stage the brick into the skewed LDS layout (transpose: clear the accumulator brick)
-  if (T) {
-    for (int i = threadIdx.x; i < FB * FR_PX + 1; i += blockDim.x) sb[i] = 0.f;
+  // stage the brick into the skewed LDS layout (transpose: clear the accumulators)
+  if constexpr (T) {
+    for (int i = threadIdx.x; i < FB * FT_PX; i += blockDim.x) L.acc[i] = 0ull;
     if (threadIdx.x == 0) L.next_dir = grp * a.dgroup;
   } else {
+    float* const sb = L.sb;
     const size_t base = (size_t)brick * (FB * FB * FB);
     for (int i = threadIdx.x; i < FB * FB * FB; i += blockDim.x) {
       const int lx = i >> 10, ly = (i >> 5) & 31, lz = i & 31;
@@ -211,7 +230,7 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
 
   const double delta = (1.0 / (n));
   const double lo[3] = {-1.0 + b0x * dc, -1.0 + b0y * dc, -1.0 + b0z * dc};
-  const double hi[3] = {-1.0 + (b0x + FB) * dc, -1.0 + (b0y + FB) * dc, -1.0 + (b0z + FB) * dc};
+  const double hi[3] = {-1.0 + (b0x + FB) * dc, -1.0 + (b0y + FB) * dc, -1.0 + (b0z + ZN) * dc};
   const double fb0x = (double)b0x, fb0y = (double)b0y, fb0z = (double)b0z;
   const int L0 = -2 * n, L1 = 2 * n - 1;  // lattice index range of i1 and i2 (bounds of R0..R1, C0..C1)
   // Faces of the cube the brick touches.  Only along those axes can a sample whose cell is
@@ -220,7 +239,7 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
   // at the ends of every row (x is monotone along a row) and mask the lanes past a row's
   // end; x == 1 exactly gives index n, which the reference maps to n - 1.
   const bool flo[3] = {b0x == 0, b0y == 0, b0z == 0};
-  const bool fhi[3] = {b0x + FB >= n, b0y + FB >= n, b0z + FB >= n};
+  const bool fhi[3] = {b0x + FB >= n, b0y + FB >= n, b0z + ZN >= n};
   const int cxh = fhi[0] ? n - 1 : INT_MAX, cyh = fhi[1] ? n - 1 : INT_MAX,
             czh = fhi[2] ? n - 1 : INT_MAX;
   const int r_lane = lane >> 3, k_lane = lane & 7;
@@ -317,10 +336,12 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
     }
     const double ia = 1.0 / sqrt(aa);
     for (int j = j0; j <= j1; ++j) {
-      float gv = 0.f;  // transpose: the section's value
-      if (T) {
-        gv = a.g[(size_t)j * a.ndir + d];
+      unsigned long long gq = 0ull;  // transpose: the section's quantized value
+      if constexpr (T) {
+        const float gv = a.g[(size_t)j * a.ndir + d];
         if (gv == 0.f) continue;
+        gq = (unsigned long long)(long long)rint((double)gv * a.gscale);
+        if (gq == 0ull) continue;
       }
       const double s = a.shift[j];
       // the plane meets the support sphere in a circle of radius sqrt(rho2 - s^2) about the
@@ -401,23 +422,16 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
               for (int c = 0; c < 3; ++c)
                 if (flo[c] || fhi[c]) cube_ends(S[c] + av[c] * f1, bv[c], flo[c], fhi[c], rl, rh);
             }
-            // steps of 8 columns (forward) or of 32 columns (transpose walk, below)
-            nst = rh >= rl ? (T ? ((rh - rl + 32) >> 5) : ((rh - rl + 8) >> 3)) : 0;
+            // steps of 8 columns
+            nst = rh >= rl ? ((rh - rl + 8) >> 3) : 0;
           }
         }
         // first and last column, 16 bits each (|i2| <= 2n < 2^15)
         rowlo[wv][t] = (int)(((unsigned)rl & 0xffffu) | ((unsigned)rh << 16));
-        if (T) {
-          // max over the 8 rows 32B + dr + 4r (r = 0..7) of a transpose phase
-          nst = max(nst, __shfl_xor(nst, 4, 64));
-          nst = max(nst, __shfl_xor(nst, 8, 64));
-          nst = max(nst, __shfl_xor(nst, 16, 64));
-        } else {
-          // max over the 8 consecutive rows of a band (lanes 8b .. 8b+7)
-          nst = max(nst, __shfl_xor(nst, 1, 64));
-          nst = max(nst, __shfl_xor(nst, 2, 64));
-          nst = max(nst, __shfl_xor(nst, 4, 64));
-        }
+        // max over the 8 consecutive rows of a band (lanes 8b .. 8b+7)
+        nst = max(nst, __shfl_xor(nst, 1, 64));
+        nst = max(nst, __shfl_xor(nst, 2, 64));
+        nst = max(nst, __shfl_xor(nst, 4, 64));
         bsteps[half] = nst;
       }
       if (__ballot((bsteps[0] | bsteps[1]) != 0) == 0)
@@ -427,7 +441,7 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       // One sample at column f2 of the row whose terms are (Rx, Ry, Rz); frh is the row's
       // last column (face bricks).  Forward: adds the cell's value to bs; transpose: adds the
-      // section's value to the cell.
+      // section's quantized value to the cell.
       auto sample = [&](double Rx, double Ry, double Rz, double f2, double frh, float& bs) {
         // per-column products b*i2, rounded as the reference rounds delta*v2*i2
         const double x = Rx + bv[0] * f2, y = Ry + bv[1] * f2, z = Rz + bv[2] * f2;
@@ -452,61 +466,54 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
           iz = min(iz, czh);
         }
         const int lx = ix - b0x, ly = iy - b0y, lz = iz - b0z;
-        // all three offsets in [0, 32): a negative offset sets high bits of the OR
-        bool ok = ((unsigned)lx | (unsigned)ly | (unsigned)lz) < (unsigned)FB;
+        // all three offsets in [0, 32) (z: [0, 16) in transpose mode, tested as 2*lz < 32):
+        // a negative offset sets high bits of the OR
+        bool ok = ((unsigned)lx | (unsigned)ly | ((unsigned)lz << (T ? 1 : 0))) < (unsigned)FB;
         if (BND) ok = ok && f2 <= frh;
-        // byte offset of the cell: slot 0 holds the zero that rejected samples read (an
-        // inline constant in the select), the brick starts at slot 1
-        const unsigned off = mad_u24((unsigned)lx, 4u * FR_PX,
-                                     mad_u24((unsigned)ly, 4u * FR_PY, (unsigned)lz * 4u + 4u));
-        if (T) {
-          if (ok) lds_add_f32((float*)((char*)sb + off), gv);
+        if constexpr (T) {
+          const unsigned off = mad_u24((unsigned)lx, 8u * FT_PX,
+                                       mad_u24((unsigned)ly, 8u * FT_PY, (unsigned)lz * 8u));
+          if (ok)
+            __hip_atomic_fetch_add(
+                (unsigned long long*)__builtin_assume_aligned((char*)L.acc + off, 8), gq,
+                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         } else {
-          bs += *(const float*)((const char*)sb + (ok ? off : 0u));
+          // byte offset of the cell: slot 0 holds the zero that rejected samples read (an
+          // inline constant in the select), the brick starts at slot 1
+          const unsigned off = mad_u24((unsigned)lx, 4u * FR_PX,
+                                       mad_u24((unsigned)ly, 4u * FR_PY, (unsigned)lz * 4u + 4u));
+          bs += *(const float*)((const char*)L.sb + (ok ? off : 0u));
         }
       };
       // Rows past R1 and columns past the lattice end need no test: a plane's cube section
       // lies within sqrt(3) < 2 of the plane centre, inside the [-2n, 2n) lattice, so only
       // the brick test can reject a sample.
-      float ks = 0.f, kc = 0.f;  // Kahan-compensated lane sum of the section
-      if (T) {
-        // Transpose walk: the 64 samples of one step lie 4 rows or 4 columns apart, i.e. at
-        // least 4 delta = 4/n apart in the plane, more than a cell's diagonal 2 sqrt(3)/n, so
-        // no two lanes add to the same cell at once (LDS atomics to one address serialise).
-        // Band B of 32 rows in 4 phases dr: lane (r, k) takes row 32B + 4r + dr and the
-        // columns rl + dc + 4k + 32m, dc = 0..3.
-        float unused = 0.f;
-        const int nb32 = (nrows + 31) >> 5;
-        for (int B = 0; B < nb32; ++B) {
-          for (int dr = 0; dr < 4; ++dr) {
-            const int t = 32 * B + 4 * r_lane + dr;  // this lane's row within the section
-            const int steps = __builtin_amdgcn_readlane(B < 2 ? bsteps[0] : bsteps[1],
-                                                        (32 * B + dr) & 63);
-            if (steps == 0) continue;
-            const int rw = rowlo[wv][t];
-            const int rlo = (rw << 16) >> 16;
-            const double frh = (double)(rw >> 16);
-            const double f1 = (double)(SR0 + t);
-            const double Rx = S[0] + av[0] * f1, Ry = S[1] + av[1] * f1, Rz = S[2] + av[2] * f1;
-            for (int dcol = 0; dcol < 4; ++dcol) {
-              double f2 = (double)(rlo + dcol + 4 * k_lane);  // column i2, exact in binary64
-              for (int m = 0; m < steps; ++m, f2 += 32.0) sample(Rx, Ry, Rz, f2, frh, unused);
-            }
+      float ks = 0.f, kc = 0.f;  // Kahan-compensated lane sum of the section (forward)
+      // Bands of 8 rows: lane (r, k) takes row 8k' + r of band k' and the columns rlo + k +
+      // 8m.  Transpose mode takes the columns rlo + dc + 2k + 16m in two phases dc = 0, 1:
+      // the lanes of one instruction then lie 2 columns apart and fewer of them add to one
+      // cell at once (the LDS integer atomic serialises those, which costs time, not
+      // results).  Measured at 256^3: 2.66 s against 2.92 s for the forward's columns and
+      // 2.90 s for bands of 16 rows taken 2 apart (their longer rows' steps waste more).
+      int rlo_next = rowlo[wv][r_lane];
+      for (int k = 0; k < nbands; ++k) {
+        const int t = 8 * k + r_lane;  // this lane's row within the section
+        const int rlo = (rlo_next << 16) >> 16;
+        const double frh = (double)(rlo_next >> 16);  // the row's last column
+        if (k + 1 < nbands) rlo_next = rowlo[wv][t + 8];
+        const int steps = (k < 8) ? __builtin_amdgcn_readlane(bsteps[0], 8 * k)
+                                  : __builtin_amdgcn_readlane(bsteps[1], 8 * (k - 8));
+        if (steps == 0) continue;
+        const double f1 = (double)(SR0 + t);
+        const double Rx = S[0] + av[0] * f1, Ry = S[1] + av[1] * f1, Rz = S[2] + av[2] * f1;
+        float bs = 0.f;
+        if constexpr (T) {
+          const int steps2 = (steps + 1) >> 1;  // 16-column steps cover the 8-column ones
+          for (int dcol = 0; dcol < 2; ++dcol) {
+            double f2 = (double)(rlo + dcol + 2 * k_lane);  // column i2, exact in binary64
+            for (int m = 0; m < steps2; ++m, f2 += 16.0) sample(Rx, Ry, Rz, f2, frh, bs);
           }
-        }
-      } else {
-        int rlo_next = rowlo[wv][r_lane];
-        for (int k = 0; k < nbands; ++k) {
-          const int t = 8 * k + r_lane;  // this lane's row within the section
-          const int rlo = (rlo_next << 16) >> 16;
-          const double frh = (double)(rlo_next >> 16);  // the row's last column
-          if (k + 1 < nbands) rlo_next = rowlo[wv][t + 8];
-          const int steps = (k < 8) ? __builtin_amdgcn_readlane(bsteps[0], 8 * k)
-                                    : __builtin_amdgcn_readlane(bsteps[1], 8 * (k - 8));
-          if (steps == 0) continue;
-          const double f1 = (double)(SR0 + t);
-          const double Rx = S[0] + av[0] * f1, Ry = S[1] + av[1] * f1, Rz = S[2] + av[2] * f1;
-          float bs = 0.f;
+        } else {
           double f2 = (double)(rlo + k_lane);  // column i2, stepped exactly in binary64
           for (int m = 0; m < steps; ++m, f2 += 8.0) sample(Rx, Ry, Rz, f2, frh, bs);
           const float y = bs - kc;
@@ -515,7 +522,7 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
           ks = tt;
         }
       }
-      if (!T) {
+      if constexpr (!T) {
         const double tot = wave_sum_dpp((double)ks - (double)kc);
         if (lane == 0 && tot != 0.0)
           unsafeAtomicAdd(a.acc + (size_t)j * a.ndir + d, tot * delta * delta);

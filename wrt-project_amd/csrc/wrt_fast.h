@@ -22,7 +22,8 @@ typedef struct {
   int n;              /* volume edge                                          */
   int nb;             /* bricks per axis                                      */
   int bf16;           /* 1: bricks hold BF16                                  */
-  const int* tasks;   /* [ntasks][2] = brick id, direction group              */
+  const int* tasks;   /* [ntasks][2] = brick id, direction group (transpose:
+                         2 * brick id + half brick)                           */
   int ntasks;
   const double* dirs; /* [ndir][9] n, a = delta*v1, b = delta*v2              */
   int ndir;
@@ -35,7 +36,9 @@ typedef struct {
                          farthest non-zero cell plus half a cell diagonal)     */
   /* transpose only */
   const float* g;     /* [nshift][ndir] plane values                          */
-  float* part;        /* [ngroups][nb^3][WRT_BRICK^3] per-group brick partials */
+  double gscale;      /* fixed-point scale: a section adds rint(g * gscale)   */
+  long long* part;    /* [ngroups][nb^3][WRT_BRICK^3] per-group brick partials,
+                         fixed point                                          */
 } wrt_radon_fast_args;
 
 /* FP64 vol[x][y][z] -> bricks; flags[brick] += its number of non-zero values (flags zeroed by the caller);
@@ -48,12 +51,15 @@ int wrtk_radon_fast(const wrt_radon_fast_args* a, int boundary, hipStream_t st);
 /* out[i] = (float)acc[i] */
 int wrtk_acc_to_f32(const double* acc, float* out, size_t count, hipStream_t st);
 
-/* Exact transpose of the plane integrals in FP32 (a.g, a.part set; a.rho2 >= 3 so that no
- * row is clipped): every task (brick, group) stores its LDS brick to part[group][brick] */
+/* Exact transpose of the plane integrals from FP32 plane values (a.g, a.gscale, a.part set;
+ * a.rho2 >= 3 so that no row is clipped): every task (brick, half, group) stores its half
+ * brick of 64-bit fixed-point sums to part[group][brick] (cells with z in [16 half, 16 half
+ * + 16)).  The caller chooses gscale so that no cell's total over all groups exceeds 2^62. */
 int wrtk_radon_T_fast(const wrt_radon_fast_args* a, int boundary, hipStream_t st);
-/* out[x][y][z] = (float)(scale * sum over groups of part[group][brick][cell]), in FP64 */
-int wrtk_radon_T_reduce(const float* part, int ngroups, int n, int nb, double scale, float* out,
-                        hipStream_t st);
+/* out[x][y][z] = (float)(scale * sum over groups of part[group][brick][cell]): the integer
+ * sum is exact, one rounding to FP64 and one to FP32 */
+int wrtk_radon_T_reduce(const long long* part, int ngroups, int n, int nb, double scale,
+                        float* out, hipStream_t st);
 
 /* ---- the persistent batch launch: many jobs' tasks in one grid ---- */
 enum { WRT_ITEM_RADON = 0, WRT_ITEM_ADJOINT = 1 };

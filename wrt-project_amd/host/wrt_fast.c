@@ -470,20 +470,42 @@ typedef struct {
   size_t nbricks;
   int dgroup, ngroups;
   int n_interior, n_boundary;
+  int qbits;       /* fixed point: |rint(g * gscale)| <= 2^qbits */
+  double gscale;
   int* tasks;      /* device, interior then boundary */
   double* staging; /* FP64 plane values [nshift][ndir] */
   float* g32;      /* FP32 plane values */
-  float* part;     /* [ngroups][nbricks][WRT_BRICK^3] */
+  long long* part; /* [ngroups][nbricks][WRT_BRICK^3] fixed-point partials */
   float* out;      /* n^3 [x][y][z] */
 } radon_T_fast;
 
 #define RT_PART_BUDGET ((size_t)1 << 30) /* bytes of group partials at most */
 
-/* groups: about 4096 tasks, at least 16 directions per group, partials within the budget */
+/* Bits of the transpose's fixed point.  One plane puts at most 16 lattice samples into one
+ * cell: its section of the cell lies in a disk of radius sqrt(3) delta (half the cell's
+ * diagonal), which holds at most 12 points of the square lattice of spacing delta.  The
+ * planes of one direction that meet a cell have shifts within the cell's projection on the
+ * normal, an interval of width at most sqrt(3) * 2/n; w counts the most shifts any such
+ * window holds.  So no cell receives more than 16 w ndir samples, and with
+ * |q| <= 2^qbits no total, over every group, exceeds 2^62. */
+static int radon_T_qbits(const wrt_job_desc* d) {
+  const double width = sqrt(3.0) * 2.0 / d->n * (1.0 + 1e-9) + 1e-12;
+  int w = 1;
+  for (int lo = 0, hi = 0; lo < d->nshift; ++lo) { /* shifts ascending */
+    if (hi < lo) hi = lo;
+    while (hi + 1 < d->nshift && d->shift[hi + 1] - d->shift[lo] <= width) ++hi;
+    if (hi - lo + 1 > w) w = hi - lo + 1;
+  }
+  const double samples = 16.0 * w * ((double)d->nphi * d->ntheta);
+  return 62 - (int)ceil(log2(samples));
+}
+
+/* groups: about 4096 tasks (two half bricks per brick), at least 16 directions per group,
+ * partials within the budget */
 static void radon_T_groups(const wrt_job_desc* d, size_t nbricks, int* ngroups, int* dgroup) {
   const size_t ndir = (size_t)d->nphi * d->ntheta;
-  const size_t slot = nbricks * WRT_BRICK * WRT_BRICK * WRT_BRICK * sizeof(float);
-  size_t g = (4096 + nbricks - 1) / nbricks;
+  const size_t slot = nbricks * WRT_BRICK * WRT_BRICK * WRT_BRICK * sizeof(long long);
+  size_t g = (4096 + 2 * nbricks - 1) / (2 * nbricks);
   const size_t by_dirs = ndir / 16 ? ndir / 16 : 1;
   const size_t by_mem = RT_PART_BUDGET / slot ? RT_PART_BUDGET / slot : 1;
   if (g > by_dirs) g = by_dirs;
@@ -500,8 +522,8 @@ static size_t radon_T_fast_bytes(const wrt_job_desc* d) {
   int ng, dg;
   radon_T_groups(d, nbr, &ng, &dg);
   const size_t nplanes = (size_t)d->nshift * d->nphi * d->ntheta;
-  return nplanes * 12 + (size_t)ng * nbr * WRT_BRICK * WRT_BRICK * WRT_BRICK * 4 +
-         2 * nbr * (size_t)ng * sizeof(int) + (size_t)d->n * d->n * d->n * 4;
+  return nplanes * 12 + (size_t)ng * nbr * WRT_BRICK * WRT_BRICK * WRT_BRICK * 8 +
+         4 * nbr * (size_t)ng * sizeof(int) + (size_t)d->n * d->n * d->n * 4;
 }
 
 static void radon_T_fast_destroy(wrt_job* j) {
@@ -525,12 +547,14 @@ static int radon_T_fast_create(wrt_job* j) {
   f->nbricks = (size_t)f->nb * f->nb * f->nb;
   radon_T_groups(d, f->nbricks, &f->ngroups, &f->dgroup);
   const size_t nplanes = (size_t)d->nshift * d->nphi * d->ntheta;
-  const size_t ntasks = f->nbricks * (size_t)f->ngroups;
+  const size_t ntasks = 2 * f->nbricks * (size_t)f->ngroups;
   if (ntasks > 0x7fffffff) return WRT_E_ARG;
+  f->qbits = radon_T_qbits(d);
+  if (f->qbits < 24) return WRT_E_ARG; /* more samples per cell than 64 bits can hold */
   if (hipMalloc((void**)&f->tasks, 2 * ntasks * sizeof(int)) != hipSuccess ||
       hipMalloc((void**)&f->staging, nplanes * 8) != hipSuccess ||
       hipMalloc((void**)&f->g32, nplanes * 4) != hipSuccess ||
-      hipMalloc((void**)&f->part, (size_t)f->ngroups * f->nbricks * WRT_BRICK * WRT_BRICK * WRT_BRICK * 4) != hipSuccess ||
+      hipMalloc((void**)&f->part, (size_t)f->ngroups * f->nbricks * WRT_BRICK * WRT_BRICK * WRT_BRICK * 8) != hipSuccess ||
       hipMalloc((void**)&f->out, (size_t)d->n * d->n * d->n * 4) != hipSuccess)
     return WRT_E_NOMEM;
   /* bricks nearest the centre first (planes with |s| <= 1 cut them from every direction, so
@@ -567,13 +591,17 @@ static int radon_T_fast_create(wrt_job* j) {
     for (size_t i = 0; i < f->nbricks; ++i) {
       const int b = order[i];
       const int bx = b / (nb * nb), by = (b / nb) % nb, bz = b % nb;
-      const int bnd = bx == 0 || by == 0 || bz == 0 || (bx + 1) * WRT_BRICK >= d->n ||
-                      (by + 1) * WRT_BRICK >= d->n || (bz + 1) * WRT_BRICK >= d->n;
-      if (bnd != pass) continue;
-      for (int g = 0; g < f->ngroups; ++g) {
-        t[2 * k] = b;
-        t[2 * k + 1] = g;
-        ++k;
+      for (int zh = 0; zh < 2; ++zh) {
+        const int z0 = bz * WRT_BRICK + zh * (WRT_BRICK / 2), z1 = z0 + WRT_BRICK / 2;
+        if (z0 >= d->n) continue; /* the half brick lies outside the cube */
+        const int bnd = bx == 0 || by == 0 || z0 == 0 || (bx + 1) * WRT_BRICK >= d->n ||
+                        (by + 1) * WRT_BRICK >= d->n || z1 >= d->n;
+        if (bnd != pass) continue;
+        for (int g = 0; g < f->ngroups; ++g) {
+          t[2 * k] = 2 * b + zh;
+          t[2 * k + 1] = g;
+          ++k;
+        }
       }
     }
     if (pass == 0) f->n_interior = (int)k;
@@ -592,6 +620,18 @@ static int radon_T_fast_create(wrt_job* j) {
 static int radon_T_fast_load(wrt_job* j, const double* in) {
   radon_T_fast* f = (radon_T_fast*)j->fast;
   const size_t nplanes = (size_t)j->d.nshift * j->d.nphi * j->d.ntheta;
+  /* fixed-point scale from the largest |value| as FP32 holds it: |rint(g * gscale)| is at
+   * most 2^qbits */
+  float gmax = 0.f;
+  int bad = 0;
+#pragma omp parallel for reduction(max : gmax) reduction(| : bad)
+  for (long long i = 0; i < (long long)nplanes; ++i) {
+    const float v = fabsf((float)in[i]);
+    if (!isfinite(v)) bad = 1;
+    else if (v > gmax) gmax = v;
+  }
+  if (bad) return WRT_E_ARG;
+  f->gscale = gmax > 0.f ? ldexp(1.0, f->qbits) / (double)gmax : 1.0;
   HIPCHK(hipMemcpyAsync(f->staging, in, nplanes * 8, hipMemcpyHostToDevice, j->st));
   const int rc = wrtk_f64_to_f32(f->staging, f->g32, nplanes, j->st);
   if (rc != WRT_OK) return rc;
@@ -614,6 +654,7 @@ static int radon_T_fast_enqueue(wrt_job* j) {
   a.nshift = d->nshift;
   a.rho2 = 4.0; /* no support clipping: every cell of the cube receives */
   a.g = f->g32;
+  a.gscale = f->gscale;
   a.part = f->part;
   HIPCHK(hipEventRecord(j->e1, st));
   a.tasks = f->tasks;
@@ -626,7 +667,8 @@ static int radon_T_fast_enqueue(wrt_job* j) {
   if (rc != WRT_OK) return rc;
   HIPCHK(hipEventRecord(j->e3, st));
   const double delta = (1.0 / (d->n));
-  return wrtk_radon_T_reduce(f->part, f->ngroups, d->n, f->nb, delta * delta, f->out, st);
+  return wrtk_radon_T_reduce(f->part, f->ngroups, d->n, f->nb, delta * delta / f->gscale,
+                             f->out, st);
 }
 
 /* -------------------------------------------------------------------- RAY_T --- */
