@@ -10,7 +10,8 @@ the FP64 dot-product test in test_gpu_parity.py):
   single flipped index decision is an error of at least a quarter of the largest value;
 - the dot-product test of the FP32 pairs, <A f, g> = <f, A^T g>, to 1e-6 relative to
   ||A f|| ||g||;
-- n not a multiple of the 32^3 brick, and the ray form above the LDS plane limit (n > 181);
+- n not a multiple of the 32^3 brick, and the ray form at and above the LDS plane limit
+  (n = 143, FP64 plane);
 - the plane form gives the same bits on every launch (integer sums), and stays exact on a
   shift grid whose shifts crowd into one cell's width (the fixed point's bound).
 """
@@ -72,9 +73,10 @@ def test_radon_f32_dot_product(ctx, n):
     assert abs(lhs - rhs) <= 1e-6 * np.linalg.norm(Af) * np.linalg.norm(g), (lhs, rhs)
 
 
-@pytest.mark.parametrize("n", [17, 33, 48, 190])
+@pytest.mark.parametrize("n", [17, 33, 48, 143, 150, 190])
 def test_ray_T_f32_matches_plain(ctx, n):
-    # n - 1 a power of two (exact scaling) and not; 190: no LDS plane
+    # n - 1 a power of two (exact scaling) and not; 143: the largest LDS plane (FP64);
+    # 150, 190: none
     s, phi = wrt.ray_grid_init(n + 1 if n < 100 else 65, 16)
     z = np.concatenate([wrt.zslice_grid(12), [1.5, -1.0000001]])
     g = wrt.gaussian((len(z), len(s), len(phi)), seed=n)

@@ -24,6 +24,8 @@
 
 #define RAY_THREADS 512
 #define RAY_LDS_N 181  // largest n whose n^2 FP32 plane fits the 128 KiB stage
+#define RAY_T_LDS_N 143  // largest n whose n^2 FP64 plane fits the 160 KiB LDS
+#define RAY_T_THREADS 512
 
 // one axis: the reference's index and fraction of coordinate x (|x| <= 1)
 template <bool POW2>
@@ -177,19 +179,25 @@ extern "C" int wrtk_ray_f32(const float* vol32, int n, const int* zi, const doub
 // slice's plane; the plane then goes to the two z-nodes of the slice with weights 1 - cz
 // and cz, times delta.  Index and fraction decisions are the forward's (the reference's).
 // Work layout: blockIdx.y = slice, blockIdx.x = one of the slice's ray blocks; a thread
-// walks rays t = blockIdx.x * RAY_THREADS + threadIdx.x + k * gridDim.x * RAY_THREADS.
-// For n <= 181 the plane is an LDS FP32 accumulator (lds_add_f32), added to the FP64
-// output accumulator once per workgroup; larger volumes add every point's weights to the
-// FP64 accumulator directly.  acc: n^3 FP64 [x][y][z], zeroed by the caller.
+// walks rays t = blockIdx.x * RAY_T_THREADS + threadIdx.x + k * gridDim.x * RAY_T_THREADS,
+// ray t being shift (r * sstride) mod nshift, r = t mod nshift, of direction t / nshift:
+// with sstride coprime to nshift this visits every ray once, and puts neighbouring lanes on
+// rays sstride shifts apart, whose cells (and corners) mostly differ, so that fewer lanes
+// of one instruction add to one LDS cell (the LDS serialises those).
+// For n <= 143 the plane is an LDS FP64 accumulator (ds_add_f64, which needs no returned
+// value), added to the FP64 output accumulator once per workgroup; larger volumes add every
+// point's weights to the FP64 accumulator directly.  acc: n^3 FP64 [x][y][z], zeroed by the
+// caller.
 template <bool LDS, bool POW2>
-__global__ __launch_bounds__(RAY_THREADS) void k_ray_T_f32(const float* __restrict__ g, int n,
+__global__ __launch_bounds__(RAY_T_THREADS) void k_ray_T_f32(const float* __restrict__ g, int n,
                                                            const int* __restrict__ zi,
                                                            const double* __restrict__ zc,
                                                            const double* __restrict__ shift,
                                                            int nshift,
                                                            const double* __restrict__ trig,
-                                                           int nphi, double* __restrict__ acc) {
-  extern __shared__ float plane[];
+                                                           int nphi, int sstride,
+                                                           double* __restrict__ acc) {
+  extern __shared__ double tplane[];
   const int iz = blockIdx.y;
   const int kz = zi[iz];
   if (kz < 0) return;  // |z| > 1: no point of the slice passes the cube test
@@ -197,15 +205,15 @@ __global__ __launch_bounds__(RAY_THREADS) void k_ray_T_f32(const float* __restri
   const double delta = (2.0 / (n - 1)) / (2.0);
   const size_t nn = (size_t)n * n;
   if (LDS) {
-    for (int k = threadIdx.x; k < (int)nn; k += RAY_THREADS) plane[k] = 0.f;
+    for (int k = threadIdx.x; k < (int)nn; k += RAY_T_THREADS) tplane[k] = 0.0;
     __syncthreads();
   }
   const long long nrays = (long long)nshift * nphi;
   const double dg = 2.0 / (n - 1), inv_dg = 1.0 / dg, h = (n - 1) / 2.0;
   const double w0 = (1.0 - czd) * delta, w1 = czd * delta;  // z-node weights (global path)
-  for (long long t = (long long)blockIdx.x * RAY_THREADS + threadIdx.x; t < nrays;
-       t += (long long)gridDim.x * RAY_THREADS) {
-    const int is = (int)(t % nshift), ip = (int)(t / nshift);
+  for (long long t = (long long)blockIdx.x * RAY_T_THREADS + threadIdx.x; t < nrays;
+       t += (long long)gridDim.x * RAY_T_THREADS) {
+    const int is = (int)((t % nshift) * sstride % nshift), ip = (int)(t / nshift);
     const float gv = g[((size_t)iz * nshift + is) * nphi + ip];
     if (gv == 0.f) continue;
     const double c = trig[4 * ip], sn = trig[4 * ip + 1], dx = trig[4 * ip + 2],
@@ -227,10 +235,12 @@ __global__ __launch_bounds__(RAY_THREADS) void k_ray_T_f32(const float* __restri
       const float w00 = gy0 * (1.f - fx), w10 = gy0 * fx, w01 = gy1 * (1.f - fx), w11 = gy1 * fx;
       const int o = ix * n + iy;
       if (LDS) {
-        lds_add_f32(&plane[o], w00);
-        lds_add_f32(&plane[o + n], w10);
-        lds_add_f32(&plane[o + 1], w01);
-        lds_add_f32(&plane[o + n + 1], w11);
+        // LDS FP64 atomic adds: no returned value to wait for (the FP32 LDS atomic runs lane
+        // by lane on gfx950)
+        unsafeAtomicAdd(&tplane[o], (double)w00);
+        unsafeAtomicAdd(&tplane[o + n], (double)w10);
+        unsafeAtomicAdd(&tplane[o + 1], (double)w01);
+        unsafeAtomicAdd(&tplane[o + n + 1], (double)w11);
       } else {
         double* p = acc + (size_t)o * n + kz;  // [x][y][z]
         const float wv[4] = {w00, w10, w01, w11};
@@ -246,12 +256,12 @@ __global__ __launch_bounds__(RAY_THREADS) void k_ray_T_f32(const float* __restri
   }
   if (LDS) {
     __syncthreads();
-    for (int k = threadIdx.x; k < (int)nn; k += RAY_THREADS) {
-      const float v = plane[k];
-      if (v == 0.f) continue;
+    for (int k = threadIdx.x; k < (int)nn; k += RAY_T_THREADS) {
+      const double v = tplane[k];
+      if (v == 0.0) continue;
       double* p = acc + (size_t)k * n + kz;  // k = x*n + y
-      unsafeAtomicAdd(p, (double)v * w0);
-      unsafeAtomicAdd(p + 1, (double)v * w1);
+      unsafeAtomicAdd(p, v * w0);
+      unsafeAtomicAdd(p + 1, v * w1);
     }
   }
 }
@@ -261,24 +271,34 @@ static void launch_ray_T(const float* g, int n, const int* zi, const double* zc,
                          const double* shift, int nshift, const double* trig, int nphi,
                          double* acc, hipStream_t st) {
   const long long nrays = (long long)nshift * nphi;
-  // about 2048 workgroups in all, at least one ray per thread of each
-  long long per = (2048 + nz - 1) / nz;
-  const long long most = (nrays + RAY_THREADS - 1) / RAY_THREADS;
+  // LDS plane: about 512 workgroups in all (two rounds of one per CU), since every workgroup
+  // ends by adding its n^2 plane to the volume with global atomics; without: about 2048.
+  // At least one ray per thread of each.
+  long long per = ((LDS ? 512 : 2048) + nz - 1) / nz;
+  const long long most = (nrays + RAY_T_THREADS - 1) / RAY_T_THREADS;
   if (per > most) per = most;
   if (per < 1) per = 1;
   const dim3 grid((unsigned)per, (unsigned)nz);
-  const size_t lds = LDS ? (size_t)n * n * sizeof(float) : 0;
+  const size_t lds = LDS ? (size_t)n * n * sizeof(double) : 0;
+  // the smallest stride >= 4 coprime to nshift (rays 4 shifts apart: at C1's 257 shifts
+  // over 128 cells, 2 cells apart; a stride of 8 measured the same)
+  int sstride = 4;
+  for (;; ++sstride) {
+    int a = sstride, b = nshift;
+    while (b) { const int r = a % b; a = b; b = r; }
+    if (a == 1) break;
+  }
   const int m = n - 1;
   if ((m & (m - 1)) == 0) {
     if (lds > 65536)
       (void)hipFuncSetAttribute((const void*)k_ray_T_f32<LDS, true>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    k_ray_T_f32<LDS, true><<<grid, RAY_THREADS, lds, st>>>(g, n, zi, zc, shift, nshift, trig, nphi, acc);
+    k_ray_T_f32<LDS, true><<<grid, RAY_T_THREADS, lds, st>>>(g, n, zi, zc, shift, nshift, trig, nphi, sstride, acc);
   } else {
     if (lds > 65536)
       (void)hipFuncSetAttribute((const void*)k_ray_T_f32<LDS, false>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    k_ray_T_f32<LDS, false><<<grid, RAY_THREADS, lds, st>>>(g, n, zi, zc, shift, nshift, trig, nphi, acc);
+    k_ray_T_f32<LDS, false><<<grid, RAY_T_THREADS, lds, st>>>(g, n, zi, zc, shift, nshift, trig, nphi, sstride, acc);
   }
 }
 
@@ -286,7 +306,7 @@ extern "C" int wrtk_ray_T_f32(const float* g, int n, const int* zi, const double
                               const double* shift, int nshift, const double* trig, int nphi,
                               double* acc, hipStream_t st) {
   if (n < 2 || nz < 1 || nz > 65535 || nshift < 1 || nphi < 1) return WRT_E_ARG;
-  if (n <= RAY_LDS_N)
+  if (n <= RAY_T_LDS_N)
     launch_ray_T<true>(g, n, zi, zc, nz, shift, nshift, trig, nphi, acc, st);
   else
     launch_ray_T<false>(g, n, zi, zc, nz, shift, nshift, trig, nphi, acc, st);

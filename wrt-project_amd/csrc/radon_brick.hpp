@@ -20,7 +20,9 @@
 // every sample whose cell lies in the half brick adds the section's value, quantized to a
 // fixed point q = rint(g * scale), to the cell with an LDS 64-bit integer atomic.  Integer
 // sums are exact and associative, so the result does not depend on the order in which waves
-// reach a cell; the scale keeps every cell's total within 2^62 (the caller's bound).
+// reach a cell; the scale keeps every cell's total within 2^62 (the caller's bound).  (On
+// gfx950 ds_add_f32 runs lane by lane, 193 clocks per wave instruction per CU against 7 for
+// ds_add_u64 and 15 for a read + ds_cmpst_rtn_b32 loop, distinct addresses.)
 #ifndef WRT_RADON_BRICK_HPP
 #define WRT_RADON_BRICK_HPP
 #include "wrt_common.hpp"
@@ -93,24 +95,6 @@ __device__ __forceinline__ double wave_sum_dpp(double v) {
   const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
   const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
   return __hiloint2double(hi, lo);
-}
-
-// v added to the LDS float at p by compare-and-swap.  gfx950 executes ds_add_f32 lane by
-// lane: measured 193 clocks per wave instruction per CU, against 5 for ds_add_u32, 7 for
-// ds_add_u64, 9 for ds_add_f64 and 15 for this read + ds_cmpst_rtn_b32 loop (distinct
-// addresses).  The loop repeats only for lanes whose cell another lane or wave changed in
-// between.
-__device__ __forceinline__ void lds_add_f32(float* p, float v) {
-  unsigned* const q = (unsigned*)p;
-  unsigned old = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-  for (;;) {
-    unsigned expect = old;
-    const unsigned nv = __float_as_uint(__uint_as_float(old) + v);
-    if (__hip_atomic_compare_exchange_strong(q, &expect, nv, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_WORKGROUP))
-      break;
-    old = expect;
-  }
 }
 
 __device__ __forceinline__ float bf16_to_f32(unsigned short v) {
