@@ -190,3 +190,23 @@ def test_fast_ray_f32_rays_on_cube_faces(ctx, n):
     # the face rays themselves
     for js in (0, len(s) - 1):
         assert orc.rel_l2(got[:, js, :], ref[:, js, :]) <= 1e-5
+
+
+@pytest.mark.parametrize("n", [32, 48, 64])
+def test_fast_f32_index_decisions_exact(ctx, n):
+    # Small integer values make every sum exact (FP32 band partials, FP64 totals), so the
+    # fast form must give the plain form's integer sum on every plane: a single sample put
+    # into another cell changes it (unless the two cells hold equal values).  Random and
+    # structured directions, shifts on cell faces and on the cube's faces.
+    rng = np.random.default_rng(100 + n)
+    vol = rng.integers(-8, 9, size=(n, n, n)).astype(np.float64)
+    phi = np.concatenate([[0.0, np.pi / 4, np.pi / 2, np.pi, 1.5 * np.pi], rng.uniform(0, 2 * np.pi, 5)])
+    _, th_gl, _, _ = ctx.radon_grid_init(4, 6, 3)
+    th = np.concatenate([th_gl, [np.pi / 2, np.pi / 4, 0.0], np.arccos(rng.uniform(-1, 1, 3))])
+    s = np.unique(np.concatenate([-1.0 + np.arange(0, n + 1, 2) * (2.0 / n), [-1.0, 1.0, 0.0],
+                                  rng.uniform(-1, 1, 9)]))
+    ref, _ = ctx.radon_grid(vol, phi, th, s, prec="f64")
+    got, _ = ctx.radon_grid(vol, phi, th, s, prec="f32")
+    d2 = (1.0 / n) ** 2
+    assert np.array_equal(np.rint(got / d2), np.rint(ref / d2))
+    assert np.max(np.abs(ref / d2 - np.rint(ref / d2))) < 1e-6

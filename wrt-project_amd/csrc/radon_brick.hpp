@@ -12,8 +12,8 @@
 // (the per-column products b*i2 are formed in registers, the division is an exact
 // power-of-two scaling or a corrected reciprocal), so only the value rounding (FP32/BF16)
 // and the summation order differ from the plain form.  Samples whose cell lies in the
-// brick add its FP32 value to a per-band partial, folded into a Kahan-compensated FP32
-// lane sum; a section's lane sums are reduced over the wavefront in FP64 and added to the
+// brick add its FP32 value to a per-band FP32 partial, folded into an FP64 lane sum; a
+// section's lane sums are reduced over the wavefront in FP64 and added to the
 // plane's FP64 accumulator (one atomic per section and brick).
 //
 // Transpose mode works on half bricks (32 x 32 x 16 cells) of 64-bit integer accumulators:
@@ -83,6 +83,15 @@ __device__ __forceinline__ int row_max16(int v) {
   v = max(v, dpp_ror<0x122>(v));
   v = max(v, dpp_ror<0x124>(v));
   return max(v, dpp_ror<0x128>(v));
+}
+
+// max over each aligned group of 8 lanes, every lane of the group ends with it: quad_perm
+// [1,0,3,2] (xor 1), quad_perm [2,3,0,1] (xor 2), then row_half_mirror (lane i of a half
+// row with lane 7 - i, which lies in the other quad)
+__device__ __forceinline__ int group8_max(int v) {
+  v = max(v, dpp_ror<0xB1>(v));
+  v = max(v, dpp_ror<0x4E>(v));
+  return max(v, dpp_ror<0x141>(v));
 }
 
 __device__ __forceinline__ double wave_sum_dpp(double v) {
@@ -319,6 +328,17 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
       }
     }
     const double ia = 1.0 / sqrt(aa);
+    // Fixed-point steps of the cell coordinate q_c = (x_c + 1) h (h = n/2, 32 fraction bits)
+    // per row (a_c h) and per column (b_c h); |a_c h|, |b_c h| <= 1/2, clamped into int32
+    // (error at most 2^-32 each).  Eight columns for the column walk.
+    int Afx[3], Dfx[3];
+    long long D8[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      Afx[c] = (int)fmin(fmax(rint(av[c] * h * 4294967296.0), -2147483648.0), 2147483647.0);
+      Dfx[c] = (int)fmin(fmax(rint(bv[c] * h * 4294967296.0), -2147483648.0), 2147483647.0);
+      D8[c] = 8ll * Dfx[c];
+    }
     for (int j = j0; j <= j1; ++j) {
       unsigned long long gq = 0ull;  // transpose: the section's quantized value
       if constexpr (T) {
@@ -332,7 +352,6 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
       // foot point s*n, where row 0 passes: rows farther than that (+1) are empty
       const double rc2 = a.rho2 - s * s;
       if (rc2 < 0.0) continue;
-      const double rr = sqrt(rc2) * ia + 1.0;
       const double S[3] = {s * nrm[0], s * nrm[1], s * nrm[2]};
       // rows of this section: the section polygon's vertices lie on the box edges; the row
       // of a point p of the plane is a.p / |a|^2 (a is orthogonal to n and b), +-2 rows margin
@@ -344,7 +363,12 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
       }
       r_lo = __builtin_amdgcn_readlane(row_min16(r_lo), 0);
       r_hi = __builtin_amdgcn_readlane(row_max16(r_hi), 0);
-      const int SR0 = max(max(R0, r_lo), (int)floor(-rr)), SR1 = min(min(R1, r_hi), (int)ceil(rr));
+      int SR0 = max(R0, r_lo), SR1 = min(R1, r_hi);
+      if (!inside) {  // a brick inside the sphere: its rows lie within the circle
+        const double rr = sqrt(rc2) * ia + 1.0;
+        SR0 = max(SR0, (int)floor(-rr));
+        SR1 = min(SR1, (int)ceil(rr));
+      }
       if (SR0 > SR1) continue;
       const int nrows = SR1 - SR0 + 1;  // <= 122 by geometry (brick diameter + margins)
       const int nbands = (nrows + 7) >> 3;
@@ -413,47 +437,75 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
         // first and last column, 16 bits each (|i2| <= 2n < 2^15)
         rowlo[wv][t] = (int)(((unsigned)rl & 0xffffu) | ((unsigned)rh << 16));
         // max over the 8 consecutive rows of a band (lanes 8b .. 8b+7)
-        nst = max(nst, __shfl_xor(nst, 1, 64));
-        nst = max(nst, __shfl_xor(nst, 2, 64));
-        nst = max(nst, __shfl_xor(nst, 4, 64));
-        bsteps[half] = nst;
+        bsteps[half] = group8_max(nst);
       }
       if (__ballot((bsteps[0] | bsteps[1]) != 0) == 0)
         continue;  // no row of the section meets the brick and the sphere
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      // One sample at column f2 of the row whose terms are (Rx, Ry, Rz); frh is the row's
-      // last column (face bricks).  Forward: adds the cell's value to bs; transpose: adds the
-      // section's quantized value to the cell.
-      auto sample = [&](double Rx, double Ry, double Rz, double f2, double frh, float& bs) {
-        // per-column products b*i2, rounded as the reference rounds delta*v2*i2
-        const double x = Rx + bv[0] * f2, y = Ry + bv[1] * f2, z = Rz + bv[2] * f2;
-        double qx, qy, qz;
-        if (POW2) {
-          // RN(x + 1) * h == RN(x*h + h) for h = n/2 a power of two: scaling by 2^k
-          // commutes with rounding, so one fma gives the reference's quotient exactly
-          qx = fma(x, h, h); qy = fma(y, h, h); qz = fma(z, h, h);
-        } else {
-          qx = div_rn(x + 1.0, dc, inv_dc);
-          qy = div_rn(y + 1.0, dc, inv_dc);
-          qz = div_rn(z + 1.0, dc, inv_dc);
+      // Cell decisions in fixed point.  Q_c holds (x_c + 1) h - b0_c with 32 fraction bits,
+      // where x_c is the sample's coordinate: Q = Qb + t A + (col - C0) D, Qb the value at
+      // (row SR0, column C0) converted from binary64.  Against the reference's quotient q
+      // (x rounded as radon_grid/src/integral.c:47-51 rounds it, then (x + 1) / (2/n)) the
+      // error is at most (1 + t + col - C0) 2^-32 (t < 128 rows; col - C0 < 160: a brick's
+      // section spans at most 64 sqrt(3) + 6 columns, plus a step's overshoot) plus the
+      // roundings of q and of Qb (below 1e-10 for n < 16384): under 2^-23.  So wherever the fraction of Q lies at least 2^-22 from 0 and 1, the
+      // integer part of Q is the reference's truncated index (less b0); the few samples
+      // nearer a cell boundary take the exact binary64 decision instead.
+      long long Qb[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const double b0c = c == 0 ? fb0x : (c == 1 ? fb0y : fb0z);
+        const double q = ((S[c] + av[c] * (double)SR0) + bv[c] * (double)C0 + 1.0) * h - b0c;
+        const double fl = floor(q);
+        const double fr = fmin(rint((q - fl) * 4294967296.0), 4294967295.0);
+        Qb[c] = (long long)fl * 4294967296ll + (long long)fr;
+      }
+      // One sample: Q the lane's fixed-point coordinates, (f1, f2) its row and column (for
+      // the exact decision), m its step and mlast the last step inside the row (face
+      // bricks).  Forward: adds the cell's value to bs; transpose: adds the section's
+      // quantized value to the cell.
+      auto sample = [&](const long long (&Q)[3], double f1, double f2, int m, int mlast,
+                        float& bs) {
+        int lx = (int)(Q[0] >> 32), ly = (int)(Q[1] >> 32), lz = (int)(Q[2] >> 32);
+        const unsigned u = min(min((unsigned)Q[0] + 1024u, (unsigned)Q[1] + 1024u),
+                               (unsigned)Q[2] + 1024u);
+        if (u < 2048u) {
+          // within 2^-22 of a cell boundary: the reference's rounding, in binary64 (the
+          // per-column products b*i2 rounded as the reference rounds delta*v2*i2)
+          const double Rx = S[0] + av[0] * f1, Ry = S[1] + av[1] * f1, Rz = S[2] + av[2] * f1;
+          const double x = Rx + bv[0] * f2, y = Ry + bv[1] * f2, z = Rz + bv[2] * f2;
+          double qx, qy, qz;
+          if (POW2) {
+            // RN(x + 1) * h == RN(x*h + h) for h = n/2 a power of two: scaling by 2^k
+            // commutes with rounding, so one fma gives the reference's quotient exactly
+            qx = fma(x, h, h); qy = fma(y, h, h); qz = fma(z, h, h);
+          } else {
+            qx = div_rn(x + 1.0, dc, inv_dc);
+            qy = div_rn(y + 1.0, dc, inv_dc);
+            qz = div_rn(z + 1.0, dc, inv_dc);
+          }
+          // truncation as in zero_interp; for an interior brick a negative quotient (x < -1)
+          // truncates to 0 <= b0 - 32 and a quotient above n lands past the brick, so the
+          // brick test below also is the cube test.  Face bricks: the row ends hold the
+          // cube test, and x == 1 (index n) maps to n - 1.
+          int ix = (int)qx, iy = (int)qy, iz = (int)qz;
+          if (BND) {
+            ix = min(ix, cxh);
+            iy = min(iy, cyh);
+            iz = min(iz, czh);
+          }
+          lx = ix - b0x;
+          ly = iy - b0y;
+          lz = iz - b0z;
         }
-        // truncation as in zero_interp; for an interior brick a negative quotient (x < -1)
-        // truncates to 0 <= b0 - 32 and a quotient above n lands past the brick, so the
-        // brick test below also is the cube test.  Face bricks: the row ends hold the cube
-        // test, and x == 1 (index n) maps to n - 1.
-        int ix = (int)qx, iy = (int)qy, iz = (int)qz;
-        if (BND) {
-          ix = min(ix, cxh);
-          iy = min(iy, cyh);
-          iz = min(iz, czh);
-        }
-        const int lx = ix - b0x, ly = iy - b0y, lz = iz - b0z;
         // all three offsets in [0, 32) (z: [0, 16) in transpose mode, tested as 2*lz < 32):
-        // a negative offset sets high bits of the OR
+        // a negative offset sets high bits of the OR.  (Away from a boundary the fixed point
+        // floors where the reference truncates: they differ only for -1 < q < 0, x < -1,
+        // which an interior brick rejects either way and a face brick's row never holds.)
         bool ok = ((unsigned)lx | (unsigned)ly | ((unsigned)lz << (T ? 1 : 0))) < (unsigned)FB;
-        if (BND) ok = ok && f2 <= frh;
+        if (BND) ok = ok && m <= mlast;
         if constexpr (T) {
           const unsigned off = mad_u24((unsigned)lx, 8u * FT_PX,
                                        mad_u24((unsigned)ly, 8u * FT_PY, (unsigned)lz * 8u));
@@ -472,7 +524,7 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
       // Rows past R1 and columns past the lattice end need no test: a plane's cube section
       // lies within sqrt(3) < 2 of the plane centre, inside the [-2n, 2n) lattice, so only
       // the brick test can reject a sample.
-      float ks = 0.f, kc = 0.f;  // Kahan-compensated lane sum of the section (forward)
+      double ks = 0.0;  // lane sum of the section's band sums (forward)
       // Bands of 8 rows: lane (r, k) takes row 8k' + r of band k' and the columns rlo + k +
       // 8m.  Transpose mode takes the columns rlo + dc + 2k + 16m in two phases dc = 0, 1:
       // the lanes of one instruction then lie 2 columns apart and fewer of them add to one
@@ -489,25 +541,40 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
                                   : __builtin_amdgcn_readlane(bsteps[1], 8 * (k - 8));
         if (steps == 0) continue;
         const double f1 = (double)(SR0 + t);
-        const double Rx = S[0] + av[0] * f1, Ry = S[1] + av[1] * f1, Rz = S[2] + av[2] * f1;
+        const int rhi = (int)frh;
         float bs = 0.f;
         if constexpr (T) {
           const int steps2 = (steps + 1) >> 1;  // 16-column steps cover the 8-column ones
           for (int dcol = 0; dcol < 2; ++dcol) {
-            double f2 = (double)(rlo + dcol + 2 * k_lane);  // column i2, exact in binary64
-            for (int m = 0; m < steps2; ++m, f2 += 16.0) sample(Rx, Ry, Rz, f2, frh, bs);
+            const int c0 = rlo + dcol + 2 * k_lane;  // first column i2 of the lane
+            long long Q[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+              Q[c] = Qb[c] + (long long)t * Afx[c] + (long long)(c0 - C0) * Dfx[c];
+            const int mlast = (rhi - c0) >> 4;  // floor: negative when the row ends before
+            for (int m = 0; m < steps2; ++m) {
+              sample(Q, f1, (double)(c0 + 16 * m), m, mlast, bs);
+#pragma unroll
+              for (int c = 0; c < 3; ++c) Q[c] += 2 * D8[c];
+            }
           }
         } else {
-          double f2 = (double)(rlo + k_lane);  // column i2, stepped exactly in binary64
-          for (int m = 0; m < steps; ++m, f2 += 8.0) sample(Rx, Ry, Rz, f2, frh, bs);
-          const float y = bs - kc;
-          const float tt = ks + y;
-          kc = (tt - ks) - y;
-          ks = tt;
+          const int c0 = rlo + k_lane;  // first column i2 of the lane
+          long long Q[3];
+#pragma unroll
+          for (int c = 0; c < 3; ++c)
+            Q[c] = Qb[c] + (long long)t * Afx[c] + (long long)(c0 - C0) * Dfx[c];
+          const int mlast = (rhi - c0) >> 3;
+          for (int m = 0; m < steps; ++m) {
+            sample(Q, f1, (double)(c0 + 8 * m), m, mlast, bs);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) Q[c] += D8[c];
+          }
+          ks += (double)bs;
         }
       }
       if constexpr (!T) {
-        const double tot = wave_sum_dpp((double)ks - (double)kc);
+        const double tot = wave_sum_dpp(ks);
         if (lane == 0 && tot != 0.0)
           unsafeAtomicAdd(a.acc + (size_t)j * a.ndir + d, tot * delta * delta);
       }
