@@ -106,6 +106,12 @@ __device__ __forceinline__ double wave_sum_dpp(double v) {
   return __hiloint2double(hi, lo);
 }
 
+// a wave-uniform double moved into scalar registers (frees vector registers)
+__device__ __forceinline__ double uniform_f64(double v) {
+  return __hiloint2double(__builtin_amdgcn_readfirstlane(__double2hiint(v)),
+                          __builtin_amdgcn_readfirstlane(__double2loint(v)));
+}
+
 __device__ __forceinline__ float bf16_to_f32(unsigned short v) {
   return __uint_as_float(((unsigned)v) << 16);
 }
@@ -254,9 +260,9 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
     d = __builtin_amdgcn_readfirstlane(d);
     if (d >= d_end) break;
     const double* D = a.dirs + 9 * (size_t)d;
-    const double nrm[3] = {D[0], D[1], D[2]};
-    const double av[3] = {D[3], D[4], D[5]};
-    const double bv[3] = {D[6], D[7], D[8]};
+    const double nrm[3] = {uniform_f64(D[0]), uniform_f64(D[1]), uniform_f64(D[2])};
+    const double av[3] = {uniform_f64(D[3]), uniform_f64(D[4]), uniform_f64(D[5])};
+    const double bv[3] = {uniform_f64(D[6]), uniform_f64(D[7]), uniform_f64(D[8])};
     // projections of the brick box on a, b and the normal (box corners extremise them)
     double pa0 = 0, pa1 = 0, pb0 = 0, pb1 = 0, pn0 = 0, pn1 = 0;
 #pragma unroll
@@ -334,9 +340,11 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
     int Afx[3], Dfx[3];
     long long D8[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      Afx[c] = (int)fmin(fmax(rint(av[c] * h * 4294967296.0), -2147483648.0), 2147483647.0);
-      Dfx[c] = (int)fmin(fmax(rint(bv[c] * h * 4294967296.0), -2147483648.0), 2147483647.0);
+    for (int c = 0; c < 3; ++c) {  // wave-uniform: held in scalar registers
+      Afx[c] = __builtin_amdgcn_readfirstlane(
+          (int)fmin(fmax(rint(av[c] * h * 4294967296.0), -2147483648.0), 2147483647.0));
+      Dfx[c] = __builtin_amdgcn_readfirstlane(
+          (int)fmin(fmax(rint(bv[c] * h * 4294967296.0), -2147483648.0), 2147483647.0));
       D8[c] = 8ll * Dfx[c];
     }
     for (int j = j0; j <= j1; ++j) {
@@ -352,7 +360,8 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
       // foot point s*n, where row 0 passes: rows farther than that (+1) are empty
       const double rc2 = a.rho2 - s * s;
       if (rc2 < 0.0) continue;
-      const double S[3] = {s * nrm[0], s * nrm[1], s * nrm[2]};
+      const double S[3] = {uniform_f64(s * nrm[0]), uniform_f64(s * nrm[1]),
+                           uniform_f64(s * nrm[2])};
       // rows of this section: the section polygon's vertices lie on the box edges; the row
       // of a point p of the plane is a.p / |a|^2 (a is orthogonal to n and b), +-2 rows margin
       int r_lo = INT_MAX, r_hi = INT_MIN;
@@ -460,7 +469,10 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
         const double q = ((S[c] + av[c] * (double)SR0) + bv[c] * (double)C0 + 1.0) * h - b0c;
         const double fl = floor(q);
         const double fr = fmin(rint((q - fl) * 4294967296.0), 4294967295.0);
-        Qb[c] = (long long)fl * 4294967296ll + (long long)fr;
+        const long long qb = (long long)fl * 4294967296ll + (long long)fr;
+        Qb[c] = (long long)(((unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane(
+                                 (int)(qb >> 32)) << 32) |
+                            (unsigned)__builtin_amdgcn_readfirstlane((int)qb));
       }
       // One sample: Q the lane's fixed-point coordinates, (f1, f2) its row and column (for
       // the exact decision), m its step and mlast the last step inside the row (face
