@@ -474,18 +474,20 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
                                  (int)(qb >> 32)) << 32) |
                             (unsigned)__builtin_amdgcn_readfirstlane((int)qb));
       }
-      // One sample: Q the lane's fixed-point coordinates, (f1, f2) its row and column (for
-      // the exact decision), m its step and mlast the last step inside the row (face
-      // bricks).  Forward: adds the cell's value to bs; transpose: adds the section's
-      // quantized value to the cell.
-      auto sample = [&](const long long (&Q)[3], double f1, double f2, int m, int mlast,
+      // One sample: Q the lane's fixed-point coordinates, f1 its row and c0 + step*m its
+      // column (for the exact decision), mlast the last step inside the row (face bricks).
+      // Forward: adds the cell's value to bs; transpose: adds the section's quantized value
+      // to the cell.
+      auto sample = [&](const long long (&Q)[3], double f1, int c0, int step, int m, int mlast,
                         float& bs) {
         int lx = (int)(Q[0] >> 32), ly = (int)(Q[1] >> 32), lz = (int)(Q[2] >> 32);
         const unsigned u = min(min((unsigned)Q[0] + 1024u, (unsigned)Q[1] + 1024u),
                                (unsigned)Q[2] + 1024u);
         if (u < 2048u) {
           // within 2^-22 of a cell boundary: the reference's rounding, in binary64 (the
-          // per-column products b*i2 rounded as the reference rounds delta*v2*i2)
+          // per-column products b*i2 rounded as the reference rounds delta*v2*i2); the
+          // step index through readfirstlane keeps the column out of the common path
+          const double f2 = (double)(c0 + step * __builtin_amdgcn_readfirstlane(m));
           const double Rx = S[0] + av[0] * f1, Ry = S[1] + av[1] * f1, Rz = S[2] + av[2] * f1;
           const double x = Rx + bv[0] * f2, y = Ry + bv[1] * f2, z = Rz + bv[2] * f2;
           double qx, qy, qz;
@@ -565,7 +567,7 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
               Q[c] = Qb[c] + (long long)t * Afx[c] + (long long)(c0 - C0) * Dfx[c];
             const int mlast = (rhi - c0) >> 4;  // floor: negative when the row ends before
             for (int m = 0; m < steps2; ++m) {
-              sample(Q, f1, (double)(c0 + 16 * m), m, mlast, bs);
+              sample(Q, f1, c0, 16, m, mlast, bs);
 #pragma unroll
               for (int c = 0; c < 3; ++c) Q[c] += 2 * D8[c];
             }
@@ -578,7 +580,7 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
             Q[c] = Qb[c] + (long long)t * Afx[c] + (long long)(c0 - C0) * Dfx[c];
           const int mlast = (rhi - c0) >> 3;
           for (int m = 0; m < steps; ++m) {
-            sample(Q, f1, (double)(c0 + 8 * m), m, mlast, bs);
+            sample(Q, f1, c0, 8, m, mlast, bs);
 #pragma unroll
             for (int c = 0; c < 3; ++c) Q[c] += D8[c];
           }
