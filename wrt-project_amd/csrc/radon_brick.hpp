@@ -459,9 +459,11 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
       // (x rounded as radon_grid/src/integral.c:47-51 rounds it, then (x + 1) / (2/n)) the
       // error is at most (1 + t + col - C0) 2^-32 (t < 128 rows; col - C0 < 160: a brick's
       // section spans at most 64 sqrt(3) + 6 columns, plus a step's overshoot) plus the
-      // roundings of q and of Qb (below 1e-10 for n < 16384): under 2^-23.  So wherever the fraction of Q lies at least 2^-22 from 0 and 1, the
-      // integer part of Q is the reference's truncated index (less b0); the few samples
-      // nearer a cell boundary take the exact binary64 decision instead.
+      // roundings of q and of Qb (below 1e-10 for n < 16384): under 2^-23.  Q carries a bias
+      // of 2^-22 (1024 units), so wherever its fraction is at least 2^-21 the reference's
+      // quotient lies inside (k + 2^-23, k + 1 - 2^-23) for the integer part k of Q, which
+      // is then the reference's truncated index (less b0); the few samples nearer a cell
+      // boundary take the exact binary64 decision instead.
       long long Qb[3];
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
@@ -469,7 +471,7 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
         const double q = ((S[c] + av[c] * (double)SR0) + bv[c] * (double)C0 + 1.0) * h - b0c;
         const double fl = floor(q);
         const double fr = fmin(rint((q - fl) * 4294967296.0), 4294967295.0);
-        const long long qb = (long long)fl * 4294967296ll + (long long)fr;
+        const long long qb = (long long)fl * 4294967296ll + (long long)fr + 1024ll;
         Qb[c] = (long long)(((unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane(
                                  (int)(qb >> 32)) << 32) |
                             (unsigned)__builtin_amdgcn_readfirstlane((int)qb));
@@ -481,12 +483,12 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
       auto sample = [&](const long long (&Q)[3], double f1, int c0, int step, int m, int mlast,
                         float& bs) {
         int lx = (int)(Q[0] >> 32), ly = (int)(Q[1] >> 32), lz = (int)(Q[2] >> 32);
-        const unsigned u = min(min((unsigned)Q[0] + 1024u, (unsigned)Q[1] + 1024u),
-                               (unsigned)Q[2] + 1024u);
+        const unsigned u = min(min((unsigned)Q[0], (unsigned)Q[1]), (unsigned)Q[2]);
         if (u < 2048u) {
-          // within 2^-22 of a cell boundary: the reference's rounding, in binary64 (the
-          // per-column products b*i2 rounded as the reference rounds delta*v2*i2); the
-          // step index through readfirstlane keeps the column out of the common path
+          // near a cell boundary (fraction of Q below 2^-21): the reference's rounding, in
+          // binary64 (the per-column products b*i2 rounded as the reference rounds
+          // delta*v2*i2); the step index through readfirstlane keeps the column out of the
+          // common path
           const double f2 = (double)(c0 + step * __builtin_amdgcn_readfirstlane(m));
           const double Rx = S[0] + av[0] * f1, Ry = S[1] + av[1] * f1, Rz = S[2] + av[2] * f1;
           const double x = Rx + bv[0] * f2, y = Ry + bv[1] * f2, z = Rz + bv[2] * f2;
