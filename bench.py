@@ -31,7 +31,9 @@ x 257 shifts: FP32 against FP64, the FP32 transpose with its dot-product test, a
 ray -> Radon reduction chain); "analytic" is radon_analytic on its own example configuration
 (256 phi x 128 theta x 129 shifts, ngrid 129, the exponential-bump example function) beside
 the reference's compiled plane_integral on the host cores; "laplace" is the spectral
-Laplacian of a 256^3 volume; "fourier" is the Chang-type Fourier-slice inversion (64 phi x
+Laplacian of a 256^3 volume; "adjoint" is the backprojection at the reference's default
+configuration (129 phi x 128 theta x 256 s -> 128^3) in FP32 and FP64, beside the oracle's C
+restatement on the host cores; "fourier" is the Chang-type Fourier-slice inversion (64 phi x
 32 theta x 129 shifts -> 64^3, FP64) with its rate against the FP64 peak; "programs" runs the radon_grid drop-in program end to end at
 256^3 with text files and with the binary format.
 
@@ -343,6 +345,44 @@ def laplace_leg(ctx, wrt, n=256):
             "alg_bytes": alg, "achieved_GBps": alg / t / 1e9, "frac_of_hbm": alg / t / HBM_PEAK}
 
 
+def adjoint_leg(ctx, wrt, seconds=5.0):
+    """The backprojection radon3d_adjoint at the reference's default configuration (129 phi x
+    128 GL theta x 256 shifts -> 128^3 voxel centres; radon_inversion/backprojection/binary/
+    config.txt) on the Radon data of a phantom: the FP32 form against the plain FP64 form
+    (time, relative L2), in (voxel, direction) evaluations per second, and the oracle's C
+    restatement (splines solved once per row, -O2, OpenMP) timed on the host cores on a
+    smaller voxel grid of the same directions and shifts."""
+    import orc
+
+    n, nphi, ntheta, nshift = 128, 129, 128, 256
+    phi, th, w, s = ctx.radon_grid_init(nphi, ntheta, nshift)
+    radon, _ = ctx.radon_grid(wrt.phantom(n, True, 10, 5, 0.0), phi, th, s, prec="f32")
+    evals = float(n) ** 3 * nphi * ntheta
+    ctx.radon_adjoint(radon, phi, th, w, s, n, prec="f32")
+    runs = [ctx.radon_adjoint(radon, phi, th, w, s, n, prec="f32") for _ in range(3)]
+    t32 = float(np.median([r[1]["kernel_seconds"] for r in runs]))
+    got = runs[-1][0]
+    ref, st64 = ctx.radon_adjoint(radon, phi, th, w, s, n, prec="f64")
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    m = 8
+    while True:
+        t0 = time.perf_counter()
+        orc.radon_adjoint(radon, phi, th, w, s, m)
+        dt = time.perf_counter() - t0
+        if dt >= 0.5 * seconds or m >= 128:
+            break
+        m = min(128, max(m + 4, int(m * min(2.0, (seconds / max(dt, 1e-3)) ** (1 / 3)))))
+    cpu = float(m) ** 3 * nphi * ntheta / dt
+    return {"workload": f"radon3d_adjoint, {nphi} phi x {ntheta} theta x {nshift} s -> {n}^3",
+            "f32_s": t32, "evals_per_s": evals / t32, "f64_s": st64["kernel_seconds"],
+            "f64_evals_per_s": evals / st64["kernel_seconds"],
+            "speedup_vs_plain": st64["kernel_seconds"] / t32, "rel_l2_vs_f64": rel_l2(got, ref),
+            "cpu_port": {"value": cpu, "unit": "(voxel, direction) evaluations/s", "cores": threads,
+                         "kind": "port", "sample": f"{m}^3 voxel centres, same directions and "
+                         f"shifts, {dt:.1f} s; the oracle's C restatement (-O2, OpenMP)"},
+            "speedup_vs_cpu_port": evals / t32 / cpu}
+
+
 def fourier_leg(ctx, wrt, nphi=64, ntheta=32, nshift=129, ngrid=64):
     """the Fourier-slice inversion (FP64) of the plane integrals of an off-centre Gaussian:
     time of the two kernels, the reference's adjoint-sum terms (voxel x direction x frequency)
@@ -474,6 +514,7 @@ def main():
     ap.add_argument("--no-analytic", action="store_true")
     ap.add_argument("--no-laplace", action="store_true")
     ap.add_argument("--no-fourier", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-programs", action="store_true")
     ap.add_argument("--batch-volumes", type=int, default=48)
     ap.add_argument("--traffic-probe", action="store_true")
@@ -555,6 +596,11 @@ def main():
         log("bench: laplace")
         laplace = laplace_leg(ctx, wrt)
         log(f"bench: laplace {laplace['seconds'] * 1e3:.2f} ms")
+    adjoint = None
+    if not args.no_adjoint:
+        log("bench: adjoint")
+        adjoint = adjoint_leg(ctx, wrt)
+        log(f"bench: adjoint {adjoint['f32_s'] * 1e3:.1f} ms")
     fourier = None
     if not args.no_fourier:
         log("bench: fourier")
@@ -634,6 +680,7 @@ def main():
         "rays": rays,
         "analytic": analytic,
         "laplace": laplace,
+        "adjoint": adjoint,
         "fourier": fourier,
         "programs": programs,
     }
