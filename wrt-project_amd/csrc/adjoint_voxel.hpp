@@ -4,13 +4,15 @@
 // radon_inversion/backprojection/C/src/integral.c:6-24 at the voxel centre
 // (-1 + 1/n + i*2/n, radon3d_ajoint.c:248-250): the sum over directions of
 // (2pi/nphi) w_theta S(x . n), S the natural cubic spline of the direction's row over the
-// shift grid and zero for |s| > 1.  Each interval's cubic (y, b, c, d) is an FP32 float4
-// rounded from the FP64 spline; s is formed in FP32.  An interval decision that differs
-// from the FP64 one near a knot changes the value only at the rounding level (the spline is
-// C2 across knots); the cut |s| <= 1 is not continuous (rows need not vanish at s = +-1), so
-// where the FP32 s lies within 1e-5 of +-1 the decision is taken again in FP64 with the
-// reference's expression ((x sin t) cos p + (y sin t) sin p) + z cos t.  The sum over
-// directions is Kahan-compensated in FP32.
+// shift grid and zero for |s| > 1.  Each interval's cubic is an FP32 float4 rounded from the
+// FP64 spline, in the interval's unit variable tt = (s - x_i)/h; s is formed in FP32, and
+// u = (s - s0)/ds with one fma gives the interval i (truncated, clamped) and tt = u - i.  An
+// interval decision that differs from the FP64 one near a knot changes the value only at
+// the rounding level (the spline is C2 across knots); the cut |s| <= 1 is not continuous
+// (rows need not vanish at s = +-1), so where the FP32 s lies within 1e-5 of +-1 the
+// decision is taken again in FP64 with the reference's expression
+// ((x sin t) cos p + (y sin t) sin p) + z cos t.  The sum over directions adds four
+// consecutive directions in FP32, then Kahan-compensates the sum of those groups.
 #ifndef WRT_ADJOINT_VOXEL_HPP
 #define WRT_ADJOINT_VOXEL_HPP
 #include "wrt_common.hpp"
@@ -43,6 +45,7 @@ __device__ __forceinline__ void adjoint_voxels_f32(const float4* __restrict__ di
     comp[v] = 0.f;
   }
   const int last = ns - 2;
+  const float u0 = -s0 * inv_ds;  // u = s * inv_ds + u0 = (s - s0) / ds
   int d = 0;
   for (; d + 4 <= ndir; d += 4) {
     float4 D[4];
@@ -63,9 +66,10 @@ __device__ __forceinline__ void adjoint_voxels_f32(const float4* __restrict__ di
           const double sd = xd[v] * T.x * T.y + yd[v] * T.x * T.z + zd[v] * T.w;
           in = !(fabs(sd) > 1.0);
         }
-        int i = (int)((s - s0) * inv_ds);
+        const float uu = fmaf(s, inv_ds, u0);
+        int i = (int)uu;
         i = i < 0 ? 0 : (i > last ? last : i);  // a valid interval even where s is cut
-        sv[v][u] = s;
+        sv[v][u] = uu - (float)i;  // tt in [0, 1] where the sample is kept
         iv[v][u] = i;
         inv[v][u] = in;
       }
@@ -74,19 +78,20 @@ __device__ __forceinline__ void adjoint_voxels_f32(const float4* __restrict__ di
 #pragma unroll
       for (int u = 0; u < 4; ++u) Cv[v][u] = cub[(long long)(d + u) * (ns - 1) + iv[v][u]];
 #pragma unroll
-    for (int v = 0; v < V; ++v)
+    for (int v = 0; v < V; ++v) {
+      float g = 0.f;  // the four directions' terms, in direction order
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        if (inv[v][u]) {
-          const float t = sv[v][u] - fmaf((float)iv[v][u], ds, s0);
-          const float4 C = Cv[v][u];
-          const float val = fmaf(fmaf(fmaf(C.w, t, C.z), t, C.y), t, C.x);
-          const float yk = fmaf(D[u].w, val, -comp[v]);
-          const float tk = acc[v] + yk;
-          comp[v] = (tk - acc[v]) - yk;
-          acc[v] = tk;
-        }
+        const float t = sv[v][u];
+        const float4 C = Cv[v][u];
+        const float val = fmaf(fmaf(fmaf(C.w, t, C.z), t, C.y), t, C.x);
+        g = fmaf(D[u].w, inv[v][u] ? val : 0.f, g);
       }
+      const float yk = g - comp[v];
+      const float tk = acc[v] + yk;
+      comp[v] = (tk - acc[v]) - yk;
+      acc[v] = tk;
+    }
   }
   for (; d < ndir; ++d) {
     const float4 D = dirs[d];
@@ -100,9 +105,10 @@ __device__ __forceinline__ void adjoint_voxels_f32(const float4* __restrict__ di
         in = !(fabs(sd) > 1.0);
       }
       if (in) {
-        int i = (int)((s - s0) * inv_ds);
+        const float uu = fmaf(s, inv_ds, u0);
+        int i = (int)uu;
         i = i < 0 ? 0 : (i > last ? last : i);
-        const float t = s - fmaf((float)i, ds, s0);
+        const float t = uu - (float)i;
         const float4 C = cub[(long long)d * (ns - 1) + i];
         const float val = fmaf(fmaf(fmaf(C.w, t, C.z), t, C.y), t, C.x);
         const float yk = fmaf(D.w, val, -comp[v]);

@@ -4,7 +4,8 @@
 #include "adjoint_voxel.hpp"
 #include "wrt_dev.h"
 
-// (y, c) packed FP64 rows [row][ns][2] -> per-interval FP32 cubics [row][ns-1] (y, b, c, d)
+// (y, c) packed FP64 rows [row][ns][2] -> per-interval FP32 cubics [row][ns-1] in the
+// interval's unit variable tt = (s - x_i) / h: y + (b h) tt + (c h^2) tt^2 + (d h^3) tt^3
 __global__ __launch_bounds__(256) void k_adjoint_cubics(const double* __restrict__ x, int ns,
                                                         const double* __restrict__ tab,
                                                         float4* __restrict__ cub, long long rows) {
@@ -18,7 +19,7 @@ __global__ __launch_bounds__(256) void k_adjoint_cubics(const double* __restrict
     const double h = x[i + 1] - x[i];
     const double b = (yh - yl) / h - h * (ch + 2.0 * cl) / 3.0;
     const double d = (ch - cl) / (3.0 * h);
-    cub[k] = make_float4((float)yl, (float)b, (float)cl, (float)d);
+    cub[k] = make_float4((float)yl, (float)(b * h), (float)(cl * h * h), (float)(d * h * h * h));
   }
 }
 
