@@ -47,6 +47,55 @@ __device__ __forceinline__ void ray_axis(double x, int n, double dg, double inv_
 
 __device__ __forceinline__ float lerpf(float t, float a, float b) { return fmaf(t, b - a, a); }
 
+// Index and fraction of both axes in fixed point.  Along a ray the reference's quotients
+// q = (x + 1)/dg are linear in the point index i, so Q = q + 2^-22 is kept with 32 fraction
+// bits and stepped by one 64-bit add per point (D = round(sx/dg * 2^32)).  Against the
+// reference's q (x rounded as ray_grid/src/integral.c:41-43 rounds it) Q errs by at most
+// (4n + 1) 2^-33 plus roundings below 1e-12: under 2^-22 for n <= RAY_FIX_N.  Wherever the
+// fraction of Q is at least 2^-21, q then lies strictly between the integer part k of Q and
+// k + 1: the index is k, and the fraction, frac(Q) - 2^-22, is the reference's
+// fmod(x + 1, dg)/dg to within 2.4e-7 (an FP32 lerp weight either way).  Points nearer an
+// integer, among them every point where the reference's rounding ties, its fmod quirk and
+// x = 1, take the exact binary64 path (ray_axis).
+#define RAY_FIX_N 500
+
+__device__ __forceinline__ long long ray_fix(double q) {  // 0 <= q < 2^31
+  const double fl = floor(q);
+  const double fr = fmin(rint((q - fl) * 4294967296.0), 4294967295.0);
+  return (long long)(((unsigned long long)(unsigned)(int)fl << 32) | (unsigned)fr) + 1024ll;
+}
+
+template <bool POW2>
+__device__ __forceinline__ void ray_point(bool fixed, long long Qx, long long Qy, double px,
+                                          double py, double sx, double sy, int i, int n,
+                                          double dg, double inv_dg, double h, int& ix, int& iy,
+                                          float& fx, float& fy) {
+  const unsigned lx = (unsigned)Qx, ly = (unsigned)Qy;
+  if (fixed && min(lx, ly) >= 2048u) {
+    ix = (int)(Qx >> 32);
+    iy = (int)(Qy >> 32);
+    fx = fmaf((float)lx, 2.3283064365386963e-10f, -2.384185791015625e-7f);
+    fy = fmaf((float)ly, 2.3283064365386963e-10f, -2.384185791015625e-7f);
+  } else {
+    const double fi = (double)i;
+    const double x = px + sx * fi, y = py + sy * fi;
+    ray_axis<POW2>(x, n, dg, inv_dg, h, ix, fx);
+    ray_axis<POW2>(y, n, dg, inv_dg, h, iy, fy);
+  }
+}
+
+// the fixed-point start (point lo) and step of both axes
+__device__ __forceinline__ void ray_fix_start(double px, double py, double sx, double sy, int lo,
+                                              int n, long long& Qx, long long& Qy, long long& Dx,
+                                              long long& Dy) {
+  const double hg = 0.5 * (n - 1);  // 1/dg
+  const double flo = (double)lo;
+  Qx = ray_fix((px + sx * flo + 1.0) * hg);
+  Qy = ray_fix((py + sy * flo + 1.0) * hg);
+  Dx = (long long)rint(sx * hg * 4294967296.0);
+  Dy = (long long)rint(sy * hg * 4294967296.0);
+}
+
 // The exact range [lo, hi] of i in [-2n, 2n] whose point x = px + sx*i, y = py + sy*i (each
 // rounded as the reference rounds it) passes |x| <= 1 and |y| <= 1.  x and y are monotone
 // in i, so the range is an interval; the continuous bounds, widened by a tolerance far
@@ -109,14 +158,14 @@ __global__ __launch_bounds__(RAY_THREADS) void k_ray_f32(const float* __restrict
   const double sx = delta * dx, sy = delta * dy;
   int lo, hi;
   ray_range(px, py, sx, sy, n, lo, hi);
+  const bool fixed = n <= RAY_FIX_N;
+  long long Qx = 0, Qy = 0, Dx = 0, Dy = 0;
+  if (fixed && lo <= hi) ray_fix_start(px, py, sx, sy, lo, n, Qx, Qy, Dx, Dy);
   float acc = 0.f, comp = 0.f;
-  for (int i = lo; i <= hi; ++i) {
-    const double fi = (double)i;
-    const double x = px + sx * fi, y = py + sy * fi;
+  for (int i = lo; i <= hi; ++i, Qx += Dx, Qy += Dy) {
     int ix, iy;
     float fx, fy;
-    ray_axis<POW2>(x, n, dg, inv_dg, h, ix, fx);
-    ray_axis<POW2>(y, n, dg, inv_dg, h, iy, fy);
+    ray_point<POW2>(fixed, Qx, Qy, px, py, sx, sy, i, n, dg, inv_dg, h, ix, iy, fx, fy);
     float p00, p10, p01, p11;
     const int o = ix * n + iy;
     if (LDS) {
@@ -223,13 +272,13 @@ __global__ __launch_bounds__(RAY_T_THREADS) void k_ray_T_f32(const float* __rest
     const double sx = delta * dx, sy = delta * dy;
     int lo, hi;
     ray_range(px, py, sx, sy, n, lo, hi);
-    for (int i = lo; i <= hi; ++i) {
-      const double fi = (double)i;
-      const double x = px + sx * fi, y = py + sy * fi;
+    const bool fixed = n <= RAY_FIX_N;
+    long long Qx = 0, Qy = 0, Dx = 0, Dy = 0;
+    if (fixed && lo <= hi) ray_fix_start(px, py, sx, sy, lo, n, Qx, Qy, Dx, Dy);
+    for (int i = lo; i <= hi; ++i, Qx += Dx, Qy += Dy) {
       int ix, iy;
       float fx, fy;
-      ray_axis<POW2>(x, n, dg, inv_dg, h, ix, fx);
-      ray_axis<POW2>(y, n, dg, inv_dg, h, iy, fy);
+      ray_point<POW2>(fixed, Qx, Qy, px, py, sx, sy, i, n, dg, inv_dg, h, ix, iy, fx, fy);
       // the forward's value: lerp(fy, lerp(fx, p00, p10), lerp(fx, p01, p11))
       const float gy0 = gv * (1.f - fy), gy1 = gv * fy;
       const float w00 = gy0 * (1.f - fx), w10 = gy0 * fx, w01 = gy1 * (1.f - fx), w11 = gy1 * fx;
