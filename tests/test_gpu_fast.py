@@ -1,5 +1,7 @@
-"""GPU tests of the fast plane-integral form (FP32 values / BF16 bricks, FP32 Kahan sums,
-exact FP64 index decisions) against the plain FP64 form and the oracle.
+"""GPU tests of the fast plane-integral form (FP32 values / BF16 bricks, FP64 sums of FP32
+band partials, the reference's index decisions: fixed point with an exact binary64 path near
+cell boundaries) against the plain FP64 form and the oracle; integer volumes pin every index
+decision (test_fast_f32_index_decisions_exact).
 
 Tolerances: FP32 relative L2 <= 1e-5 and max |error| <= 1e-5 * max|R| (the target of the
 form); BF16 bricks relative L2 <= 2e-3 (BF16 keeps 8 significand bits per voxel)."""
