@@ -24,7 +24,8 @@ all in one persistent graph-captured launch (direction groups autotuned on the f
 the replay's time is compared with the same 96 jobs launched one by one, and with their
 plain FP64 forms.
 
-The "transpose" object is the FP32 exact transpose of the plane integrals on the same grid
+The "bf16" object is the same pass from BF16 bricks (time, rate, its algorithmic bytes
+against the HBM peak, relL2 against the plain FP64 subset).  The "transpose" object is the FP32 exact transpose of the plane integrals on the same grid
 (rate, error against the plain FP64 transpose on a phi subset, and the dot-product test of
 the FP32 pair at full size); "rays" is the line-integral workload at 128^3 (128 z x 180 phi
 x 257 shifts: FP32 against FP64, the FP32 transpose with its dot-product test, and the FP64
@@ -515,6 +516,7 @@ def main():
     ap.add_argument("--no-laplace", action="store_true")
     ap.add_argument("--no-fourier", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-bf16", action="store_true")
     ap.add_argument("--no-programs", action="store_true")
     ap.add_argument("--batch-volumes", type=int, default=48)
     ap.add_argument("--traffic-probe", action="store_true")
@@ -567,6 +569,21 @@ def main():
     log(f"bench: plain f64 subset {pst['kernel_seconds']:.3f} s, relL2 {err_l2:.3e}")
 
     incube = ctx.radon_count_incube(n, phi, th, s)
+    bf16 = None
+    if args.prec == "f32" and not args.no_bf16:
+        # the same pass from BF16 bricks (half the brick bytes; FP32 in LDS)
+        log("bench: bf16 bricks")
+        bj = ctx.job("radon", "bf16", n=n, phi=phi, theta=th, shift=s)
+        bj.load(vol)
+        bj.run()
+        bst = bj.run()
+        bout = bj.fetch().reshape(len(s), len(phi), len(th))
+        bj.close()
+        balg = incube * 2 + nplanes * 4 + n ** 3 * 2
+        bf16 = {"seconds": bst["kernel_seconds"], "samples_per_s": logical / bst["kernel_seconds"],
+                "alg_bytes": balg, "frac_of_hbm": balg / bst["kernel_seconds"] / HBM_PEAK,
+                "rel_l2_vs_f64": rel_l2(bout[:, ::args.plain_phi_stride, :], plain)}
+        log(f"bench: bf16 {bst['kernel_seconds']:.3f} s, relL2 {bf16['rel_l2_vs_f64']:.2e}")
     eb = 2 if args.prec == "bf16" else 4
     alg_bytes = incube * eb + nplanes * 4 + n ** 3 * eb
     kavg = float(np.mean(ks))
@@ -680,6 +697,7 @@ def main():
         "rays": rays,
         "analytic": analytic,
         "laplace": laplace,
+        "bf16": bf16,
         "adjoint": adjoint,
         "fourier": fourier,
         "programs": programs,
