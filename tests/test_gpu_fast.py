@@ -194,7 +194,7 @@ def test_fast_ray_f32_rays_on_cube_faces(ctx, n):
         assert orc.rel_l2(got[:, js, :], ref[:, js, :]) <= 1e-5
 
 
-@pytest.mark.parametrize("n", [32, 48, 64])
+@pytest.mark.parametrize("n", [32, 33, 40, 48, 64])
 def test_fast_f32_index_decisions_exact(ctx, n):
     # Small integer values make every sum exact (FP32 band partials, FP64 totals), so the
     # fast form must give the plain form's integer sum on every plane: a single sample put
