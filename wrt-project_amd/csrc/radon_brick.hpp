@@ -480,15 +480,35 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
       // column (for the exact decision), mlast the last step inside the row (face bricks).
       // Forward: adds the cell's value to bs; transpose: adds the section's quantized value
       // to the cell.
+      // The cell of brick offsets (lx, ly, lz): all three in [0, 32) (z: [0, 16) in transpose
+      // mode, tested as 2*lz < 32; a negative offset sets high bits of the OR) and, on a face
+      // brick, the step within its row.  ok and the cell's byte offset in the LDS brick
+      // (forward: slot 0 holds the zero that rejected samples read, the brick starts at
+      // slot 1, and a rejected sample gets offset 0).
+      auto cell = [&](int lx, int ly, int lz, int m, int mlast, bool& ok, unsigned& off) {
+        ok = ((unsigned)lx | (unsigned)ly | ((unsigned)lz << (T ? 1 : 0))) < (unsigned)FB;
+        if (BND) ok = ok && m <= mlast;
+        if constexpr (T) {
+          off = mad_u24((unsigned)lx, 8u * FT_PX, mad_u24((unsigned)ly, 8u * FT_PY, (unsigned)lz * 8u));
+        } else {
+          off = mad_u24((unsigned)lx, 4u * FR_PX, mad_u24((unsigned)ly, 4u * FR_PY, (unsigned)lz * 4u + 4u));
+          off = ok ? off : 0u;
+        }
+      };
       auto sample = [&](const long long (&Q)[3], double f1, int c0, int step, int m, int mlast,
                         float& bs) {
-        int lx = (int)(Q[0] >> 32), ly = (int)(Q[1] >> 32), lz = (int)(Q[2] >> 32);
         const unsigned u = min(min((unsigned)Q[0], (unsigned)Q[1]), (unsigned)Q[2]);
+        // the fixed point's cell (away from a boundary the fixed point floors where the
+        // reference truncates: they differ only for -1 < q < 0, x < -1, which an interior
+        // brick rejects either way and a face brick's row never holds) ...
+        bool ok;
+        unsigned off;
+        cell((int)(Q[0] >> 32), (int)(Q[1] >> 32), (int)(Q[2] >> 32), m, mlast, ok, off);
         if (u < 2048u) {
-          // near a cell boundary (fraction of Q below 2^-21): the reference's rounding, in
-          // binary64 (the per-column products b*i2 rounded as the reference rounds
-          // delta*v2*i2); the step index through readfirstlane keeps the column out of the
-          // common path
+          // ... replaced near a cell boundary (fraction of Q below 2^-21) by the reference's
+          // rounding, in binary64 (the per-column products b*i2 rounded as the reference
+          // rounds delta*v2*i2); the step index through readfirstlane keeps the column out
+          // of the common path
           const double f2 = (double)(c0 + step * __builtin_amdgcn_readfirstlane(m));
           const double Rx = S[0] + av[0] * f1, Ry = S[1] + av[1] * f1, Rz = S[2] + av[2] * f1;
           const double x = Rx + bv[0] * f2, y = Ry + bv[1] * f2, z = Rz + bv[2] * f2;
@@ -504,37 +524,23 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
           }
           // truncation as in zero_interp; for an interior brick a negative quotient (x < -1)
           // truncates to 0 <= b0 - 32 and a quotient above n lands past the brick, so the
-          // brick test below also is the cube test.  Face bricks: the row ends hold the
-          // cube test, and x == 1 (index n) maps to n - 1.
+          // brick test also is the cube test.  Face bricks: the row ends hold the cube test,
+          // and x == 1 (index n) maps to n - 1.
           int ix = (int)qx, iy = (int)qy, iz = (int)qz;
           if (BND) {
             ix = min(ix, cxh);
             iy = min(iy, cyh);
             iz = min(iz, czh);
           }
-          lx = ix - b0x;
-          ly = iy - b0y;
-          lz = iz - b0z;
+          cell(ix - b0x, iy - b0y, iz - b0z, m, mlast, ok, off);
         }
-        // all three offsets in [0, 32) (z: [0, 16) in transpose mode, tested as 2*lz < 32):
-        // a negative offset sets high bits of the OR.  (Away from a boundary the fixed point
-        // floors where the reference truncates: they differ only for -1 < q < 0, x < -1,
-        // which an interior brick rejects either way and a face brick's row never holds.)
-        bool ok = ((unsigned)lx | (unsigned)ly | ((unsigned)lz << (T ? 1 : 0))) < (unsigned)FB;
-        if (BND) ok = ok && m <= mlast;
         if constexpr (T) {
-          const unsigned off = mad_u24((unsigned)lx, 8u * FT_PX,
-                                       mad_u24((unsigned)ly, 8u * FT_PY, (unsigned)lz * 8u));
           if (ok)
             __hip_atomic_fetch_add(
                 (unsigned long long*)__builtin_assume_aligned((char*)L.acc + off, 8), gq,
                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         } else {
-          // byte offset of the cell: slot 0 holds the zero that rejected samples read (an
-          // inline constant in the select), the brick starts at slot 1
-          const unsigned off = mad_u24((unsigned)lx, 4u * FR_PX,
-                                       mad_u24((unsigned)ly, 4u * FR_PY, (unsigned)lz * 4u + 4u));
-          bs += *(const float*)((const char*)L.sb + (ok ? off : 0u));
+          bs += *(const float*)((const char*)L.sb + off);
         }
       };
       // Rows past R1 and columns past the lattice end need no test: a plane's cube section
