@@ -220,8 +220,9 @@ def transpose_leg(ctx, wrt, vol, fast, phi, th, s, stride):
 def rays_leg(ctx, wrt):
     """The line-integral workload at 128^3: 128 z-slices x 180 phi in [0, pi) x 257 shifts,
     FP32 against the plain FP64 form; the FP32 exact transpose and the dot-product test of
-    the FP32 pair on seeded Gaussian f and g; and the FP64 reduction chain (rays on 128
-    slices x 64 phi in [0, 2 pi) x 129 shifts -> Radon 64 phi x 32 theta x 129 shifts)."""
+    the FP32 pair on seeded Gaussian f and g; the FP64 reduction chain (rays on 128
+    slices x 64 phi in [0, 2 pi) x 129 shifts -> Radon 64 phi x 32 theta x 129 shifts); and
+    the same Radon grid reduced from the FP64 throughput rays themselves."""
     n = 128
     z = wrt.zslice_grid(128)
     s, phi2 = wrt.ray_grid_init(257, 360)
@@ -261,7 +262,14 @@ def rays_leg(ctx, wrt):
     dj = ctx.job("reduction", "f64", phi=rphi, theta=th, shift=rs, z=rz, nz=len(rz))
     dj.load(rays)
     dst = dj.run()
+    red = dj.fetch()
     dj.close()
+    # the same Radon grid straight from the throughput rays (180 angles in [0, pi) x 257
+    # shifts): rows at each azimuth by P(z, s, phi + pi) = P(z, -s, phi) and cubic Lagrange
+    # interpolation in phi, splines over the 257 ray shifts
+    trays = res["f64"][0].reshape(len(z), len(s), len(phi))
+    for _ in range(2):  # the second call is the one reported
+        gred, gst = ctx.radon_reduction_rays(trays, s, True, rs, rphi, th)
     f32, st32 = res["f32"]
     f64, st64 = res["f64"]
     return {"workload": "ray_grid 128^3, 128 z x 180 phi in [0,pi) x 257 shifts",
@@ -274,7 +282,11 @@ def rays_leg(ctx, wrt):
             "dot_rel": abs(lhs - rhs) / (np.linalg.norm(Af) * np.linalg.norm(g)),
             "reduction": {"grid": "rays 128 z x 64 phi x 129 s -> 64 phi x 32 theta x 129 s",
                           "rays_f64_s": rst["kernel_seconds"], "reduction_f64_s": dst["kernel_seconds"],
-                          "tau_steps_per_s": dst["logical_samples"] / dst["kernel_seconds"]}}
+                          "tau_steps_per_s": dst["logical_samples"] / dst["kernel_seconds"]},
+            "reduction_from_throughput_rays": {
+                "grid": "rays 128 z x 180 phi in [0,pi) x 257 s -> 64 phi x 32 theta x 129 s",
+                "seconds": gst["seconds"], "reduction_f64_s": gst["kernel_seconds"],
+                "rel_l2_vs_parity_chain": rel_l2(gred.ravel(), red)}}
 
 
 def analytic_leg(ctx, wrt, seconds=6.0):

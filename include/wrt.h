@@ -114,6 +114,21 @@ int wrt_radon_reduction(wrt_ctx* ctx, const double* rays, int nz, const double* 
                         int nshift, const double* phi, int nphi, const double* theta,
                         int ntheta, double* out, wrt_stats* st);
 
+/* The reduction on independent ray and plane grids (no reference counterpart: the
+ * reference indexes its rays with the output's phi and shift grids,
+ * radon_reduction/slicing.c:50-60,114).  rays[z][s][k] on nz slices
+ * z_l = -1 + l*2/(nz-1), the increasing ray shifts rshift[nrshift] and nrphi >= 4 angles
+ * phi_k = k*pi/nrphi (half_turn = 1; rshift must then be symmetric about 0) or
+ * k*2pi/nrphi (half_turn = 0).  The row at each plane azimuth phi[j] (any real value) comes
+ * from P(z, s, phi + pi) = P(z, -s, phi) and 4-point Lagrange interpolation over the
+ * periodic angle grid; an azimuth on a ray angle takes that row alone, so with
+ * half_turn = 0 and identical grids the result equals wrt_radon_reduction's bit for bit.
+ * Splines over rshift, planes at the offsets shift[nshift]; out[s][phi][theta].  FP64. */
+int wrt_radon_reduction_rays(wrt_ctx* ctx, const double* rays, int nz, const double* rshift,
+                             int nrshift, int nrphi, int half_turn, const double* shift,
+                             int nshift, const double* phi, int nphi, const double* theta,
+                             int ntheta, double* out, wrt_stats* st);
+
 /* Backprojection at the n^3 voxel centres: replaces radon3d_adjoint + radon_interpolate_s
  * (radon_inversion/backprojection/C/src/integral.c:6-24, interp.c:5-28) called from
  * radon3d_ajoint.c:240-254.  radon[s][phi][theta]; out[z][x][y]. */

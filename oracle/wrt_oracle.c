@@ -509,6 +509,138 @@ double orc_radon_reduction_one(const double* rays, int nz, const double* shift, 
 }
 
 /* ------------------------------------------------------------------------------------ */
+/* the reduction on independent ray and plane grids (no reference counterpart)          */
+/* ------------------------------------------------------------------------------------ */
+
+/* The ray row at azimuth phi from rays on nrphi angles k*period/nrphi (period pi or 2 pi):
+ * P(z, s, phi + pi) = P(z, -s, phi) (the same line, sampled by the symmetric parameter set
+ * of ray_grid/src/integral.c:11-50) maps phi into the period, and cubic Lagrange
+ * interpolation over the four nearest angles, each row's spline read at s or -s, gives the
+ * value.  An azimuth within 1e-12 of a node spacing of a ray angle takes that row alone. */
+typedef struct {
+  int k[4], flip[4];
+  double w[4];
+} phi_stencil;
+
+static void phi_stencil_make(double phi, int nrphi, int half_turn, phi_stencil* ps) {
+  const double period = half_turn ? M_PI : 2.0 * M_PI;
+  double a = fmod(phi, 2.0 * M_PI);
+  if (a < 0) a += 2.0 * M_PI;
+  int flip = 0;
+  if (half_turn && a >= M_PI) {
+    a -= M_PI;
+    flip = 1;
+  }
+  const double u = a / (period / nrphi);
+  int k = (int)floor(u);
+  double fr = u - k;
+  if (fr < 1e-12) fr = 0.0;
+  if (fr > 1.0 - 1e-12) {
+    fr = 0.0;
+    k += 1;
+  }
+  const double node[4] = {-1.0, 0.0, 1.0, 2.0};
+  for (int m = 0; m < 4; ++m) {
+    double w = 1.0;
+    for (int i = 0; i < 4; ++i)
+      if (i != m) w *= (fr - node[i]) / (node[m] - node[i]);
+    ps->w[m] = w;
+    /* index k-1+m on the unrolled grid; every period crossed is a half turn when
+     * half_turn, i.e. one more s -> -s */
+    int idx = k - 1 + m, f = flip;
+    const int q = (int)floor((double)idx / nrphi);
+    idx -= q * nrphi;
+    if (half_turn && (q & 1)) f ^= 1;
+    ps->k[m] = idx;
+    ps->flip[m] = f;
+  }
+}
+
+/* ray_interpolate (radon_reduction/slicing.c:5-77) with each z-row formed by the stencil */
+static double ray_interpolate_stencil(const ray_table* t, const double* rshift, double slice_z,
+                                      double slice_shift, const phi_stencil* ps) {
+  if ((fabs(slice_z) > 1.0) || (fabs(slice_shift) > 1.0)) return 0;
+  const int ngrid = t->nz, nshift = t->nshift;
+  const double delta_z = 2.0 / (ngrid - 1);
+  int iz = (int)((slice_z + 1.0) / delta_z);
+  if (iz == (ngrid - 1)) iz -= 1;
+  double za[3], va[3];
+  za[0] = (-1.0) + delta_z * iz;
+  za[1] = (-1.0) + delta_z * (iz + 1);
+  za[2] = (-1.0) + delta_z * (iz + 2);
+  for (int j = 0; j < 3; ++j) {
+    if (j == 2 && iz >= (ngrid - 2)) {
+      va[2] = 0;
+      break;
+    }
+    double v = 0.0;
+    for (int m = 0; m < 4; ++m) {
+      const size_t r = ((size_t)(iz + j) * t->nphi + ps->k[m]) * nshift;
+      const double sv = ps->flip[m] ? -slice_shift : slice_shift;
+      v += ps->w[m] * orc_cspline_eval(rshift, t->rows + r, t->coef + r, nshift, sv);
+    }
+    va[j] = v;
+  }
+  return orc_poly3_eval(za, va, slice_z);
+}
+
+/* radon_reduction's tau loop (radon_reduction/slicing.c:80-119) at the plane (p, ., th) */
+static double reduction_rays_one(const ray_table* t, const double* rshift, double p_shift,
+                                 double p_theta, const phi_stencil* ps) {
+  const int ngrid = t->nz;
+  const double dtau = (2.0 / (ngrid - 1)) / (2.0);
+  const int nsize = 4 * ngrid;
+  double integral = 0, add = 0;
+  for (int i_tau = -(nsize / 2); i_tau < (nsize / 2) + 1; ++i_tau) {
+    const double tau = (i_tau) * dtau;
+    const double slice_z = p_shift * cos(p_theta) + tau * sin(p_theta);
+    const double slice_shift = p_shift * sin(p_theta) - tau * cos(p_theta);
+    if ((fabs(tau * tau + p_shift * p_shift) > 1.0) || (fabs(slice_z) > 1.0) ||
+        (fabs(slice_shift) > 1.0)) {
+      add = 0;
+    } else {
+      add = ray_interpolate_stencil(t, rshift, slice_z, slice_shift, ps);
+    }
+    integral += (add * dtau);
+  }
+  return integral;
+}
+
+/* out[s][phi][theta] at the planes (shift, phi, theta) from rays[z][rs][k] */
+void orc_radon_reduction_rays(const double* rays, int nz, const double* rshift, int nrshift,
+                              int nrphi, int half_turn, const double* shift, int nshift,
+                              const double* phi, int nphi, const double* theta, int ntheta,
+                              double* out) {
+  ray_table t;
+  ray_table_build(&t, rays, nz, rshift, nrshift, nrphi);
+  phi_stencil* ps = (phi_stencil*)malloc(sizeof(phi_stencil) * (size_t)nphi);
+  for (int j = 0; j < nphi; ++j) phi_stencil_make(phi[j], nrphi, half_turn, ps + j);
+  const long long total = (long long)nshift * nphi * ntheta;
+#pragma omp parallel for schedule(dynamic, 16)
+  for (long long k = 0; k < total; ++k) {
+    const int it = (int)(k % ntheta);
+    const int ip = (int)((k / ntheta) % nphi);
+    const int is = (int)(k / ((long long)ntheta * nphi));
+    out[k] = reduction_rays_one(&t, rshift, shift[is], theta[it], ps + ip);
+  }
+  free(ps);
+  ray_table_free(&t);
+}
+
+/* one plane (i_shift, i_phi, i_theta) of orc_radon_reduction_rays */
+double orc_radon_reduction_rays_one(const double* rays, int nz, const double* rshift,
+                                    int nrshift, int nrphi, int half_turn, double p_shift,
+                                    double p_phi, double p_theta) {
+  ray_table t;
+  ray_table_build(&t, rays, nz, rshift, nrshift, nrphi);
+  phi_stencil ps;
+  phi_stencil_make(p_phi, nrphi, half_turn, &ps);
+  const double r = reduction_rays_one(&t, rshift, p_shift, p_theta, &ps);
+  ray_table_free(&t);
+  return r;
+}
+
+/* ------------------------------------------------------------------------------------ */
 /* radon3d_adjoint                                                                      */
 /* ------------------------------------------------------------------------------------ */
 

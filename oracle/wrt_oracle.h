@@ -64,6 +64,16 @@ void   orc_radon_reduction_grid(const double* rays, int nz, const double* shift,
 double orc_radon_reduction_one(const double* rays, int nz, const double* shift, int nshift,
                                const double* phi, int nphi, const double* theta, int ntheta,
                                int i_shift, int i_phi, int i_theta);
+/* the same shift integral on independent grids (no reference counterpart): rays[z][rs][k]
+ * on nrphi angles k*pi/nrphi (half_turn) or k*2pi/nrphi, rows at each plane azimuth by
+ * P(z,s,phi+pi) = P(z,-s,phi) and cubic Lagrange interpolation in phi */
+void   orc_radon_reduction_rays(const double* rays, int nz, const double* rshift, int nrshift,
+                                int nrphi, int half_turn, const double* shift, int nshift,
+                                const double* phi, int nphi, const double* theta, int ntheta,
+                                double* out);
+double orc_radon_reduction_rays_one(const double* rays, int nz, const double* rshift,
+                                    int nrshift, int nrphi, int half_turn, double p_shift,
+                                    double p_phi, double p_theta);
 
 /* ---- radon3d_adjoint (R10, R11) ---- */
 /* radon in file order [nshift][nphi][ntheta]; out[z][x][y] with n^3 voxel centres */

@@ -195,3 +195,22 @@ def gauss2d_on_grid(ngrid, c=GAUSS_C[:2], sigma=GAUSS_SIGMA):
     x = (np.arange(ngrid) - ngrid // 2) * (2.0 / ngrid)
     X, Y = np.meshgrid(x, x, indexing="ij")
     return np.exp(-((X - c[0]) ** 2 + (Y - c[1]) ** 2) / sigma ** 2)
+
+
+def gauss_rays(z, s, phi, c=GAUSS_C, sigma=GAUSS_SIGMA):
+    """rays[z][s][phi] of f(x) = exp(-|x - c|^2 / sigma^2) along the lines
+    (s cos phi - t sin phi, s sin phi + t cos phi, z): sigma sqrt(pi) exp(-d^2 / sigma^2), d the
+    distance from c to the line"""
+    z, s, phi = (np.asarray(a, dtype=np.float64) for a in (z, s, phi))
+    cs = c[0] * np.cos(phi) + c[1] * np.sin(phi)
+    d2 = (z[:, None, None] - c[2]) ** 2 + (s[None, :, None] - cs[None, None, :]) ** 2
+    return sigma * math.sqrt(math.pi) * np.exp(-d2 / sigma ** 2)
+
+
+def gauss_planes(p, phi, theta, c=GAUSS_C, sigma=GAUSS_SIGMA):
+    """R[p][phi][theta] of the same f: pi sigma^2 exp(-(p - c.u)^2 / sigma^2),
+    u = (sin theta cos phi, sin theta sin phi, cos theta)"""
+    p, phi, theta = (np.asarray(a, dtype=np.float64) for a in (p, phi, theta))
+    cu = (c[0] * np.sin(theta)[None, :] * np.cos(phi)[:, None]
+          + c[1] * np.sin(theta)[None, :] * np.sin(phi)[:, None] + c[2] * np.cos(theta)[None, :])
+    return math.pi * sigma ** 2 * np.exp(-(p[:, None, None] - cu[None]) ** 2 / sigma ** 2)

@@ -39,6 +39,8 @@ def L():
         l.orc_poly3_eval.argtypes = [P, P, D]
         l.orc_radon_reduction_one.restype = D
         l.orc_radon_reduction_one.argtypes = [P, I, P, I, P, I, P, I, I, I, I]
+        l.orc_radon_reduction_rays_one.restype = D
+        l.orc_radon_reduction_rays_one.argtypes = [P, I, P, I, I, I, D, D, D]
         l.orc_radon_adjoint_point.restype = D
         l.orc_radon_adjoint_point.argtypes = [P, P, I, P, P, I, P, I, D, D, D]
         l.orc_ellipsoid_plane.restype = D
@@ -170,6 +172,24 @@ def radon_reduction(rays_, shift, phi, theta):
     L().orc_radon_reduction_grid(_p(rays_), rays_.shape[0], _p(shift), len(shift), _p(phi),
                                  len(phi), _p(theta), len(theta), _p(out))
     return out
+
+
+def radon_reduction_rays(rays_, rshift, half_turn, shift, phi, theta):
+    """rays_[z][rs][k] on nrphi = rays_.shape[2] angles k*pi/nrphi (half_turn) or k*2pi/nrphi
+    -> out[s][phi][theta]"""
+    rays_, rshift, shift, phi, theta = map(_f, (rays_, rshift, shift, phi, theta))
+    out = np.empty((len(shift), len(phi), len(theta)))
+    L().orc_radon_reduction_rays(_p(rays_), rays_.shape[0], _p(rshift), len(rshift),
+                                 rays_.shape[2], int(bool(half_turn)), _p(shift), len(shift),
+                                 _p(phi), len(phi), _p(theta), len(theta), _p(out))
+    return out
+
+
+def radon_reduction_rays_one(rays_, rshift, half_turn, p, phi, theta):
+    rays_, rshift = _f(rays_), _f(rshift)
+    return L().orc_radon_reduction_rays_one(_p(rays_), rays_.shape[0], _p(rshift), len(rshift),
+                                            rays_.shape[2], int(bool(half_turn)), float(p),
+                                            float(phi), float(theta))
 
 
 def radon_adjoint(radon, phi, theta, theta_w, shift, n):

@@ -288,18 +288,21 @@ __device__ __forceinline__ double tab_eval(const double* __restrict__ x,
 // ------------------------------------------------------------ radon_reduction (FP64) ---
 // radon_reduction + ray_interpolate (radon_reduction/slicing.c:5-119), one thread per
 // plane, tau-steps in order.  Spline rows (z, phi) over the shift grid come from the
-// packed table; the quadratic in z is the 3-point divided-difference polynomial.
+// packed table; the quadratic in z is the 3-point divided-difference polynomial.  The
+// planes' offsets pshift[np] and the spline nodes shift[ns] are separate arrays; the
+// reference's form passes the same grid for both.
 __global__ __launch_bounds__(256) void k_reduction(const double* __restrict__ tab, int nz,
                                                    const double* __restrict__ shift, int ns,
+                                                   const double* __restrict__ pshift, int np,
                                                    int nphi, const double* __restrict__ ttrig,
                                                    int ntheta, double* __restrict__ out) {
   const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long long total = (long long)ns * nphi * ntheta;
+  const long long total = (long long)np * nphi * ntheta;
   if (k >= total) return;
   const int it = (int)(k % ntheta);
   const int ip = (int)((k / ntheta) % nphi);
   const int is = (int)(k / ((long long)ntheta * nphi));
-  const double p = shift[is];
+  const double p = pshift[is];
   const double ct = ttrig[2 * it], st = ttrig[2 * it + 1];
   const double dtau = (2.0 / (nz - 1)) / (2.0);
   const double dz = 2.0 / (nz - 1);
@@ -338,12 +341,51 @@ __global__ __launch_bounds__(256) void k_reduction(const double* __restrict__ ta
 }
 
 extern "C" int wrtk_reduction(const double* tab, int nz, const double* shift, int nshift,
-                              int nphi, const double* theta_trig, int ntheta, double* out,
+                              const double* pshift, int np, int nphi,
+                              const double* theta_trig, int ntheta, double* out,
                               hipStream_t st) {
-  const long long total = (long long)nshift * nphi * ntheta;
+  const long long total = (long long)np * nphi * ntheta;
   if (total <= 0) return WRT_OK;
   k_reduction<<<dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st>>>(
-      tab, nz, shift, nshift, nphi, theta_trig, ntheta, out);
+      tab, nz, shift, nshift, pshift, np, nphi, theta_trig, ntheta, out);
+  WRT_LAUNCH_CHECK();
+  return WRT_OK;
+}
+
+// Ray rows at the planes' azimuths (the reduction on independent ray and plane grids):
+// out[z][j][s] = sum_m w[j][m] * rays[z][s_m][k[j][m]] with s_m = ns-1-s where the stencil
+// row is taken at the opposite azimuth (P(z, s, phi + pi) = P(z, -s, phi) on a shift grid
+// symmetric about 0), else s.  stencil[j] = {w0..w3, k0..k3, flip0..flip3} as doubles; the
+// four terms are summed in stencil order.
+__global__ __launch_bounds__(256) void k_ray_phi_resample(const double* __restrict__ rays,
+                                                          int nz, int ns, int nrphi,
+                                                          const double* __restrict__ stencil,
+                                                          int nphi, double* __restrict__ out) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long total = (long long)nz * nphi * ns;
+  if (k >= total) return;
+  const int is = (int)(k % ns);
+  const int j = (int)((k / ns) % nphi);
+  const long long iz = k / ((long long)ns * nphi);
+  const double* sj = stencil + 12 * j;
+  const double* zr = rays + iz * ns * (long long)nrphi;
+  double acc = 0.0;
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    const int kr = (int)sj[4 + m];
+    const int sr = sj[8 + m] != 0.0 ? ns - 1 - is : is;
+    acc += sj[m] * zr[(long long)sr * nrphi + kr];
+  }
+  out[k] = acc;
+}
+
+extern "C" int wrtk_ray_phi_resample(const double* rays, int nz, int nshift, int nrphi,
+                                     const double* stencil, int nphi, double* out,
+                                     hipStream_t st) {
+  const long long total = (long long)nz * nphi * nshift;
+  if (total <= 0) return WRT_OK;
+  k_ray_phi_resample<<<dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st>>>(
+      rays, nz, nshift, nrphi, stencil, nphi, out);
   WRT_LAUNCH_CHECK();
   return WRT_OK;
 }

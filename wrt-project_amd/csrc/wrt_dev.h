@@ -40,10 +40,16 @@ int wrtk_spline_rows(const double* x, int ns, const double* fac, const double* y
                      int n_outer, int n_inner, long long outer_stride, long long es,
                      hipStream_t st);
 
-/* tab rows (z, phi): r = iz*nphi + iphi */
-int wrtk_reduction(const double* tab, int nz, const double* shift, int nshift, int nphi,
+/* tab rows (z, phi): r = iz*nphi + iphi, splines over shift[nshift]; planes at the offsets
+ * pshift[np] (the same array as shift in the reference's form) */
+int wrtk_reduction(const double* tab, int nz, const double* shift, int nshift,
+                   const double* pshift, int np, int nphi,
                    const double* theta_trig /* [ntheta][2] cos sin */, int ntheta, double* out,
                    hipStream_t st);
+/* rays[z][s][k] (nrphi angles) -> out[z][j][s] at nphi plane azimuths; stencil[j][12] =
+ * 4 weights, 4 ray-angle indices, 4 flags (1: row taken at -s) */
+int wrtk_ray_phi_resample(const double* rays, int nz, int nshift, int nrphi,
+                          const double* stencil, int nphi, double* out, hipStream_t st);
 /* tab rows (phi, theta): r = iphi*ntheta + itheta;
  * wgt[theta] = (2pi/nphi) * w_theta, rounded once (integral.c:20) */
 int wrtk_adjoint(const double* tab, const double* shift, int nshift,

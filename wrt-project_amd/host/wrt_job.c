@@ -307,8 +307,8 @@ static int job_enqueue_f64(wrt_job* j) {
                             d->nz, d->nphi, (long long)d->nshift * d->nphi, d->nphi, st);
       if (rc != WRT_OK) return rc;
       HIPCHK(hipEventRecord(j->e1, st));
-      rc = wrtk_reduction(j->d_spl, d->nz, j->d_shift, d->nshift, d->nphi, j->d_ttrig,
-                          d->ntheta, (double*)j->d_out, st);
+      rc = wrtk_reduction(j->d_spl, d->nz, j->d_shift, d->nshift, j->d_shift, d->nshift,
+                          d->nphi, j->d_ttrig, d->ntheta, (double*)j->d_out, st);
       break;
     case WRT_OP_ADJOINT:
       rc = wrtk_spline_rows(j->d_shift, d->nshift, j->d_fac, (const double*)j->d_in, j->d_spl,

@@ -74,7 +74,7 @@ _lib = None
 ABI_SYMBOLS = [
     "wrt_init", "wrt_free", "wrt_strerror", "wrt_mem_info", "wrt_gl_nodes",
     "wrt_radon_grid_init", "wrt_ray_grid_init", "wrt_zslice_grid", "wrt_radon_grid",
-    "wrt_ray_grid", "wrt_radon_reduction", "wrt_radon_adjoint", "wrt_radon_grid_T",
+    "wrt_ray_grid", "wrt_radon_reduction", "wrt_radon_reduction_rays", "wrt_radon_adjoint", "wrt_radon_grid_T",
     "wrt_ray_grid_T", "wrt_job_bytes", "wrt_job_input_count", "wrt_job_output_count",
     "wrt_job_create", "wrt_job_load", "wrt_job_load_synthetic", "wrt_job_run",
     "wrt_job_fetch", "wrt_job_destroy", "wrt_batch_run", "wrt_phantom",
@@ -185,6 +185,7 @@ def lib():
         L.wrt_radon_grid.argtypes = [P, P, I, P, I, P, I, P, I, I, P, P]
         L.wrt_ray_grid.argtypes = [P, P, I, P, I, P, I, P, I, I, P, P]
         L.wrt_radon_reduction.argtypes = [P, P, I, P, I, P, I, P, I, P, P]
+        L.wrt_radon_reduction_rays.argtypes = [P, P, I, P, I, I, I, P, I, P, I, P, I, P, P]
         L.wrt_radon_adjoint.argtypes = [P, P, P, I, P, P, I, P, I, I, I, P, P]
         L.wrt_radon_grid_T.argtypes = [P, P, I, P, I, P, I, P, I, I, P, P]
         L.wrt_ray_grid_T.argtypes = [P, P, I, P, I, P, I, P, I, I, P, P]
@@ -345,6 +346,21 @@ class Context:
         _chk(lib().wrt_radon_reduction(self.h, _p(rays), rays.shape[0], _p(shift), len(shift),
                                        _p(phi), len(phi), _p(theta), len(theta), _p(out),
                                        C.byref(st)), "wrt_radon_reduction")
+        return out, st.as_dict()
+
+    def radon_reduction_rays(self, rays, rshift, half_turn, shift, phi, theta):
+        """rays[z][s][k] on their own shift grid and nrphi = rays.shape[2] angles k*pi/nrphi
+        (half_turn) or k*2pi/nrphi -> out[s][phi][theta] at the planes (shift, phi, theta)."""
+        rays, rshift, shift, phi, theta = map(_f64, (rays, rshift, shift, phi, theta))
+        if rays.ndim != 3 or rays.shape[1] != len(rshift):
+            raise ValueError("rays must be [z][s][phi] on the ray shift grid")
+        out = np.empty((len(shift), len(phi), len(theta)))
+        st = Stats()
+        _chk(lib().wrt_radon_reduction_rays(self.h, _p(rays), rays.shape[0], _p(rshift),
+                                            len(rshift), rays.shape[2], int(bool(half_turn)),
+                                            _p(shift), len(shift), _p(phi), len(phi), _p(theta),
+                                            len(theta), _p(out), C.byref(st)),
+             "wrt_radon_reduction_rays")
         return out, st.as_dict()
 
     def radon_adjoint(self, radon, phi, theta, theta_w, shift, n, prec="f64"):
