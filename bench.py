@@ -74,6 +74,7 @@ def rel_l2(a, b):
 
 
 REF_RADON_LIB = os.path.join(ROOT, "oracle", "_ref", "libref_radon_grid.so")
+REF_RAY_LIB = os.path.join(ROOT, "oracle", "_ref", "libref_ray_grid.so")
 
 
 def _seeded_planes(rng, phi, th, s, m):
@@ -149,6 +150,74 @@ def cpu_baseline(vol, phi, th, s, seconds=12.0):
                       "C restatement at -O2 with OpenMP"}
 
 
+def _jagged(v):
+    """the reference's double*** view of a contiguous n^3 buffer (kept alive by the caller)"""
+    import ctypes as C
+
+    n = v.shape[0]
+    rows = (C.c_void_p * (n * n))(*[v.ctypes.data + 8 * n * r for r in range(n * n)])
+    top = (C.c_void_p * n)(*[C.addressof(rows) + C.sizeof(C.c_void_p) * n * x for x in range(n)])
+    return rows, top
+
+
+def rays_cpu(vol, z, s, phi, rays_in, rs, rphi, th, seconds):
+    """CPU rates beside the rays leg on bounded seeded samples: the reference's own
+    ray_transform (ray_grid/src/integral.c:11-50, built from its sources at -O0 by
+    oracle/Makefile) over seeded rays of the grid, one thread per core as its OpenMP driver
+    runs it (ray_grid/src/ray_grid.c:242-255); and the oracle's reduction (its spline rows
+    solved once, OpenMP over planes) on a subset of theta of the reduction chain."""
+    import ctypes as C
+    from concurrent.futures import ThreadPoolExecutor
+
+    import orc
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    rng = np.random.default_rng(4321)
+    n = vol.shape[0]
+    out = {}
+    if os.path.exists(REF_RAY_LIB):
+        lib = C.CDLL(REF_RAY_LIB)
+        lib.ray_transform.restype = C.c_double
+        lib.ray_transform.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double]
+        v = np.ascontiguousarray(vol, dtype=np.float64)
+        rows, top = _jagged(v)
+
+        def run(args):
+            return [lib.ray_transform(top, n, float(a), float(b), float(c)) for a, b, c in zip(*args)]
+
+        m = 4 * threads
+        while True:
+            zz, ss, pp = (a[rng.integers(len(a), size=m)] for a in (z, s, phi))
+            parts = [(zz[k::threads], ss[k::threads], pp[k::threads]) for k in range(threads)]
+            with ThreadPoolExecutor(threads) as ex:
+                t0 = time.perf_counter()
+                list(ex.map(run, parts))
+                dt = time.perf_counter() - t0
+            if dt >= 0.8 * seconds or m >= 1 << 22:
+                break
+            m = max(m + threads, int(m * min(16.0, 1.1 * seconds / max(dt, 1e-3))) // threads * threads)
+        out["ray_grid"] = {"value": m * (4.0 * n + 1.0) / dt, "unit": "ray samples/s",
+                           "cores": threads, "kind": "reference",
+                           "sample": f"{m} seeded rays of the {n}^3 grid, {dt:.1f} s; the "
+                                     "reference's ray_transform built from its sources at -O0"}
+    k = 1
+    while True:
+        sub = np.ascontiguousarray(th[:: max(1, len(th) // k)][:k])
+        t0 = time.perf_counter()
+        orc.radon_reduction(rays_in, rs, rphi, sub)
+        dt = time.perf_counter() - t0
+        if dt >= 0.5 * seconds or k >= len(th):
+            break
+        k = min(len(th), max(k + 1, int(k * 0.9 * seconds / max(dt, 1e-3))))
+    nz = rays_in.shape[0]
+    out["reduction"] = {"value": (4.0 * nz + 1.0) * len(rs) * len(rphi) * k / dt,
+                        "unit": "tau-steps/s", "cores": threads, "kind": "port",
+                        "sample": f"{k} of {len(th)} theta of the reduction chain, {dt:.1f} s; "
+                                  "the oracle's C restatement at -O2 with OpenMP (spline rows "
+                                  "solved once, where the reference solves 2-3 per tau-step)"}
+    return out
+
+
 def batch_leg(ctx, wrt, nvol, seed=3):
     rng = np.random.default_rng(seed)
     edges = [int(e) for e in rng.choice([32, 48, 64, 96, 128, 160, 192], size=nvol)]
@@ -217,7 +286,7 @@ def transpose_leg(ctx, wrt, vol, fast, phi, th, s, stride):
             "plain_f64_subset_s": out["f64"][1], "f32_subset_s": out["f32"][1]}
 
 
-def rays_leg(ctx, wrt):
+def rays_leg(ctx, wrt, cpu_seconds=4.0):
     """The line-integral workload at 128^3: 128 z-slices x 180 phi in [0, pi) x 257 shifts,
     FP32 against the plain FP64 form; the FP32 exact transpose and the dot-product test of
     the FP32 pair on seeded Gaussian f and g; the FP64 reduction chain (rays on 128
@@ -270,6 +339,8 @@ def rays_leg(ctx, wrt):
     trays = res["f64"][0].reshape(len(z), len(s), len(phi))
     for _ in range(2):  # the second call is the one reported
         gred, gst = ctx.radon_reduction_rays(trays, s, True, rs, rphi, th)
+    cpu = rays_cpu(vol, z, s, phi, rays.reshape(len(rz), len(rs), len(rphi)), rs, rphi, th,
+                   cpu_seconds) if cpu_seconds > 0 else None
     f32, st32 = res["f32"]
     f64, st64 = res["f64"]
     return {"workload": "ray_grid 128^3, 128 z x 180 phi in [0,pi) x 257 shifts",
@@ -286,7 +357,8 @@ def rays_leg(ctx, wrt):
             "reduction_from_throughput_rays": {
                 "grid": "rays 128 z x 180 phi in [0,pi) x 257 s -> 64 phi x 32 theta x 129 s",
                 "seconds": gst["seconds"], "reduction_f64_s": gst["kernel_seconds"],
-                "rel_l2_vs_parity_chain": rel_l2(gred.ravel(), red)}}
+                "rel_l2_vs_parity_chain": rel_l2(gred.ravel(), red)},
+            "cpu": cpu}
 
 
 def analytic_leg(ctx, wrt, seconds=6.0):
@@ -612,7 +684,7 @@ def main():
     rays = None
     if not args.no_rays:
         log("bench: rays")
-        rays = rays_leg(ctx, wrt)
+        rays = rays_leg(ctx, wrt, cpu_seconds=0.0 if args.no_cpu else 4.0)
         log(f"bench: rays f32 {rays['f32_s'] * 1e3:.2f} ms, f64 {rays['f64_s'] * 1e3:.2f} ms, "
             f"dot rel {rays['dot_rel']:.2e}")
     analytic = None
