@@ -30,6 +30,17 @@ def test_fast_f32_matches_plain(ctx, n):
     assert st["logical_samples"] == 16.0 * n * n * ref.size
 
 
+@pytest.mark.parametrize("n", [300, 520])
+def test_fast_f32_large_odd_edges(ctx, n):
+    # edges past the benchmark's 256 that are not multiples of the 32^3 brick (partial face
+    # bricks on three faces), on a coarse direction grid
+    vol, phi, th, s = _phantom_case(ctx, n, 8, 4, 33, seed=n, sigma=0.01)
+    ref, _ = ctx.radon_grid(vol, phi, th, s, prec="f64")
+    got, _ = ctx.radon_grid(vol, phi, th, s, prec="f32")
+    assert orc.rel_l2(got, ref) <= 1e-5, orc.rel_l2(got, ref)
+    assert orc.max_rel(got, ref) <= 1e-5, orc.max_rel(got, ref)
+
+
 def test_fast_f32_c0_full_grid(ctx):
     vol, phi, th, s = _phantom_case(ctx, 64, 64, 32, 129)
     ref, _ = ctx.radon_grid(vol, phi, th, s, prec="f64")
@@ -160,6 +171,20 @@ def test_fast_ray_f32_matches_plain(ctx, n):
     assert np.max(np.abs(got - ref)) <= 1e-5 * np.max(np.abs(ref))
     assert np.all(got[-2:] == 0.0)
     assert st["logical_samples"] == (4.0 * n + 1) * got.size
+
+
+def test_fast_ray_f32_above_fixed_point_edge(ctx):
+    # n = 520 is past the edge (500) below which ray points step in 32.32 fixed point; here
+    # every point takes the binary64 form of the reference's index arithmetic
+    n = 520
+    vol = wrt.phantom(n, False, 10, 7, 0.01)
+    s, phi = wrt.ray_grid_init(33, 8)
+    phi = np.concatenate([phi, [0.3, 2.0]])
+    z = np.array([-0.51, 0.0, 0.37])
+    ref, _ = ctx.ray_grid(vol, z, s, phi, prec="f64")
+    got, _ = ctx.ray_grid(vol, z, s, phi, prec="f32")
+    assert orc.rel_l2(got, ref) <= 1e-5, orc.rel_l2(got, ref)
+    assert np.max(np.abs(got - ref)) <= 1e-5 * np.max(np.abs(ref))
 
 
 def test_fast_ray_f32_random_volume_and_angles(ctx):
