@@ -220,13 +220,20 @@ def test_fast_ray_f32_rays_on_cube_faces(ctx, n):
 
 
 @pytest.mark.parametrize("n", [32, 33, 40, 48, 64])
-def test_fast_f32_index_decisions_exact(ctx, n):
+@pytest.mark.parametrize("support", ["cube", "ball"])
+def test_fast_f32_index_decisions_exact(ctx, n, support):
     # Small integer values make every sum exact (FP32 band partials, FP64 totals), so the
     # fast form must give the plain form's integer sum on every plane: a single sample put
     # into another cell changes it (unless the two cells hold equal values).  Random and
     # structured directions, shifts on cell faces and on the cube's faces.
     rng = np.random.default_rng(100 + n)
     vol = rng.integers(-8, 9, size=(n, n, n)).astype(np.float64)
+    if support == "ball":
+        # zero outside radius 0.7: rows are clipped to the support sphere, whose edge then
+        # cuts through non-zero cells on every plane that meets it
+        c = -1.0 + (np.arange(n) + 0.5) * (2.0 / n)
+        X, Y, Z = np.meshgrid(c, c, c, indexing="ij")
+        vol[X ** 2 + Y ** 2 + Z ** 2 > 0.49] = 0.0
     phi = np.concatenate([[0.0, np.pi / 4, np.pi / 2, np.pi, 1.5 * np.pi], rng.uniform(0, 2 * np.pi, 5)])
     _, th_gl, _, _ = ctx.radon_grid_init(4, 6, 3)
     th = np.concatenate([th_gl, [np.pi / 2, np.pi / 4, 0.0], np.arccos(rng.uniform(-1, 1, 3))])

@@ -112,6 +112,10 @@ __device__ __forceinline__ double uniform_f64(double v) {
                           __builtin_amdgcn_readfirstlane(__double2loint(v)));
 }
 
+__device__ __forceinline__ float uniform_f32(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
 __device__ __forceinline__ float bf16_to_f32(unsigned short v) {
   return __uint_as_float(((unsigned)v) << 16);
 }
@@ -250,6 +254,20 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
     for (int c = 0; c < 3; ++c) f2c += fmax(lo[c] * lo[c], hi[c] * hi[c]);
     inside = f2c * (1.0 + 1e-12) < a.rho2;
   }
+  // The rows' candidate column intervals are computed in FP32 against a slab widened by
+  // RI_TOL in coordinates and a support sphere whose rho^2 is widened by 1e-4 (1 + rho^2),
+  // and then widened by RI_COLS columns.  Coordinates are at most 3 in magnitude, so FP32
+  // rounding moves a row's point by under 1e-6 and a column bound (|t| <= 4n + 2, n <= 4096)
+  // by under 5e-3 columns: the intervals still hold every column whose sample can fall in the
+  // brick and the sphere.  Which samples count is decided per sample, exactly.
+  constexpr float RI_TOL = 4e-6f, RI_COLS = 1.0f / 64.0f;
+  float lof[3], hcf[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    lof[c] = uniform_f32((float)lo[c] - RI_TOL);
+    hcf[c] = uniform_f32((float)fmin(hi[c], 1.0) + RI_TOL);
+  }
+  const float rho2f = uniform_f32((float)(a.rho2 + 1e-4 * (1.0 + a.rho2)));
 
   const int d_end = min((grp + 1) * a.dgroup, a.ndir);
   // waves take the group's directions one at a time from a shared counter, so that they
@@ -296,6 +314,14 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
 #pragma unroll
     for (int c = 0; c < 3; ++c) ib[c] = (fabs(bv[c]) * (4.0 * n) < 1e-9 * dc) ? 0.0 : 1.0 / bv[c];
     const double ibb = 1.0 / bb, iaa = 1.0 / aa;
+    float avf[3], bvf[3], ibf[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      avf[c] = uniform_f32((float)av[c]);
+      bvf[c] = uniform_f32((float)bv[c]);
+      ibf[c] = uniform_f32((float)ib[c]);
+    }
+    const float bbf = uniform_f32((float)bb), ibbf = uniform_f32((float)ibb);
     // this lane's box edge (lanes 0..11): axis ek, the other two coordinates at lo/hi
     // corners.  The plane n.p = s meets it (if at all) at coordinate pk = (s - rest)/n_k,
     // which is on the edge for s in [e_s0, e_s1]; the vertex's row a.p/|a|^2 is then the
@@ -362,6 +388,8 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
       if (rc2 < 0.0) continue;
       const double S[3] = {uniform_f64(s * nrm[0]), uniform_f64(s * nrm[1]),
                            uniform_f64(s * nrm[2])};
+      const float Sf[3] = {uniform_f32((float)S[0]), uniform_f32((float)S[1]),
+                           uniform_f32((float)S[2])};
       // rows of this section: the section polygon's vertices lie on the box edges; the row
       // of a point p of the plane is a.p / |a|^2 (a is orthogonal to n and b), +-2 rows margin
       int r_lo = INT_MAX, r_hi = INT_MIN;
@@ -393,48 +421,46 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
         const int t = lane + 64 * half;
         int rl = C0, rh = C0 - 1, nst = 0;
         if (t < nrows) {
-          const double f1 = (double)(SR0 + t);
-          double tlo = -1e300, thi = 1e300;
+          const float f1f = (float)(SR0 + t);  // exact: |row| <= 2n
+          float tlo = -3.0e38f, thi = 3.0e38f;
           bool empty = false;
           // the row's points p(t) = w + b t within the support sphere |p|^2 <= rho2 (every
           // sample outside it reads a zero cell, so clipping the row there changes no sum);
           // nothing to clip when the whole brick lies inside the sphere
           if (!inside) {
-            const double wx = S[0] + av[0] * f1, wy = S[1] + av[1] * f1, wz = S[2] + av[2] * f1;
-            const double qb = wx * bv[0] + wy * bv[1] + wz * bv[2];
-            const double qc = wx * wx + wy * wy + wz * wz - a.rho2;
-            const double disc = qb * qb - bb * qc;
-            if (disc < 0.0) {
+            const float wx = fmaf(avf[0], f1f, Sf[0]), wy = fmaf(avf[1], f1f, Sf[1]),
+                        wz = fmaf(avf[2], f1f, Sf[2]);
+            const float qb = wx * bvf[0] + wy * bvf[1] + wz * bvf[2];
+            const float qc = wx * wx + wy * wy + wz * wz - rho2f;
+            const float disc = qb * qb - bbf * qc;
+            if (disc < 0.0f) {
               empty = true;
             } else {
-              const double sq = sqrt(disc);
-              tlo = (-qb - sq) * ibb;
-              thi = (-qb + sq) * ibb;
+              const float sq = sqrtf(disc);
+              tlo = (-qb - sq) * ibbf;
+              thi = (-qb + sq) * ibbf;
             }
           }
 #pragma unroll
           for (int c = 0; c < 3; ++c) {
-            const double w = S[c] + av[c] * f1;
-            const double tol = 1e-9 * dc;
-            const double hc = fmin(hi[c], 1.0);  // the cube's face (n not a multiple of 32)
-            if (ib[c] == 0.0) {
-              if (w < lo[c] - tol || w > hc + tol) empty = true;
-            } else if (bv[c] > 0) {
-              tlo = fmax(tlo, (lo[c] - tol - w) * ib[c]);
-              thi = fmin(thi, (hc + tol - w) * ib[c]);
+            const float w = fmaf(avf[c], f1f, Sf[c]);
+            if (ibf[c] == 0.0f) {
+              if (w < lof[c] || w > hcf[c]) empty = true;
+            } else if (bvf[c] > 0.0f) {
+              tlo = fmaxf(tlo, (lof[c] - w) * ibf[c]);
+              thi = fminf(thi, (hcf[c] - w) * ibf[c]);
             } else {
-              tlo = fmax(tlo, (hc + tol - w) * ib[c]);
-              thi = fmin(thi, (lo[c] - tol - w) * ib[c]);
+              tlo = fmaxf(tlo, (hcf[c] - w) * ibf[c]);
+              thi = fminf(thi, (lof[c] - w) * ibf[c]);
             }
           }
-          if (!empty && tlo <= thi + 2.0) {
-            // the continuous bounds are accurate to ~1e-12 columns (and the slab tolerance
-            // widens them), so the integer columns within 1e-6 of them are the candidates
-            tlo = fmax(tlo, (double)C0 - 1.0);
-            thi = fmin(thi, (double)C1 + 1.0);
-            rl = max(C0, (int)ceil(tlo - 1e-6));
-            rh = min(C1, (int)floor(thi + 1e-6));
+          if (!empty && tlo <= thi + 2.0f) {
+            tlo = fmaxf(tlo, (float)(C0 - 1));
+            thi = fminf(thi, (float)(C1 + 1));
+            rl = max(C0, (int)ceilf(tlo - RI_COLS));
+            rh = min(C1, (int)floorf(thi + RI_COLS));
             if (BND) {
+              const double f1 = (double)(SR0 + t);
 #pragma unroll
               for (int c = 0; c < 3; ++c)
                 if (flo[c] || fhi[c]) cube_ends(S[c] + av[c] * f1, bv[c], flo[c], fhi[c], rl, rh);
