@@ -123,8 +123,9 @@ __device__ __forceinline__ float bf16_to_f32(unsigned short v) {
 // Exact cube test of one axis on a row: x(c) = w + b*c in binary64 (the reference's
 // rounding) is monotone in c, so {c : x >= -1} and {c : x <= 1} are a prefix or a suffix of
 // the row.  Narrows [rl, rh] to the columns passing the tests that lo / hi ask for.  The
-// continuous bounds already lie within ~1e-6 columns of the answer, so the first probe
-// usually settles it; a bisection covers rows almost parallel to the face.
+// candidate interval comes from continuous bounds widened by a small slack, so the first
+// probe or its neighbour usually settles it; a bisection covers rows almost parallel to the
+// face.
 __device__ __forceinline__ void cube_ends(double w, double b, bool lo, bool hi, int& rl,
                                           int& rh) {
   if (rl > rh) return;
@@ -140,7 +141,9 @@ __device__ __forceinline__ void cube_ends(double w, double b, bool lo, bool hi, 
       if (!P(rl)) rh = rl - 1;
     } else if (suffix) {
       if (!P(rl)) {
-        if (!P(rh)) {
+        if (rl < rh && P(rl + 1)) {
+          rl = rl + 1;  // the candidate interval's slack added one column
+        } else if (!P(rh)) {
           rh = rl - 1;
         } else {
           int l = rl, h = rh;  // P(l) false, P(h) true
@@ -153,7 +156,9 @@ __device__ __forceinline__ void cube_ends(double w, double b, bool lo, bool hi, 
       }
     } else {
       if (!P(rh)) {
-        if (!P(rl)) {
+        if (rl < rh && P(rh - 1)) {
+          rh = rh - 1;
+        } else if (!P(rl)) {
           rh = rl - 1;
         } else {
           int l = rl, h = rh;  // P(l) true, P(h) false
