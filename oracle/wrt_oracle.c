@@ -18,7 +18,10 @@
  * cube_trilinear_interp: bit-exact), numpy leggauss and the README theta values (GL),
  * scipy CubicSpline(bc_type='natural') (spline), exact quadratics (polynomial),
  * closed-form ellipsoid plane/line integrals and constant-data adjoint (4*pi*c),
- * and the dot-product test for the transposes.
+ * and the dot-product test for the transposes.  The reduction on independent grids (no
+ * reference counterpart) is pinned to the reference form on identical grids (bit for bit),
+ * to reflected full-turn rays, to cubics in phi and to an off-centre Gaussian's closed-form
+ * plane integrals, with planted stencil mistakes that fail the last check.
  */
 #include "wrt_oracle.h"
 
