@@ -507,7 +507,7 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
                                  (int)(qb >> 32)) << 32) |
                             (unsigned)__builtin_amdgcn_readfirstlane((int)qb));
       }
-      // One sample: Q the lane's fixed-point coordinates, f1 its row and c0 + step*m its
+      // One sample: Q the lane's fixed-point coordinates, row its row and c0 + step*m its
       // column (for the exact decision), mlast the last step inside the row (face bricks).
       // Forward: adds the cell's value to bs; transpose: adds the section's quantized value
       // to the cell.
@@ -526,7 +526,7 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
           off = ok ? off : 0u;
         }
       };
-      auto sample = [&](const long long (&Q)[3], double f1, int c0, int step, int m, int mlast,
+      auto sample = [&](const long long (&Q)[3], int row, int c0, int step, int m, int mlast,
                         float& bs) {
         const unsigned u = min(min((unsigned)Q[0], (unsigned)Q[1]), (unsigned)Q[2]);
         // the fixed point's cell (away from a boundary the fixed point floors where the
@@ -541,6 +541,7 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
           // rounds delta*v2*i2); the step index through readfirstlane keeps the column out
           // of the common path
           const double f2 = (double)(c0 + step * __builtin_amdgcn_readfirstlane(m));
+          const double f1 = (double)row;
           const double Rx = S[0] + av[0] * f1, Ry = S[1] + av[1] * f1, Rz = S[2] + av[2] * f1;
           const double x = Rx + bv[0] * f2, y = Ry + bv[1] * f2, z = Rz + bv[2] * f2;
           double qx, qy, qz;
@@ -593,33 +594,40 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
         const int steps = (k < 8) ? __builtin_amdgcn_readlane(bsteps[0], 8 * k)
                                   : __builtin_amdgcn_readlane(bsteps[1], 8 * (k - 8));
         if (steps == 0) continue;
-        const double f1 = (double)(SR0 + t);
+        const int row = SR0 + t;
         const int rhi = (int)frh;
         float bs = 0.f;
         if constexpr (T) {
           const int steps2 = (steps + 1) >> 1;  // 16-column steps cover the 8-column ones
           for (int dcol = 0; dcol < 2; ++dcol) {
             const int c0 = rlo + dcol + 2 * k_lane;  // first column i2 of the lane
+            int tq = t;  // as in the forward walk below
+            asm volatile("" : "+v"(tq));
             long long Q[3];
 #pragma unroll
             for (int c = 0; c < 3; ++c)
-              Q[c] = Qb[c] + (long long)t * Afx[c] + (long long)(c0 - C0) * Dfx[c];
+              Q[c] = Qb[c] + (long long)tq * Afx[c] + (long long)(c0 - C0) * Dfx[c];
             const int mlast = (rhi - c0) >> 4;  // floor: negative when the row ends before
             for (int m = 0; m < steps2; ++m) {
-              sample(Q, f1, c0, 16, m, mlast, bs);
+              sample(Q, row, c0, 16, m, mlast, bs);
 #pragma unroll
               for (int c = 0; c < 3; ++c) Q[c] += 2 * D8[c];
             }
           }
         } else {
           const int c0 = rlo + k_lane;  // first column i2 of the lane
+          // t passes through an empty asm so that its products stay signed 32 x 32 -> 64
+          // multiply-adds (one instruction each) instead of the two unsigned ones the
+          // compiler picks for a known non-negative factor
+          int tq = t;
+          asm volatile("" : "+v"(tq));
           long long Q[3];
 #pragma unroll
           for (int c = 0; c < 3; ++c)
-            Q[c] = Qb[c] + (long long)t * Afx[c] + (long long)(c0 - C0) * Dfx[c];
+            Q[c] = Qb[c] + (long long)tq * Afx[c] + (long long)(c0 - C0) * Dfx[c];
           const int mlast = (rhi - c0) >> 3;
           for (int m = 0; m < steps; ++m) {
-            sample(Q, f1, c0, 8, m, mlast, bs);
+            sample(Q, row, c0, 8, m, mlast, bs);
 #pragma unroll
             for (int c = 0; c < 3; ++c) Q[c] += D8[c];
           }
