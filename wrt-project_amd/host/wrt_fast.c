@@ -50,6 +50,10 @@ typedef struct {
   /* host copy of the non-empty bricks, heaviest first (batch planning) */
   int *ne_brick, *ne_count, n_ne;
   unsigned char* ne_bnd;
+  /* the face-brick launch runs on its own stream (default priority), forked from and
+   * joined back into the job's stream, so that it overlaps the interior launch's tail */
+  hipStream_t side;
+  hipEvent_t fork, join;
 } radon_fast;
 
 /* ADJOINT in FP32: per-interval FP32 cubics of the FP64 spline, float4 direction table */
@@ -299,6 +303,12 @@ static size_t radon_fast_bytes(const wrt_job_desc* d) {
 static void radon_fast_destroy(wrt_job* j) {
   radon_fast* f = (radon_fast*)j->fast;
   if (!f) return;
+  if (f->side) {
+    hipStreamSynchronize(f->side);
+    hipStreamDestroy(f->side);
+  }
+  if (f->fork) hipEventDestroy(f->fork);
+  if (f->join) hipEventDestroy(f->join);
   hipFree(f->bricks);
   hipFree(f->flags);
   hipFree(f->tasks);
@@ -331,6 +341,12 @@ static int radon_fast_create(wrt_job* j) {
       hipMalloc((void**)&f->staging, (size_t)d->n * d->n * d->n * sizeof(double)) != hipSuccess) {
     radon_fast_destroy(j);
     return WRT_E_NOMEM;
+  }
+  if (hipStreamCreateWithFlags(&f->side, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&f->fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&f->join, hipEventDisableTiming) != hipSuccess) {
+    radon_fast_destroy(j);
+    return WRT_E_HIP;
   }
   return WRT_OK;
 }
@@ -452,14 +468,21 @@ static int radon_fast_enqueue(wrt_job* j) {
   a.nshift = d->nshift;
   a.acc = f->acc;
   a.rho2 = f->rho2;
+  /* face bricks on the side stream, interior bricks on the job's stream, concurrently:
+   * both add into the same FP64 accumulators with atomics */
+  HIPCHK(hipEventRecord(f->fork, st));
+  HIPCHK(hipStreamWaitEvent(f->side, f->fork, 0));
+  wrt_radon_fast_args b = a;
+  b.tasks = f->tasks + 2 * (size_t)f->n_interior;
+  b.ntasks = f->n_boundary;
+  rc = wrtk_radon_fast(&b, 1, f->side);
+  if (rc != WRT_OK) return rc;
+  HIPCHK(hipEventRecord(f->join, f->side));
   a.tasks = f->tasks;
   a.ntasks = f->n_interior;
   rc = wrtk_radon_fast(&a, 0, st);
   if (rc != WRT_OK) return rc;
-  a.tasks = f->tasks + 2 * (size_t)f->n_interior;
-  a.ntasks = f->n_boundary;
-  rc = wrtk_radon_fast(&a, 1, st);
-  if (rc != WRT_OK) return rc;
+  HIPCHK(hipStreamWaitEvent(st, f->join, 0));
   HIPCHK(hipEventRecord(j->e3, st));
   return wrtk_acc_to_f32(f->acc, f->out, nout, st);
 }
