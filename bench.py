@@ -774,7 +774,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic_from_profile(),
                      "alg_bytes": alg_bytes,
-                     "kernel": "k_radon_fast (interior + face-brick launches)",
+                     "kernel": "k_radon_fast (interior and face-brick launches, concurrent; their span)",
                      "kernel_seconds": kavg},
         "cpu_baseline": cpu,
         "serve": serve,
