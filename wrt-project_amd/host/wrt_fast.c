@@ -50,9 +50,7 @@ typedef struct {
   /* host copy of the non-empty bricks, heaviest first (batch planning) */
   int *ne_brick, *ne_count, n_ne;
   unsigned char* ne_bnd;
-  /* the face-brick launch runs on its own stream (default priority), forked from and
-   * joined back into the job's stream, so that it overlaps the interior launch's tail */
-  hipStream_t side;
+  hipStream_t side; /* the face-brick launch (side_init) */
   hipEvent_t fork, join;
 } radon_fast;
 
@@ -300,15 +298,30 @@ static size_t radon_fast_bytes(const wrt_job_desc* d) {
          (size_t)d->nshift * ndir * 12 + (size_t)d->n * d->n * d->n * 8;
 }
 
+/* A side stream (default priority) with the events that fork it from and join it back into
+ * a job's stream: a job's face-brick launch runs there, concurrently with its interior
+ * launch, so that neither launch's tail leaves the device idle. */
+static int side_init(hipStream_t* side, hipEvent_t* fork, hipEvent_t* join) {
+  if (hipStreamCreateWithFlags(side, hipStreamNonBlocking) != hipSuccess) return WRT_E_HIP;
+  if (hipEventCreateWithFlags(fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(join, hipEventDisableTiming) != hipSuccess)
+    return WRT_E_HIP;
+  return WRT_OK;
+}
+
+static void side_free(hipStream_t side, hipEvent_t fork, hipEvent_t join) {
+  if (side) {
+    hipStreamSynchronize(side);
+    hipStreamDestroy(side);
+  }
+  if (fork) hipEventDestroy(fork);
+  if (join) hipEventDestroy(join);
+}
+
 static void radon_fast_destroy(wrt_job* j) {
   radon_fast* f = (radon_fast*)j->fast;
   if (!f) return;
-  if (f->side) {
-    hipStreamSynchronize(f->side);
-    hipStreamDestroy(f->side);
-  }
-  if (f->fork) hipEventDestroy(f->fork);
-  if (f->join) hipEventDestroy(f->join);
+  side_free(f->side, f->fork, f->join);
   hipFree(f->bricks);
   hipFree(f->flags);
   hipFree(f->tasks);
@@ -342,9 +355,7 @@ static int radon_fast_create(wrt_job* j) {
     radon_fast_destroy(j);
     return WRT_E_NOMEM;
   }
-  if (hipStreamCreateWithFlags(&f->side, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreateWithFlags(&f->fork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&f->join, hipEventDisableTiming) != hipSuccess) {
+  if (side_init(&f->side, &f->fork, &f->join) != WRT_OK) {
     radon_fast_destroy(j);
     return WRT_E_HIP;
   }
@@ -500,6 +511,8 @@ typedef struct {
   float* g32;      /* FP32 plane values */
   long long* part; /* [ngroups][nbricks][WRT_BRICK^3] fixed-point partials */
   float* out;      /* n^3 [x][y][z] */
+  hipStream_t side; /* the face-brick launch (side_init) */
+  hipEvent_t fork, join;
 } radon_T_fast;
 
 #define RT_PART_BUDGET ((size_t)1 << 30) /* bytes of group partials at most */
@@ -552,6 +565,7 @@ static size_t radon_T_fast_bytes(const wrt_job_desc* d) {
 static void radon_T_fast_destroy(wrt_job* j) {
   radon_T_fast* f = (radon_T_fast*)j->fast;
   if (!f) return;
+  side_free(f->side, f->fork, f->join);
   hipFree(f->tasks);
   hipFree(f->staging);
   hipFree(f->g32);
@@ -580,6 +594,7 @@ static int radon_T_fast_create(wrt_job* j) {
       hipMalloc((void**)&f->part, (size_t)f->ngroups * f->nbricks * WRT_BRICK * WRT_BRICK * WRT_BRICK * 8) != hipSuccess ||
       hipMalloc((void**)&f->out, (size_t)d->n * d->n * d->n * 4) != hipSuccess)
     return WRT_E_NOMEM;
+  if (side_init(&f->side, &f->fork, &f->join) != WRT_OK) return WRT_E_HIP;
   /* bricks nearest the centre first (planes with |s| <= 1 cut them from every direction, so
    * their tasks are the longest), interior bricks then bricks on the cube's faces */
   int* order = (int*)malloc(f->nbricks * sizeof(int));
@@ -680,14 +695,21 @@ static int radon_T_fast_enqueue(wrt_job* j) {
   a.gscale = f->gscale;
   a.part = f->part;
   HIPCHK(hipEventRecord(j->e1, st));
+  /* face half bricks on the side stream, concurrently with the interior ones: every task
+   * writes its own partial slot */
+  HIPCHK(hipEventRecord(f->fork, st));
+  HIPCHK(hipStreamWaitEvent(f->side, f->fork, 0));
+  wrt_radon_fast_args b = a;
+  b.tasks = f->tasks + 2 * (size_t)f->n_interior;
+  b.ntasks = f->n_boundary;
+  int rc = wrtk_radon_T_fast(&b, 1, f->side);
+  if (rc != WRT_OK) return rc;
+  HIPCHK(hipEventRecord(f->join, f->side));
   a.tasks = f->tasks;
   a.ntasks = f->n_interior;
-  int rc = wrtk_radon_T_fast(&a, 0, st);
+  rc = wrtk_radon_T_fast(&a, 0, st);
   if (rc != WRT_OK) return rc;
-  a.tasks = f->tasks + 2 * (size_t)f->n_interior;
-  a.ntasks = f->n_boundary;
-  rc = wrtk_radon_T_fast(&a, 1, st);
-  if (rc != WRT_OK) return rc;
+  HIPCHK(hipStreamWaitEvent(st, f->join, 0));
   HIPCHK(hipEventRecord(j->e3, st));
   const double delta = (1.0 / (d->n));
   return wrtk_radon_T_reduce(f->part, f->ngroups, d->n, f->nb, delta * delta / f->gscale,
