@@ -299,9 +299,14 @@ static size_t radon_fast_bytes(const wrt_job_desc* d) {
 }
 
 /* A side stream (default priority) with the events that fork it from and join it back into
- * a job's stream: a job's face-brick launch runs there, concurrently with its interior
- * launch, so that neither launch's tail leaves the device idle. */
-static int side_init(hipStream_t* side, hipEvent_t* fork, hipEvent_t* join) {
+ * a job's stream: a large job's face-brick launch runs there, concurrently with its interior
+ * launch, so that neither launch's tail leaves the device idle.  Jobs below SIDE_MIN_N
+ * keep one stream: their launches are short, and a second stream per job would make the
+ * serve loop's concurrent jobs share the process's few hardware queues more often. */
+#define SIDE_MIN_N 192
+static int side_init(const wrt_job_desc* d, hipStream_t* side, hipEvent_t* fork,
+                     hipEvent_t* join) {
+  if (d->n < SIDE_MIN_N) return WRT_OK;
   if (hipStreamCreateWithFlags(side, hipStreamNonBlocking) != hipSuccess) return WRT_E_HIP;
   if (hipEventCreateWithFlags(fork, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(join, hipEventDisableTiming) != hipSuccess)
@@ -355,7 +360,7 @@ static int radon_fast_create(wrt_job* j) {
     radon_fast_destroy(j);
     return WRT_E_NOMEM;
   }
-  if (side_init(&f->side, &f->fork, &f->join) != WRT_OK) {
+  if (side_init(d, &f->side, &f->fork, &f->join) != WRT_OK) {
     radon_fast_destroy(j);
     return WRT_E_HIP;
   }
@@ -481,19 +486,22 @@ static int radon_fast_enqueue(wrt_job* j) {
   a.rho2 = f->rho2;
   /* face bricks on the side stream, interior bricks on the job's stream, concurrently:
    * both add into the same FP64 accumulators with atomics */
-  HIPCHK(hipEventRecord(f->fork, st));
-  HIPCHK(hipStreamWaitEvent(f->side, f->fork, 0));
+  hipStream_t sb = f->side ? f->side : st;
+  if (f->side) {
+    HIPCHK(hipEventRecord(f->fork, st));
+    HIPCHK(hipStreamWaitEvent(f->side, f->fork, 0));
+  }
   wrt_radon_fast_args b = a;
   b.tasks = f->tasks + 2 * (size_t)f->n_interior;
   b.ntasks = f->n_boundary;
-  rc = wrtk_radon_fast(&b, 1, f->side);
+  rc = wrtk_radon_fast(&b, 1, sb);
   if (rc != WRT_OK) return rc;
-  HIPCHK(hipEventRecord(f->join, f->side));
+  if (f->side) HIPCHK(hipEventRecord(f->join, f->side));
   a.tasks = f->tasks;
   a.ntasks = f->n_interior;
   rc = wrtk_radon_fast(&a, 0, st);
   if (rc != WRT_OK) return rc;
-  HIPCHK(hipStreamWaitEvent(st, f->join, 0));
+  if (f->side) HIPCHK(hipStreamWaitEvent(st, f->join, 0));
   HIPCHK(hipEventRecord(j->e3, st));
   return wrtk_acc_to_f32(f->acc, f->out, nout, st);
 }
@@ -594,7 +602,7 @@ static int radon_T_fast_create(wrt_job* j) {
       hipMalloc((void**)&f->part, (size_t)f->ngroups * f->nbricks * WRT_BRICK * WRT_BRICK * WRT_BRICK * 8) != hipSuccess ||
       hipMalloc((void**)&f->out, (size_t)d->n * d->n * d->n * 4) != hipSuccess)
     return WRT_E_NOMEM;
-  if (side_init(&f->side, &f->fork, &f->join) != WRT_OK) return WRT_E_HIP;
+  if (side_init(d, &f->side, &f->fork, &f->join) != WRT_OK) return WRT_E_HIP;
   /* bricks nearest the centre first (planes with |s| <= 1 cut them from every direction, so
    * their tasks are the longest), interior bricks then bricks on the cube's faces */
   int* order = (int*)malloc(f->nbricks * sizeof(int));
@@ -697,19 +705,22 @@ static int radon_T_fast_enqueue(wrt_job* j) {
   HIPCHK(hipEventRecord(j->e1, st));
   /* face half bricks on the side stream, concurrently with the interior ones: every task
    * writes its own partial slot */
-  HIPCHK(hipEventRecord(f->fork, st));
-  HIPCHK(hipStreamWaitEvent(f->side, f->fork, 0));
+  hipStream_t sb = f->side ? f->side : st;
+  if (f->side) {
+    HIPCHK(hipEventRecord(f->fork, st));
+    HIPCHK(hipStreamWaitEvent(f->side, f->fork, 0));
+  }
   wrt_radon_fast_args b = a;
   b.tasks = f->tasks + 2 * (size_t)f->n_interior;
   b.ntasks = f->n_boundary;
-  int rc = wrtk_radon_T_fast(&b, 1, f->side);
+  int rc = wrtk_radon_T_fast(&b, 1, sb);
   if (rc != WRT_OK) return rc;
-  HIPCHK(hipEventRecord(f->join, f->side));
+  if (f->side) HIPCHK(hipEventRecord(f->join, f->side));
   a.tasks = f->tasks;
   a.ntasks = f->n_interior;
   rc = wrtk_radon_T_fast(&a, 0, st);
   if (rc != WRT_OK) return rc;
-  HIPCHK(hipStreamWaitEvent(st, f->join, 0));
+  if (f->side) HIPCHK(hipStreamWaitEvent(st, f->join, 0));
   HIPCHK(hipEventRecord(j->e3, st));
   const double delta = (1.0 / (d->n));
   return wrtk_radon_T_reduce(f->part, f->ngroups, d->n, f->nb, delta * delta / f->gscale,
