@@ -10,8 +10,9 @@ the FP64 dot-product test in test_gpu_parity.py):
   single flipped index decision is an error of at least a quarter of the largest value;
 - the dot-product test of the FP32 pairs, <A f, g> = <f, A^T g>, to 1e-6 relative to
   ||A f|| ||g||;
-- n not a multiple of the 32^3 brick, and the ray form at and above the LDS plane limit
-  (n = 143, FP64 plane);
+- n not a multiple of the 32^3 brick, n = 200 (the face half bricks then run on a second
+  stream, concurrently with the interior ones), and the ray form at and above the LDS plane
+  limit (n = 143, FP64 plane);
 - the plane form gives the same bits on every launch (integer sums), and stays exact on a
   shift grid whose shifts crowd into one cell's width (the fixed point's bound).
 """
@@ -30,7 +31,8 @@ def _check(got, ref, tol=1e-5):
         np.max(np.abs(got - ref)) / np.max(np.abs(ref))
 
 
-@pytest.mark.parametrize("n,nphi,nth", [(16, 12, 6), (40, 16, 8), (64, 32, 16), (96, 24, 12)])
+@pytest.mark.parametrize("n,nphi,nth", [(16, 12, 6), (40, 16, 8), (64, 32, 16), (96, 24, 12),
+                                         (200, 8, 4)])
 def test_radon_T_f32_matches_plain(ctx, n, nphi, nth):
     phi, th, w, s = ctx.radon_grid_init(nphi, nth, n + 1)
     g = wrt.gaussian((len(s), nphi, nth), seed=n)
@@ -41,7 +43,7 @@ def test_radon_T_f32_matches_plain(ctx, n, nphi, nth):
     assert st["logical_samples"] == 16.0 * n * n * g.size
 
 
-@pytest.mark.parametrize("n", [32, 40, 64])
+@pytest.mark.parametrize("n", [32, 40, 64, 200])
 def test_radon_T_f32_one_hot_structured(ctx, n):
     phi = np.array([0.0, np.pi / 4, np.pi / 2, np.pi, 1.5 * np.pi, 0.3])
     _, th_gl, _, _ = ctx.radon_grid_init(4, 6, 3)
