@@ -744,8 +744,8 @@ def main():
         import tempfile
 
         log("bench: drop-in program end to end")
-        with tempfile.TemporaryDirectory(dir=os.path.join(ROOT, "gpurun_out") if os.path.isdir(
-                os.path.join(ROOT, "gpurun_out")) else None) as tmp:
+        # beside the repository (the program's text output is about 340 MB), removed after
+        with tempfile.TemporaryDirectory(dir=ROOT, prefix=".bench_programs_") as tmp:
             programs = cli_leg(tmp)
         log(f"bench: programs {programs}")
 
