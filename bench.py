@@ -609,7 +609,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
-              "(every box here has one GPU)", file=sys.stderr)
+              "(the path is built and measured on a single GPU; multi-GPU sharding is a "
+              "design only)", file=sys.stderr)
         sys.exit(2)
 
     if args.traffic_probe:
