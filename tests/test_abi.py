@@ -79,6 +79,26 @@ def test_job_sizes():
     assert wrt.job_bytes("adjoint", "f32", n=8, phi=phi, theta=th, theta_w=w, shift=s2) == 0
 
 
+def test_shift_order_contract():
+    # plane and line integrals take shifts in any order (plane_integral and ray_transform
+    # take any s); the reduction and the backprojection spline over the shift grid, which
+    # GSL's gsl_interp_init rejects unless strictly increasing
+    phi, th, w = np.zeros(4), np.zeros(3), np.ones(3)
+    s = np.linspace(-1, 1, 9)
+    rng = np.random.default_rng(0)
+    for bad in (s[::-1], rng.permutation(s), np.r_[s[:4], s[3:]]):
+        for prec in ("f64", "f32", "bf16"):
+            assert wrt.job_bytes("radon", prec, n=8, phi=phi, theta=th, shift=bad) > 0
+        assert wrt.job_bytes("radon_T", "f32", n=8, phi=phi, theta=th, shift=bad) > 0
+        assert wrt.job_bytes("ray", "f32", n=8, phi=phi, z=s, shift=bad) > 0
+        for prec in ("f64", "f32"):
+            assert wrt.job_bytes("adjoint", prec, n=8, phi=phi, theta=th, theta_w=w,
+                                 shift=bad) == 0
+        assert wrt.job_bytes("reduction", "f64", phi=phi, theta=th, shift=bad, nz=8) == 0
+    assert wrt.job_bytes("adjoint", "f64", n=8, phi=phi, theta=th, theta_w=w, shift=s) > 0
+    assert wrt.job_bytes("reduction", "f64", phi=phi, theta=th, shift=s, nz=8) > 0
+
+
 def test_init_without_device_reports_nodev():
     h = C.c_void_p()
     rc = wrt.lib().wrt_init(C.byref(h), 0)

@@ -15,7 +15,7 @@ import wrt
 pytestmark = pytest.mark.gpu
 
 
-def _jobs(ctx):
+def _jobs(ctx, seed=0):
     jobs = []
     for k, (op, prec, e) in enumerate([("radon", "f32", 32), ("radon", "f32", 48),
                                         ("radon", "f32", 64), ("radon", "bf16", 64),
@@ -24,10 +24,10 @@ def _jobs(ctx):
         phi, th, w, s = ctx.radon_grid_init(e // 2, e // 4, e + 1)
         if op == "radon":
             j = ctx.job("radon", prec, n=e, phi=phi, theta=th, shift=s)
-            j.load(wrt.phantom(e, True, 10, 100 + k, 0.01))
+            j.load(wrt.phantom(e, True, 10, 100 + k + seed, 0.01))
         else:
             j = ctx.job("adjoint", prec, n=e, phi=phi, theta=th, theta_w=w, shift=s)
-            j.load(wrt.gaussian((len(s), len(phi), len(th)), 200 + k))
+            j.load(wrt.gaussian((len(s), len(phi), len(th)), 200 + k + seed))
         jobs.append((op, prec, j))
     return jobs
 
@@ -73,6 +73,23 @@ def test_batch_equals_alone_and_replays(ctx, tmp_path):
         _check(jobs, want)
         for _, _, j in jobs:
             j.close()
+    finally:
+        del os.environ["WRT_TUNE_CACHE"]
+
+
+def test_batch_after_jobs_closed_and_recreated(ctx, tmp_path):
+    # A plan lists its jobs by address and load sequence; jobs closed and recreated with the
+    # same count (a new job may reuse a freed job's address) and loaded once must get a new
+    # plan, not a replay of kernels that write into the freed buffers.
+    os.environ["WRT_TUNE_CACHE"] = str(tmp_path / "tune.txt")
+    try:
+        for rnd in range(3):
+            jobs = _jobs(ctx, seed=1000 * rnd)  # new data, each job loaded once
+            want = _alone(jobs)
+            ctx.batch_run([j for _, _, j in jobs])
+            _check(jobs, want)
+            for _, _, j in jobs:
+                j.close()
     finally:
         del os.environ["WRT_TUNE_CACHE"]
 

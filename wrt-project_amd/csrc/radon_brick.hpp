@@ -302,7 +302,7 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
     const int R0 = max(L0, (int)floor(pa0 / aa) - 2), R1 = min(L1, (int)ceil(pa1 / aa) + 2);
     const int C0 = max(L0, (int)floor(pb0 / bb) - 2), C1 = min(L1, (int)ceil(pb1 / bb) + 2);
     if (R0 > R1 || C0 > C1) continue;
-    // shifts with s in [pn0 - eps, pn1 + eps] (sorted shift grid, binary search)
+    // shifts with s in [pn0 - eps, pn1 + eps] (the ascending copy of the grid, bisected)
     const double eps = 4.0 * delta;
     int j0, j1;
     {
@@ -379,9 +379,10 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
       D8[c] = 8ll * Dfx[c];
     }
     for (int j = j0; j <= j1; ++j) {
+      const int jo = a.sidx[j];  // the plane's row in the caller's shift order
       unsigned long long gq = 0ull;  // transpose: the section's quantized value
       if constexpr (T) {
-        const float gv = a.g[(size_t)j * a.ndir + d];
+        const float gv = a.g[(size_t)jo * a.ndir + d];
         if (gv == 0.f) continue;
         gq = (unsigned long long)(long long)rint((double)gv * a.gscale);
         if (gq == 0ull) continue;
@@ -637,7 +638,7 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
       if constexpr (!T) {
         const double tot = wave_sum_dpp(ks);
         if (lane == 0 && tot != 0.0)
-          unsafeAtomicAdd(a.acc + (size_t)j * a.ndir + d, tot * delta * delta);
+          unsafeAtomicAdd(a.acc + (size_t)jo * a.ndir + d, tot * delta * delta);
       }
     }
   }

@@ -28,14 +28,16 @@ typedef struct {
   const double* dirs; /* [ndir][9] n, a = delta*v1, b = delta*v2              */
   int ndir;
   int dgroup;         /* directions per group                                 */
-  const double* shift;
+  const double* shift; /* ascending (the kernel bisects it for a brick's shifts) */
+  const int* sidx;    /* [nshift] output row of the j-th smallest shift: the
+                         caller's grid may be in any order                    */
   int nshift;
   double* acc;        /* [nshift][ndir] FP64 accumulators, zeroed by caller   */
   double rho2;        /* squared support radius: every sample farther from the
                          origin reads a zero cell (cell-centre radius of the
                          farthest non-zero cell plus half a cell diagonal)     */
   /* transpose only */
-  const float* g;     /* [nshift][ndir] plane values                          */
+  const float* g;     /* [nshift][ndir] plane values (rows as acc's)          */
   double gscale;      /* fixed-point scale: a section adds rint(g * gscale)   */
   long long* part;    /* [ngroups][nb^3][WRT_BRICK^3] per-group brick partials,
                          fixed point                                          */

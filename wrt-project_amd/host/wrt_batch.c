@@ -69,6 +69,17 @@ void wrt_batch_release(wrt_ctx* ctx) {
   ctx->batch = NULL;
 }
 
+void wrt_batch_forget(wrt_ctx* ctx, const wrt_job* j) {
+  const batch_plan* p = ctx ? (const batch_plan*)ctx->batch : NULL;
+  if (!p) return;
+  for (int i = 0; i < p->njobs; ++i)
+    if (p->jobs[i] == j) {
+      hipStreamSynchronize(ctx->stream); /* a replay still running reads the job's buffers */
+      wrt_batch_release(ctx);
+      return;
+    }
+}
+
 /* tasks address voxels with 32-bit offsets: larger backprojections run beside the graph */
 static int batchable(const wrt_job* j) {
   return j->d.prec != WRT_F64 && wrt_fast_supported(&j->d) &&

@@ -47,6 +47,7 @@ typedef struct {
   double* acc;      /* [nshift][ndir] */
   float* out;       /* [nshift][ndir] */
   double* staging;  /* n^3 FP64 input */
+  void* sorted;     /* ascending shifts, then their rows (sorted_shifts) */
   /* host copy of the non-empty bricks, heaviest first (batch planning) */
   int *ne_brick, *ne_count, n_ne;
   unsigned char* ne_bnd;
@@ -288,6 +289,54 @@ static int fetch_f32(wrt_job* j, const float* src, size_t count, double* out) {
 
 static int nb_of(int n) { return (n + WRT_BRICK - 1) / WRT_BRICK; }
 
+/* The brick kernel bisects the shift grid for the shifts that cut a brick, so it reads an
+ * ascending copy together with each position's row in the caller's grid: plane_integral
+ * (radon_grid/src/integral.c:26) takes any s, so a grid may come in any order and repeat
+ * values.  dev receives nshift doubles (ascending) then nshift ints (rows); hs, when not
+ * NULL, the ascending values. */
+typedef struct {
+  double v;
+  int i;
+} shift_key;
+
+static int by_shift(const void* a, const void* b) {
+  const shift_key *x = (const shift_key*)a, *y = (const shift_key*)b;
+  if (x->v != y->v) return x->v < y->v ? -1 : 1;
+  return (x->i > y->i) - (x->i < y->i);
+}
+
+static size_t sorted_shift_bytes(int nshift) { return (size_t)nshift * (sizeof(double) + sizeof(int)); }
+
+static int sorted_shifts(const wrt_job_desc* d, void* dev, double* hs, hipStream_t st) {
+  const int ns = d->nshift;
+  shift_key* k = (shift_key*)malloc(sizeof(shift_key) * (size_t)ns);
+  char* h = (char*)malloc(sorted_shift_bytes(ns));
+  if (!k || !h) {
+    free(k);
+    free(h);
+    return WRT_E_NOMEM;
+  }
+  for (int j = 0; j < ns; ++j) {
+    k[j].v = d->shift[j];
+    k[j].i = j;
+  }
+  qsort(k, (size_t)ns, sizeof(shift_key), by_shift);
+  double* v = (double*)h;
+  int* row = (int*)(h + (size_t)ns * sizeof(double));
+  for (int j = 0; j < ns; ++j) {
+    v[j] = k[j].v;
+    row[j] = k[j].i;
+    if (hs) hs[j] = k[j].v;
+  }
+  int rc = WRT_OK;
+  if (hipMemcpyAsync(dev, h, sorted_shift_bytes(ns), hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    rc = WRT_E_HIP;
+  free(k);
+  free(h);
+  return rc;
+}
+
 static size_t radon_fast_bytes(const wrt_job_desc* d) {
   const size_t nb = (size_t)nb_of(d->n);
   const size_t nbr = nb * nb * nb;
@@ -295,7 +344,8 @@ static size_t radon_fast_bytes(const wrt_job_desc* d) {
   const size_t ngroups = (ndir + 7) / 8;
   const size_t eb = d->prec == WRT_BF16 ? 2 : 4;
   return nbr * WRT_BRICK * WRT_BRICK * WRT_BRICK * eb + nbr * 4 + 2 * nbr * ngroups * 4 +
-         (size_t)d->nshift * ndir * 12 + (size_t)d->n * d->n * d->n * 8;
+         (size_t)d->nshift * ndir * 12 + (size_t)d->n * d->n * d->n * 8 +
+         sorted_shift_bytes(d->nshift);
 }
 
 /* A side stream (default priority) with the events that fork it from and join it back into
@@ -333,6 +383,7 @@ static void radon_fast_destroy(wrt_job* j) {
   hipFree(f->acc);
   hipFree(f->out);
   hipFree(f->staging);
+  hipFree(f->sorted);
   free(f->ne_brick);
   free(f->ne_count);
   free(f->ne_bnd);
@@ -356,7 +407,8 @@ static int radon_fast_create(wrt_job* j) {
       hipMalloc((void**)&f->tasks, 2 * f->nbricks * ngroups * sizeof(int)) != hipSuccess ||
       hipMalloc((void**)&f->acc, nout * sizeof(double)) != hipSuccess ||
       hipMalloc((void**)&f->out, nout * sizeof(float)) != hipSuccess ||
-      hipMalloc((void**)&f->staging, (size_t)d->n * d->n * d->n * sizeof(double)) != hipSuccess) {
+      hipMalloc((void**)&f->staging, (size_t)d->n * d->n * d->n * sizeof(double)) != hipSuccess ||
+      hipMalloc(&f->sorted, sorted_shift_bytes(d->nshift)) != hipSuccess) {
     radon_fast_destroy(j);
     return WRT_E_NOMEM;
   }
@@ -364,7 +416,9 @@ static int radon_fast_create(wrt_job* j) {
     radon_fast_destroy(j);
     return WRT_E_HIP;
   }
-  return WRT_OK;
+  const int rc = sorted_shifts(d, f->sorted, NULL, j->st);
+  if (rc != WRT_OK) radon_fast_destroy(j);
+  return rc;
 }
 
 static int radon_fast_load(wrt_job* j, const double* in) {
@@ -488,7 +542,8 @@ static int radon_fast_enqueue(wrt_job* j) {
   a.dirs = j->d_dirs;
   a.ndir = (int)ndir;
   a.dgroup = f->dgroup;
-  a.shift = j->d_shift;
+  a.shift = (const double*)f->sorted;
+  a.sidx = (const int*)((const char*)f->sorted + (size_t)d->nshift * sizeof(double));
   a.nshift = d->nshift;
   a.acc = f->acc;
   a.rho2 = f->rho2;
@@ -523,6 +578,7 @@ typedef struct {
   int qbits;       /* fixed point: |rint(g * gscale)| <= 2^qbits */
   double gscale;
   int* tasks;      /* device, interior then boundary */
+  void* sorted;    /* ascending shifts, then their rows (sorted_shifts) */
   double* staging; /* FP64 plane values [nshift][ndir] */
   float* g32;      /* FP32 plane values */
   long long* part; /* [ngroups][nbricks][WRT_BRICK^3] fixed-point partials */
@@ -540,12 +596,12 @@ typedef struct {
  * normal, an interval of width at most sqrt(3) * 2/n; w counts the most shifts any such
  * window holds.  So no cell receives more than 16 w ndir samples, and with
  * |q| <= 2^qbits no total, over every group, exceeds 2^62. */
-static int radon_T_qbits(const wrt_job_desc* d) {
+static int radon_T_qbits(const wrt_job_desc* d, const double* sorted) {
   const double width = sqrt(3.0) * 2.0 / d->n * (1.0 + 1e-9) + 1e-12;
   int w = 1;
-  for (int lo = 0, hi = 0; lo < d->nshift; ++lo) { /* shifts ascending */
+  for (int lo = 0, hi = 0; lo < d->nshift; ++lo) { /* the shifts in ascending order */
     if (hi < lo) hi = lo;
-    while (hi + 1 < d->nshift && d->shift[hi + 1] - d->shift[lo] <= width) ++hi;
+    while (hi + 1 < d->nshift && sorted[hi + 1] - sorted[lo] <= width) ++hi;
     if (hi - lo + 1 > w) w = hi - lo + 1;
   }
   const double samples = 16.0 * w * ((double)d->nphi * d->ntheta);
@@ -575,7 +631,8 @@ static size_t radon_T_fast_bytes(const wrt_job_desc* d) {
   radon_T_groups(d, nbr, &ng, &dg);
   const size_t nplanes = (size_t)d->nshift * d->nphi * d->ntheta;
   return nplanes * 12 + (size_t)ng * nbr * WRT_BRICK * WRT_BRICK * WRT_BRICK * 8 +
-         4 * nbr * (size_t)ng * sizeof(int) + (size_t)d->n * d->n * d->n * 4;
+         4 * nbr * (size_t)ng * sizeof(int) + (size_t)d->n * d->n * d->n * 4 +
+         sorted_shift_bytes(d->nshift);
 }
 
 static void radon_T_fast_destroy(wrt_job* j) {
@@ -583,6 +640,7 @@ static void radon_T_fast_destroy(wrt_job* j) {
   if (!f) return;
   side_free(f->side, f->fork, f->join);
   hipFree(f->tasks);
+  hipFree(f->sorted);
   hipFree(f->staging);
   hipFree(f->g32);
   hipFree(f->part);
@@ -602,7 +660,15 @@ static int radon_T_fast_create(wrt_job* j) {
   const size_t nplanes = (size_t)d->nshift * d->nphi * d->ntheta;
   const size_t ntasks = 2 * f->nbricks * (size_t)f->ngroups;
   if (ntasks > 0x7fffffff) return WRT_E_ARG;
-  f->qbits = radon_T_qbits(d);
+  if (hipMalloc(&f->sorted, sorted_shift_bytes(d->nshift)) != hipSuccess) return WRT_E_NOMEM;
+  {
+    double* hs = (double*)malloc(sizeof(double) * (size_t)d->nshift);
+    if (!hs) return WRT_E_NOMEM;
+    const int rc = sorted_shifts(d, f->sorted, hs, j->st);
+    if (rc == WRT_OK) f->qbits = radon_T_qbits(d, hs);
+    free(hs);
+    if (rc != WRT_OK) return rc;
+  }
   if (f->qbits < 24) return WRT_E_ARG; /* more samples per cell than 64 bits can hold */
   if (hipMalloc((void**)&f->tasks, 2 * ntasks * sizeof(int)) != hipSuccess ||
       hipMalloc((void**)&f->staging, nplanes * 8) != hipSuccess ||
@@ -704,7 +770,8 @@ static int radon_T_fast_enqueue(wrt_job* j) {
   a.dirs = j->d_dirs;
   a.ndir = d->nphi * d->ntheta;
   a.dgroup = f->dgroup;
-  a.shift = j->d_shift;
+  a.shift = (const double*)f->sorted;
+  a.sidx = (const int*)((const char*)f->sorted + (size_t)d->nshift * sizeof(double));
   a.nshift = d->nshift;
   a.rho2 = 4.0; /* no support clipping: every cell of the cube receives */
   a.g = f->g32;
@@ -894,7 +961,8 @@ int wrt_fast_describe(wrt_job* j, wrt_fast_desc* fd) {
     it->ra.bf16 = d->prec == WRT_BF16;
     it->ra.dirs = j->d_dirs;
     it->ra.ndir = d->nphi * d->ntheta;
-    it->ra.shift = j->d_shift;
+    it->ra.shift = (const double*)f->sorted;
+    it->ra.sidx = (const int*)((const char*)f->sorted + (size_t)d->nshift * sizeof(double));
     it->ra.nshift = d->nshift;
     it->ra.acc = f->acc;
     it->ra.rho2 = f->rho2;

@@ -19,10 +19,15 @@ struct wrt_ctx {
   int cu_count;
   /* batch graph cache */
   void* batch;
+  /* loads so far on this context: a job's gen is the sequence number of its last load, so
+   * a job created at a freed job's address never matches a plan made for the old one */
+  unsigned long long load_seq;
   /* spectral Laplacian: FFT plans of the last size (host/wrt_spectral.c) */
   void* spectral;
 };
 void wrt_spectral_release(wrt_ctx* c);
+/* drop the context's batch plan if it lists job j (host/wrt_batch.c) */
+void wrt_batch_forget(wrt_ctx* c, const struct wrt_job* j);
 
 struct wrt_job {
   wrt_ctx* ctx;
@@ -51,7 +56,7 @@ struct wrt_job {
   size_t work_bytes;
   void* fast;      /* fast-path plan (wrt_fast.h) */
   int loaded;
-  unsigned long long gen; /* incremented by every load (batch plans key on it) */
+  unsigned long long gen; /* ctx->load_seq at the last load (batch plans key on it) */
   size_t bytes;
 };
 

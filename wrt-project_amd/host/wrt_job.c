@@ -29,6 +29,17 @@ int wrt_fast_enqueue(wrt_job* j);
 int wrt_fast_fetch(wrt_job* j, double* out);
 void wrt_fast_destroy(wrt_job* j);
 
+/* The reduction and the backprojection spline their data over the shift grid, which GSL's
+ * gsl_interp_init (called through gsl_spline_init at radon_reduction/slicing.c:45-64 and
+ * radon_inversion/backprojection/C/src/interp.c:17-25) rejects unless strictly increasing;
+ * the plane and line integrals accept shifts in any order, as plane_integral and
+ * ray_transform take any s. */
+static int increasing(const double* s, int n) {
+  for (int i = 1; i < n; ++i)
+    if (!(s[i - 1] < s[i])) return 0;
+  return 1;
+}
+
 static int desc_ok(const wrt_job_desc* d) {
   if (!d) return 0;
   if (d->prec != WRT_F64 && d->prec != WRT_F32 && d->prec != WRT_BF16) return 0;
@@ -46,12 +57,12 @@ static int desc_ok(const wrt_job_desc* d) {
     case WRT_OP_REDUCTION:
       return d->prec == WRT_F64 && d->nz >= 3 && wrt_check_grid(d->phi, d->nphi, 1) == WRT_OK &&
              wrt_check_grid(d->theta, d->ntheta, 1) == WRT_OK &&
-             wrt_check_grid(d->shift, d->nshift, 3) == WRT_OK;
+             wrt_check_grid(d->shift, d->nshift, 3) == WRT_OK && increasing(d->shift, d->nshift);
     case WRT_OP_ADJOINT:
       return d->n >= 1 && d->n <= 4096 && wrt_check_grid(d->phi, d->nphi, 1) == WRT_OK &&
              wrt_check_grid(d->theta, d->ntheta, 1) == WRT_OK &&
              wrt_check_grid(d->theta_w, d->ntheta, 1) == WRT_OK &&
-             wrt_check_grid(d->shift, d->nshift, 3) == WRT_OK;
+             wrt_check_grid(d->shift, d->nshift, 3) == WRT_OK && increasing(d->shift, d->nshift);
     default: return 0;
   }
 }
@@ -117,6 +128,8 @@ void wrt_job_destroy(wrt_job* j) {
   if (!j) return;
   hipSetDevice(j->ctx->device);
   if (j->st) hipStreamSynchronize(j->st);
+  /* a captured batch graph that names this job's buffers must never be replayed */
+  wrt_batch_forget(j->ctx, j);
   if (j->fast) wrt_fast_destroy(j);
   hipFree(j->d_in);
   hipFree(j->d_out);
@@ -255,7 +268,7 @@ int wrt_job_load(wrt_job* j, const double* in) {
   }
   if (rc == WRT_OK) {
     j->loaded = 1;
-    j->gen++;
+    j->gen = ++j->ctx->load_seq;
   }
   return rc;
 }
