@@ -40,6 +40,8 @@ extern "C" {
 #define WRT_E_ADMIT (-5)   /* serve loop: the job does not fit the free-HBM budget    */
 #define WRT_E_IO (-6)      /* file open / parse error (config, volume, data files)    */
 #define WRT_E_STATE (-7)   /* call out of order (e.g. run before load)                */
+#define WRT_E_SCRATCH (-8) /* wrt_radon_grid_noalloc: the call needs more device memory
+                              than the library's fixed scratch holds                  */
 
 typedef enum { WRT_F64 = 0, WRT_F32 = 1, WRT_BF16 = 2 } wrt_prec;
 
@@ -99,6 +101,23 @@ int wrt_zslice_grid(int nslices, double* z);
 int wrt_radon_grid(wrt_ctx* ctx, const double* vol, int n, const double* phi, int nphi,
                    const double* theta, int ntheta, const double* shift, int nshift,
                    wrt_prec prec, double* out, wrt_stats* st);
+
+/* wrt_radon_grid in its FP32 form, allocating nothing: no device allocation (hipMalloc,
+ * hipFree or any library routine that allocates) and no host heap.  No counterpart in the
+ * reference: it is for callers that cannot allocate, such as tight serving loops and
+ * harnesses that forbid allocation.  Every device buffer is carved from the library's fixed
+ * scratch of 64 MiB per GPU, held from wrt_init to the end of the process; host tables use
+ * static storage.  Limits: nphi * ntheta <= 65536, nshift <= 8192, and the call's bricks,
+ * one 32-plane FP64 slab of the volume, its tables and FP64 plane sums within the scratch:
+ * about 6.7e6 planes at n = 128, 3.5e6 at n = 192, and no n above 224 (the 32^3 FP32 bricks
+ * of a 256^3 volume alone fill it); beyond them it returns WRT_E_SCRATCH before any device
+ * work and writes nothing.  Same arguments, output
+ * layout and values as wrt_radon_grid(..., WRT_F32, ...) up to the summation order of the
+ * FP64 plane accumulators.  Synchronous; calls that overlap on one GPU, or from several host
+ * threads, share the scratch and race. */
+int wrt_radon_grid_noalloc(wrt_ctx* ctx, const double* vol, int n, const double* phi,
+                           int nphi, const double* theta, int ntheta, const double* shift,
+                           int nshift, double* out, wrt_stats* st);
 
 /* Line integrals in z-slices: replaces ray_transform (ray_grid/src/integral.c:11-50)
  * called from ray_grid/src/ray_grid.c:242-255.  out[z][s][phi]. */

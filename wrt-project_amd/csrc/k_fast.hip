@@ -107,20 +107,24 @@ __device__ __forceinline__ unsigned short f32_to_bf16_rne(float f) {
   return (unsigned short)(u >> 16);
 }
 
+// bricks of the rows bx in [bx0, bx0 + nbx); vol holds the planes x >= 32 bx0
 __global__ __launch_bounds__(256) void k_brickify(const double* __restrict__ vol, int n, int nb,
-                                                  int bf16, void* __restrict__ bricks,
+                                                  int bx0, int nbx, int bf16,
+                                                  void* __restrict__ bricks,
                                                   int* __restrict__ flags,
                                                   unsigned long long* __restrict__ r2max) {
-  const size_t total = (size_t)nb * nb * nb * (FB * FB * FB);
+  const size_t first = (size_t)bx0 * nb * nb * (FB * FB * FB);
+  const size_t total = first + (size_t)nbx * nb * nb * (FB * FB * FB);
+  const int x0 = bx0 * FB;
   double r2 = 0.0;
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+  for (size_t i = first + (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
        i += (size_t)gridDim.x * blockDim.x) {
     const size_t brick = i / (FB * FB * FB);
     const int li = (int)(i % (FB * FB * FB));
     const int bx = (int)(brick / ((size_t)nb * nb)), by = (int)((brick / nb) % nb), bz = (int)(brick % nb);
     const int x = bx * FB + (li >> 10), y = by * FB + ((li >> 5) & 31), z = bz * FB + (li & 31);
     double v = 0.0;
-    if (x < n && y < n && z < n) v = vol[((size_t)x * n + y) * n + z];
+    if (x < n && y < n && z < n) v = vol[((size_t)(x - x0) * n + y) * n + z];
     const float f = (float)v;
     if (bf16) ((unsigned short*)bricks)[i] = f32_to_bf16_rne(f);
     else ((float*)bricks)[i] = f;
@@ -142,12 +146,13 @@ __global__ __launch_bounds__(256) void k_brickify(const double* __restrict__ vol
   if ((threadIdx.x & 63) == 0 && b) atomicMax(r2max, b);
 }
 
-extern "C" int wrtk_brickify(const double* vol, int n, int nb, int bf16, void* bricks, int* flags,
-                             unsigned long long* r2max, hipStream_t st) {
-  const size_t total = (size_t)nb * nb * nb * (FB * FB * FB);
+extern "C" int wrtk_brickify(const double* vol, int n, int nb, int bx0, int nbx, int bf16,
+                             void* bricks, int* flags, unsigned long long* r2max, hipStream_t st) {
+  if (bx0 < 0 || nbx < 1 || bx0 + nbx > nb) return WRT_E_ARG;
+  const size_t total = (size_t)nbx * nb * nb * (FB * FB * FB);
   size_t blocks = (total + 255) / 256;
   if (blocks > 65536) blocks = 65536;
-  k_brickify<<<(unsigned)blocks, 256, 0, st>>>(vol, n, nb, bf16, bricks, flags, r2max);
+  k_brickify<<<(unsigned)blocks, 256, 0, st>>>(vol, n, nb, bx0, nbx, bf16, bricks, flags, r2max);
   WRT_LAUNCH_CHECK();
   return WRT_OK;
 }

@@ -43,11 +43,13 @@ typedef struct {
                          fixed point                                          */
 } wrt_radon_fast_args;
 
-/* FP64 vol[x][y][z] -> bricks; flags[brick] += its number of non-zero values (flags zeroed by the caller);
- * r2max = largest squared cell-centre radius of a non-zero cell (as the bit pattern of a
- * non-negative double, so unsigned max orders it), zeroed by the caller */
-int wrtk_brickify(const double* vol, int n, int nb, int bf16, void* bricks, int* flags,
-                  unsigned long long* r2max, hipStream_t st);
+/* FP64 vol[x][y][z] -> the bricks of the brick rows bx in [bx0, bx0 + nbx) (vol holds the
+ * planes x >= 32 bx0: the whole volume for bx0 = 0, nbx = nb); flags[brick] += its number of
+ * non-zero values (flags zeroed by the caller); r2max = largest squared cell-centre radius
+ * of a non-zero cell (as the bit pattern of a non-negative double, so unsigned max orders
+ * it), zeroed by the caller */
+int wrtk_brickify(const double* vol, int n, int nb, int bx0, int nbx, int bf16, void* bricks,
+                  int* flags, unsigned long long* r2max, hipStream_t st);
 /* boundary = 1: the tasks' bricks touch the cube's faces (cube test and edge clamp) */
 int wrtk_radon_fast(const wrt_radon_fast_args* a, int boundary, hipStream_t st);
 /* out[i] = (float)acc[i] */
@@ -62,6 +64,22 @@ int wrtk_radon_T_fast(const wrt_radon_fast_args* a, int boundary, hipStream_t st
  * sum is exact, one rounding to FP64 and one to FP32 */
 int wrtk_radon_T_reduce(const long long* part, int ngroups, int n, int nb, double scale,
                         float* out, hipStream_t st);
+
+/* ---- the no-allocation plane integrals (k_noalloc.hip, host/wrt_noalloc.c) ---- */
+#define WRT_NOALLOC_BYTES ((size_t)64 << 20) /* the fixed device scratch per GPU      */
+#define WRT_NOALLOC_MAX_BRICKS 512           /* n <= 256 (the scratch binds first)   */
+/* once per device, with the device current: the scratch's device address */
+int wrtk_noalloc_setup(int device);
+/* that address (NULL before setup) */
+void* wrtk_noalloc_scratch(int device);
+/* zero `words` 8-byte words */
+int wrtk_noalloc_zero(void* p, size_t words, hipStream_t st);
+/* the brick kernel's task list from the bricks' non-zero counts: interior tasks, then face
+ * tasks; counts[0], counts[1] = their numbers */
+int wrtk_noalloc_tasks(const int* flags, int n, int nb, int ngroups, int* tasks, int* counts,
+                       hipStream_t st);
+/* acc[i] = (double)(float)acc[i] */
+int wrtk_noalloc_round(double* acc, size_t count, hipStream_t st);
 
 /* ---- the persistent batch launch: many jobs' tasks in one grid ---- */
 enum { WRT_ITEM_RADON = 0, WRT_ITEM_ADJOINT = 1 };

@@ -37,6 +37,7 @@ const char* wrt_strerror(int code) {
     case WRT_E_ADMIT: return "job does not fit the free-HBM budget";
     case WRT_E_IO: return "file open or parse error";
     case WRT_E_STATE: return "call out of order";
+    case WRT_E_SCRATCH: return "the call needs more than the fixed no-allocation scratch";
     default: return "unknown error";
   }
 }
@@ -62,6 +63,16 @@ int wrt_init(wrt_ctx** out, int device) {
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
     cus = 256;
   c->cu_count = cus;
+  /* the device address of this GPU's copy of the no-allocation scratch */
+  if (wrtk_noalloc_setup(device) != WRT_OK) {
+    hipEventDestroy(c->ev0);
+    hipEventDestroy(c->ev1);
+    hipEventDestroy(c->ev2);
+    hipEventDestroy(c->ev3);
+    hipStreamDestroy(c->stream);
+    free(c);
+    return WRT_E_HIP;
+  }
   *out = c;
   return WRT_OK;
 }

@@ -431,7 +431,8 @@ static int radon_fast_load(wrt_job* j, const double* in) {
   const size_t r2off = (f->nbricks * sizeof(int) + 7) & ~(size_t)7;
   unsigned long long* d_r2 = (unsigned long long*)((char*)f->flags + r2off);
   HIPCHK(hipMemsetAsync(f->flags, 0, r2off + 8, st));
-  int rc = wrtk_brickify(f->staging, d->n, f->nb, d->prec == WRT_BF16, f->bricks, f->flags, d_r2, st);
+  int rc = wrtk_brickify(f->staging, d->n, f->nb, 0, f->nb, d->prec == WRT_BF16, f->bricks,
+                         f->flags, d_r2, st);
   if (rc != WRT_OK) return rc;
   int* flags = (int*)malloc(r2off + 8);
   if (!flags) return WRT_E_NOMEM;

@@ -14,8 +14,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libwrt.so")
 
-WRT_OK, WRT_E_ARG, WRT_E_NODEV, WRT_E_HIP, WRT_E_NOMEM, WRT_E_ADMIT, WRT_E_IO, WRT_E_STATE = (
-    0, -1, -2, -3, -4, -5, -6, -7)
+(WRT_OK, WRT_E_ARG, WRT_E_NODEV, WRT_E_HIP, WRT_E_NOMEM, WRT_E_ADMIT, WRT_E_IO, WRT_E_STATE,
+ WRT_E_SCRATCH) = (0, -1, -2, -3, -4, -5, -6, -7, -8)
 PREC = {"f64": 0, "f32": 1, "bf16": 2}
 OP = {"radon": 0, "ray": 1, "reduction": 2, "adjoint": 3, "radon_T": 4, "ray_T": 5}
 
@@ -84,7 +84,7 @@ ABI_SYMBOLS = [
     "wrt_fn_example", "wrt_fn_phantom", "wrt_analytic_zslice_grid", "wrt_radon_analytic",
     "wrt_ray_analytic", "wrt_laplace3d", "wrt_laplace2d", "wrt_read_text",
     "wrt_write_text_grid", "wrt_write_text_values", "wrt_write_f64", "wrt_read_f64",
-    "wrt_fourier_inversion", "wrt_fourier_inversion2d",
+    "wrt_fourier_inversion", "wrt_fourier_inversion2d", "wrt_radon_grid_noalloc",
 ]
 
 FN = {"exp_bump": 0, "sine_bump": 1, "ellipsoid": 2}
@@ -183,6 +183,7 @@ def lib():
         L.wrt_ray_grid_init.argtypes = [I, I, P, P]
         L.wrt_zslice_grid.argtypes = [I, P]
         L.wrt_radon_grid.argtypes = [P, P, I, P, I, P, I, P, I, I, P, P]
+        L.wrt_radon_grid_noalloc.argtypes = [P, P, I, P, I, P, I, P, I, P, P]
         L.wrt_ray_grid.argtypes = [P, P, I, P, I, P, I, P, I, I, P, P]
         L.wrt_radon_reduction.argtypes = [P, P, I, P, I, P, I, P, I, P, P]
         L.wrt_radon_reduction_rays.argtypes = [P, P, I, P, I, I, I, P, I, P, I, P, I, P, P]
@@ -327,6 +328,17 @@ class Context:
         _chk(lib().wrt_radon_grid(self.h, _p(vol), n, _p(phi), len(phi), _p(theta), len(theta),
                                   _p(shift), len(shift), PREC[prec], _p(out), C.byref(st)),
              "wrt_radon_grid")
+        return out, st.as_dict()
+
+    def radon_grid_noalloc(self, vol, phi, theta, shift):
+        """wrt_radon_grid's FP32 form through the entry point that allocates nothing"""
+        vol, phi, theta, shift = map(_f64, (vol, phi, theta, shift))
+        n = vol.shape[0]
+        out = np.empty((len(shift), len(phi), len(theta)))
+        st = Stats()
+        _chk(lib().wrt_radon_grid_noalloc(self.h, _p(vol), n, _p(phi), len(phi), _p(theta),
+                                          len(theta), _p(shift), len(shift), _p(out),
+                                          C.byref(st)), "wrt_radon_grid_noalloc")
         return out, st.as_dict()
 
     def ray_grid(self, vol, z, shift, phi, prec="f64"):
