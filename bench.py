@@ -4,8 +4,9 @@ Workload (the metric's configuration): a 256^3 seeded ellipsoid phantom with N(0
 texture, FP32 bricks, Gauss-uniform plane grid 256 phi x 128 Gauss-Legendre theta x 257
 shifts -- 8,421,376 planes, 16 n^2 = 1,048,576 lattice samples each (8.83e12 plane samples
 per step).  A step is one pass of the FP32 form over that grid with the volume resident in
-HBM.  Printed: one JSON line with plane samples/s, the dominant kernel's roofline
-(algorithmic bytes from the exact in-cube sample count), the FP32 form's relative L2 and
+HBM.  Printed: one JSON line with plane samples/s, the dominant kernel's roofline (VALU
+instruction issue: SQ_INSTS_VALU per pass from the committed counter summary over the pass
+time measured here; the measured HBM traffic and a byte model beside it), the FP32 form's relative L2 and
 max error against the plain FP64 form, the plain FP64 form's rate on a subset of the same
 grid, and a CPU baseline (the reference's own plane_integral, compiled from its sources, on
 the host cores over a bounded sample of planes; the oracle's restatement when that build is
@@ -24,8 +25,9 @@ all in one persistent graph-captured launch (direction groups autotuned on the f
 the replay's time is compared with the same 96 jobs launched one by one, and with their
 plain FP64 forms.
 
-The "bf16" object is the same pass from BF16 bricks (time, rate, its algorithmic bytes
-against the HBM peak, relL2 against the plain FP64 subset).  The "transpose" object is the FP32 exact transpose of the plane integrals on the same grid
+The "bf16" object is the same pass from BF16 bricks (time, rate, its byte model, relL2
+against the plain FP64 subset): an accuracy-reduced storage form, not a faster one, since the
+pass is bound by instruction issue and not by the bricks' bytes.  The "transpose" object is the FP32 exact transpose of the plane integrals on the same grid
 (rate, error against the plain FP64 transpose on a phi subset, and the dot-product test of
 the FP32 pair at full size); "rays" is the line-integral workload at 128^3 (128 z x 180 phi
 x 257 shifts: FP32 against FP64, the FP32 transpose with its dot-product test, and the FP64
@@ -61,6 +63,8 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
 FP64_PEAK = 78.6e12  # MI355X dense FP64, flop/s
+# VALU issue: 256 CUs x 4 SIMDs x 2.4 GHz, one wave64 instruction per SIMD per 4 cycles
+VALU_PEAK = 256 * 4 * 2.4e9 / 4
 
 
 def log(*a):
@@ -75,6 +79,9 @@ def rel_l2(a, b):
 
 REF_RADON_LIB = os.path.join(ROOT, "oracle", "_ref", "libref_radon_grid.so")
 REF_RAY_LIB = os.path.join(ROOT, "oracle", "_ref", "libref_ray_grid.so")
+# the same sources at -O2 (oracle/Makefile): the stronger CPU figure
+REF_RADON_LIB_O2 = os.path.join(ROOT, "oracle", "_ref", "libref_radon_grid_O2.so")
+REF_RAY_LIB_O2 = os.path.join(ROOT, "oracle", "_ref", "libref_ray_grid_O2.so")
 
 
 def _seeded_planes(rng, phi, th, s, m):
@@ -82,57 +89,57 @@ def _seeded_planes(rng, phi, th, s, m):
         s[rng.integers(len(s), size=m)]
 
 
-def _reference_planes(vol, threads, seconds, rng, phi, th, s):
-    """The reference's own plane_integral (radon_grid/src/integral.c:26-59, compiled from its
-    sources by oracle/Makefile at -O0 as its Makefile builds it) over seeded planes, one
-    host thread per core as its OpenMP driver runs it (radon_grid/src/radon_grid.c:229-264).
-    ctypes releases the GIL for each call, so the threads run the C code concurrently."""
+def _ref_batch(path, fn, vol, threads, seconds, draw):
+    """Time the reference's own per-element function `fn` (plane_integral or ray_transform,
+    compiled from its sources by oracle/Makefile) through the OpenMP batch driver linked
+    beside it (oracle/ref_batch.c), one thread per core as the reference's OpenMP drivers run
+    it (radon_grid/src/radon_grid.c:229-264, ray_grid/src/ray_grid.c:224-262), over seeded
+    elements draw(m) -> (a, b, c), growing m until the sample takes about `seconds`.
+    Returns (m, seconds taken)."""
     import ctypes as C
-    from concurrent.futures import ThreadPoolExecutor
 
-    lib = C.CDLL(REF_RADON_LIB)
-    lib.plane_integral.restype = C.c_double
-    lib.plane_integral.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double]
+    lib = C.CDLL(path)
+    f = getattr(lib, fn)
+    f.restype = None
+    f.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_long, C.c_int,
+                  C.c_void_p]
     n = vol.shape[0]
     v = np.ascontiguousarray(vol, dtype=np.float64)
-    # the reference's jagged double*** view of the same n^3 buffer
-    rows = (C.c_void_p * (n * n))(*[v.ctypes.data + 8 * n * r for r in range(n * n)])
-    top = (C.c_void_p * n)(*[C.addressof(rows) + C.sizeof(C.c_void_p) * n * x for x in range(n)])
-
-    def run(args):
-        p, t, sh = args
-        return [lib.plane_integral(top, n, float(a), float(b), float(c)) for a, b, c in zip(p, t, sh)]
-
-    def timed(m):
-        p, t, sh = _seeded_planes(rng, phi, th, s, m)
-        parts = [(p[k::threads], t[k::threads], sh[k::threads]) for k in range(threads)]
-        with ThreadPoolExecutor(threads) as ex:
-            t0 = time.perf_counter()
-            list(ex.map(run, parts))
-            return time.perf_counter() - t0
-
-    m = 2 * threads
+    rows, top = _jagged(v)
+    m = 4 * threads
     while True:
-        dt = timed(m)
-        if dt >= 0.8 * seconds or m >= 1 << 22:
+        a, b, c = (np.ascontiguousarray(x, dtype=np.float64) for x in draw(m))
+        out = np.empty(m)
+        t0 = time.perf_counter()
+        f(top, n, a.ctypes.data, b.ctypes.data, c.ctypes.data, m, threads, out.ctypes.data)
+        dt = time.perf_counter() - t0
+        if dt >= 0.8 * seconds or m >= 1 << 24:
             return m, dt
-        m = max(m + threads, int(m * min(16.0, 1.1 * seconds / max(dt, 1e-3))) // threads * threads)
+        m = max(m + threads, int(m * min(16.0, 1.1 * seconds / max(dt, 1e-3))))
 
 
-def cpu_baseline(vol, phi, th, s, seconds=12.0):
+def cpu_baseline(vol, phi, th, s, seconds=10.0):
     """CPU rate of the plain plane integrals on the host cores, on a bounded seeded sample of
-    the grid's planes: the reference's own plane_integral when oracle/_ref is built, else the
-    oracle's restatement (plain C, -O2, OpenMP)."""
+    the grid's planes: the reference's own plane_integral built from its sources at -O2
+    (value) and, beside it, at -O0 as its Makefile builds it, when oracle/_ref is built;
+    else the oracle's restatement (plain C, -O2, OpenMP)."""
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     rng = np.random.default_rng(1234)
     n = vol.shape[0]
-    if os.path.exists(REF_RADON_LIB):
-        m, dt = _reference_planes(vol, threads, seconds, rng, phi, th, s)
+    if os.path.exists(REF_RADON_LIB_O2) and os.path.exists(REF_RADON_LIB):
+        def draw(m):
+            return _seeded_planes(rng, phi, th, s, m)
+
+        m, dt = _ref_batch(REF_RADON_LIB_O2, "ref_plane_batch", vol, threads, seconds, draw)
+        m0, dt0 = _ref_batch(REF_RADON_LIB, "ref_plane_batch", vol, threads, 0.6 * seconds, draw)
         return {"value": m * 16.0 * n * n / dt, "unit": "plane samples/s", "cores": threads,
                 "kind": "reference",
                 "sample": f"{m} seeded uniform planes of the {n}^3 grid, {dt:.1f} s; the "
-                          "reference's plane_integral built from its sources at -O0 (its "
-                          "Makefile's flags), one thread per core"}
+                          "reference's plane_integral built from its sources at -O2, "
+                          "called per plane from an OpenMP loop, one thread per core",
+                "reference_O0": {"value": m0 * 16.0 * n * n / dt0, "unit": "plane samples/s",
+                                 "sample": f"{m0} seeded planes, {dt0:.1f} s; the same sources "
+                                           "at -O0 (its Makefile's flags)"}}
     import orc
 
     m = max(threads, 8)
@@ -162,44 +169,29 @@ def _jagged(v):
 
 def rays_cpu(vol, z, s, phi, rays_in, rs, rphi, th, seconds):
     """CPU rates beside the rays leg on bounded seeded samples: the reference's own
-    ray_transform (ray_grid/src/integral.c:11-50, built from its sources at -O0 by
+    ray_transform (ray_grid/src/integral.c:11-50, built from its sources at -O2 and at -O0 by
     oracle/Makefile) over seeded rays of the grid, one thread per core as its OpenMP driver
     runs it (ray_grid/src/ray_grid.c:242-255); and the oracle's reduction (its spline rows
     solved once, OpenMP over planes) on a subset of theta of the reduction chain."""
-    import ctypes as C
-    from concurrent.futures import ThreadPoolExecutor
-
     import orc
 
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     rng = np.random.default_rng(4321)
     n = vol.shape[0]
     out = {}
-    if os.path.exists(REF_RAY_LIB):
-        lib = C.CDLL(REF_RAY_LIB)
-        lib.ray_transform.restype = C.c_double
-        lib.ray_transform.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double]
-        v = np.ascontiguousarray(vol, dtype=np.float64)
-        rows, top = _jagged(v)
+    for path, flags, key in ((REF_RAY_LIB_O2, "-O2", "ray_grid"), (REF_RAY_LIB, "-O0", "ray_grid_O0")):
+        if not os.path.exists(path):
+            continue
 
-        def run(args):
-            return [lib.ray_transform(top, n, float(a), float(b), float(c)) for a, b, c in zip(*args)]
+        def draw(m):
+            return (a[rng.integers(len(a), size=m)] for a in (z, s, phi))
 
-        m = 4 * threads
-        while True:
-            zz, ss, pp = (a[rng.integers(len(a), size=m)] for a in (z, s, phi))
-            parts = [(zz[k::threads], ss[k::threads], pp[k::threads]) for k in range(threads)]
-            with ThreadPoolExecutor(threads) as ex:
-                t0 = time.perf_counter()
-                list(ex.map(run, parts))
-                dt = time.perf_counter() - t0
-            if dt >= 0.8 * seconds or m >= 1 << 22:
-                break
-            m = max(m + threads, int(m * min(16.0, 1.1 * seconds / max(dt, 1e-3))) // threads * threads)
-        out["ray_grid"] = {"value": m * (4.0 * n + 1.0) / dt, "unit": "ray samples/s",
-                           "cores": threads, "kind": "reference",
-                           "sample": f"{m} seeded rays of the {n}^3 grid, {dt:.1f} s; the "
-                                     "reference's ray_transform built from its sources at -O0"}
+        m, dt = _ref_batch(path, "ref_ray_batch", vol, threads, seconds, draw)
+        out[key] = {"value": m * (4.0 * n + 1.0) / dt, "unit": "ray samples/s",
+                    "cores": threads, "kind": "reference",
+                    "sample": f"{m} seeded rays of the {n}^3 grid, {dt:.1f} s; the "
+                              f"reference's ray_transform built from its sources at {flags}, "
+                              "called per ray from an OpenMP loop"}
     k = 1
     while True:
         sub = np.ascontiguousarray(th[:: max(1, len(th) // k)][:k])
@@ -567,15 +559,47 @@ def traffic_probe(args):
           flush=True)
 
 
-def traffic_from_profile():
-    """HBM bytes per pass from the committed counter summary (None if absent)."""
+VALU_FILE = os.path.join(ROOT, "profiles", "radon_fast_256_valu_pmc.md")
+# the sources the plane-integral kernel is built from: a counter summary holds their hash,
+# and its figures are used only while the sources are unchanged
+KERNEL_SOURCES = [os.path.join(ROOT, "wrt-project_amd", "csrc", f)
+                  for f in ("radon_brick.hpp", "k_fast.hip", "wrt_fast.h")]
+
+
+def kernel_sources_sha():
+    import hashlib
+
+    h = hashlib.sha256()
+    for f in KERNEL_SOURCES:
+        with open(f, "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()
+
+
+def _profile_figure(path, key):
+    """the figure `key:` of a committed counter summary, if it was taken on the current
+    kernel sources (None otherwise)"""
+    vals = {}
     try:
-        for line in open(TRAFFIC_FILE):
-            if line.startswith("traffic_bytes_per_pass:"):
-                return float(line.split(":")[1])
+        for line in open(path):
+            if ":" in line and not line.startswith((" ", "#", "|")):
+                k, v = line.split(":", 1)
+                vals[k.strip()] = v.strip()
     except OSError:
-        pass
-    return None
+        return None
+    if vals.get("kernel_sources_sha256") != kernel_sources_sha() or key not in vals:
+        return None
+    return float(vals[key])
+
+
+def traffic_from_profile():
+    """HBM bytes per pass from the committed counter summary (None if absent or stale)."""
+    return _profile_figure(TRAFFIC_FILE, "traffic_bytes_per_pass")
+
+
+def valu_from_profile():
+    """VALU wavefront instructions per pass from the committed counter summary"""
+    return _profile_figure(VALU_FILE, "valu_insts_per_pass")
 
 
 def main():
@@ -668,13 +692,37 @@ def main():
         bj.close()
         balg = incube * 2 + nplanes * 4 + n ** 3 * 2
         bf16 = {"seconds": bst["kernel_seconds"], "samples_per_s": logical / bst["kernel_seconds"],
-                "alg_bytes": balg, "frac_of_hbm": balg / bst["kernel_seconds"] / HBM_PEAK,
+                "alg_bytes_model": balg, "alg_bytes_model_per_s": balg / bst["kernel_seconds"],
                 "rel_l2_vs_f64": rel_l2(bout[:, ::args.plain_phi_stride, :], plain)}
         log(f"bench: bf16 {bst['kernel_seconds']:.3f} s, relL2 {bf16['rel_l2_vs_f64']:.2e}")
     eb = 2 if args.prec == "bf16" else 4
     alg_bytes = incube * eb + nplanes * 4 + n ** 3 * eb
     kavg = float(np.mean(ks))
-    achieved = alg_bytes / kavg
+    # The pass is bound by VALU instruction issue (the gathers come from the brick staged in
+    # LDS): achieved = VALU wavefront instructions per pass (rocprofv3 SQ_INSTS_VALU, committed
+    # with the hash of the kernel sources it was taken on) over the pass time measured here.
+    # The HBM figures beside it are the measured traffic and a byte model that prices every
+    # in-cube sample's voxel read as an HBM read, which it is not.
+    valu = valu_from_profile()
+    traffic = traffic_from_profile()
+    roofline = {
+        "bound": "valu",
+        "achieved": valu / kavg / 1e9 if valu else None,
+        "peak": VALU_PEAK / 1e9,
+        "unit": "G VALU wavefront instructions/s",
+        "frac": valu / kavg / VALU_PEAK if valu else None,
+        "traffic": traffic,
+        "hbm_frac": traffic / kavg / HBM_PEAK if traffic else None,
+        "valu_insts_per_pass": valu,
+        "valu_lane_ops_per_incube_sample": valu * 64 / incube if valu else None,
+        "kernel": "k_radon_fast (interior and face-brick launches, concurrent; their span)",
+        "kernel_seconds": kavg,
+        "alg_bytes_model": {
+            "bytes": alg_bytes, "per_second": alg_bytes / kavg,
+            "note": "every in-cube sample's voxel priced at the brick's bytes as if read "
+                    "from HBM, plus outputs and one volume read; the kernel gathers from "
+                    "LDS, so this is a work measure, not a roofline"},
+    }
     ms = wall / args.steps * 1e3
     value = logical / (wall / args.steps)
 
@@ -772,11 +820,7 @@ def main():
         "plain_f64": {"value": plain_rate, "unit": "plane samples/s",
                       "subset": f"every {args.plain_phi_stride}th phi ({plain.size} planes)"},
         "speedup_vs_plain": value / plain_rate,
-        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic_from_profile(),
-                     "alg_bytes": alg_bytes,
-                     "kernel": "k_radon_fast (interior and face-brick launches, concurrent; their span)",
-                     "kernel_seconds": kavg},
+        "roofline": roofline,
         "cpu_baseline": cpu,
         "serve": serve,
         "batch": batch,
