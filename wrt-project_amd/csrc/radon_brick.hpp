@@ -87,11 +87,16 @@ __device__ __forceinline__ int row_max16(int v) {
 
 // max over each aligned group of 8 lanes, every lane of the group ends with it: quad_perm
 // [1,0,3,2] (xor 1), quad_perm [2,3,0,1] (xor 2), then row_half_mirror (lane i of a half
-// row with lane 7 - i, which lies in the other quad)
+// row with lane 7 - i, which lies in the other quad).  Every lane has a source lane in these
+// patterns, so the moves need no old value and each step combines into one DPP max.
+template <int CTRL>
+__device__ __forceinline__ int dpp_perm(int v) {
+  return __builtin_amdgcn_mov_dpp(v, CTRL, 0xf, 0xf, true);
+}
 __device__ __forceinline__ int group8_max(int v) {
-  v = max(v, dpp_ror<0xB1>(v));
-  v = max(v, dpp_ror<0x4E>(v));
-  return max(v, dpp_ror<0x141>(v));
+  v = max(v, dpp_perm<0xB1>(v));
+  v = max(v, dpp_perm<0x4E>(v));
+  return max(v, dpp_perm<0x141>(v));
 }
 
 __device__ __forceinline__ double wave_sum_dpp(double v) {
@@ -261,10 +266,17 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
   }
   // The rows' candidate column intervals are computed in FP32 against a slab widened by
   // RI_TOL in coordinates and a support sphere whose rho^2 is widened by 1e-4 (1 + rho^2),
-  // and then widened by RI_COLS columns.  Coordinates are at most 3 in magnitude, so FP32
-  // rounding moves a row's point by under 1e-6 and a column bound (|t| <= 4n + 2, n <= 4096)
-  // by under 5e-3 columns: the intervals still hold every column whose sample can fall in the
-  // brick and the sphere.  Which samples count is decided per sample, exactly.
+  // and then widened by RI_COLS columns.  A slab face F of axis c bounds the columns of row r
+  // at (F - (s n_c + a_c r)) / b_c, evaluated as kF - s kn + beta r with the per-direction
+  // products kF = F / b_c, kn = n_c / b_c, beta = -a_c / b_c rounded to FP32: each term's
+  // rounding, times |b_c|, is a coordinate error of at most 2^-24 times |F|, |s n_c| or
+  // |a_c r| (all below 3), so together under 6e-7 < RI_TOL whatever b_c, and the bound's own
+  // rounding (|t| <= 4n + 2, n <= 4096) moves it by under 5e-3 < RI_COLS columns.  The
+  // sphere |p|^2 <= rho^2 meets row r (p = s n + a r + b t, with n, a, b orthogonal) in the
+  // columns |t - t_c| <= sqrt(rho^2 - s^2 - |a|^2 r^2) / |b|, t_c = -(s n.b + r a.b) / |b|^2
+  // (zero up to the rounding of the basis).  The intervals therefore hold every column whose
+  // sample can fall in the brick and the sphere; which samples count is decided per sample,
+  // exactly.
   constexpr float RI_TOL = 4e-6f, RI_COLS = 1.0f / 64.0f;
   float lof[3], hcf[3];
 #pragma unroll
@@ -272,7 +284,7 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
     lof[c] = uniform_f32((float)lo[c] - RI_TOL);
     hcf[c] = uniform_f32((float)fmin(hi[c], 1.0) + RI_TOL);
   }
-  const float rho2f = uniform_f32((float)(a.rho2 + 1e-4 * (1.0 + a.rho2)));
+  const double rho2w = a.rho2 + 1e-4 * (1.0 + a.rho2);
 
   const int d_end = min((grp + 1) * a.dgroup, a.ndir);
   // waves take the group's directions one at a time from a shared counter, so that they
@@ -299,9 +311,14 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
     }
     const double aa = av[0] * av[0] + av[1] * av[1] + av[2] * av[2];
     const double bb = bv[0] * bv[0] + bv[1] * bv[1] + bv[2] * bv[2];
+    // The direction's row window R0..R1 and column window C0..C1 over the brick box.  The
+    // box spans (pa1 - pa0) / |a|^2 = 64 sum_c |a_c| / |a| <= 64 sqrt(3) < 111 rows (a box
+    // edge of 32 cells is 64 lattice steps of |a| = half a cell; 16 cells in z in transpose
+    // mode), so with the margins R1 - R0 < 118 and the rows fit the two 64-row passes.
     const int R0 = max(L0, (int)floor(pa0 / aa) - 2), R1 = min(L1, (int)ceil(pa1 / aa) + 2);
     const int C0 = max(L0, (int)floor(pb0 / bb) - 2), C1 = min(L1, (int)ceil(pb1 / bb) + 2);
     if (R0 > R1 || C0 > C1) continue;
+    const int nrow = R1 - R0 + 1;
     // shifts with s in [pn0 - eps, pn1 + eps] (the ascending copy of the grid, bisected)
     const double eps = 4.0 * delta;
     int j0, j1;
@@ -314,57 +331,32 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
       j1 = l - 1;
     }
     if (j0 > j1) continue;
-    // reciprocals of the column steps (for the conservative row intervals)
-    double ib[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) ib[c] = (fabs(bv[c]) * (4.0 * n) < 1e-9 * dc) ? 0.0 : 1.0 / bv[c];
-    const double ibb = 1.0 / bb, iaa = 1.0 / aa;
-    float avf[3], bvf[3], ibf[3];
+    // FP32 constants of the rows' column bounds (see above): for an axis with b_c != 0 the
+    // lower and upper slab faces give kL, kU; an axis with b_c == 0 (to 1e-9 of a cell over
+    // the lattice) bounds the row instead, tested per row, and its column bounds are the
+    // neutral -3e38 and 3e38 (kn = beta = 0), so that every row takes the max and min of
+    // three bounds with no per-axis selection.  kL and kU stay in vector registers: with kn
+    // and the row in scalar ones, each bound is one FMA within the one-scalar-operand limit.
+    float kL[3], kU[3], kn[3], beta[3], avf[3], nf[3];
+    bool zb[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
+      const double ibc = (fabs(bv[c]) * (4.0 * n) < 1e-9 * dc) ? 0.0 : 1.0 / bv[c];
+      const float ib = (float)ibc;
+      zb[c] = ibc == 0.0;
+      const float Lf = bv[c] > 0.0 ? lof[c] : hcf[c], Uf = bv[c] > 0.0 ? hcf[c] : lof[c];
+      kL[c] = zb[c] ? -3.0e38f : Lf * ib;
+      kU[c] = zb[c] ? 3.0e38f : Uf * ib;
+      kn[c] = uniform_f32((float)nrm[c] * ib);
+      beta[c] = uniform_f32(-(float)av[c] * ib);
       avf[c] = uniform_f32((float)av[c]);
-      bvf[c] = uniform_f32((float)bv[c]);
-      ibf[c] = uniform_f32((float)ib[c]);
+      nf[c] = uniform_f32((float)nrm[c]);
     }
-    const float bbf = uniform_f32((float)bb), ibbf = uniform_f32((float)ibb);
-    // this lane's box edge (lanes 0..11): axis ek, the other two coordinates at lo/hi
-    // corners.  The plane n.p = s meets it (if at all) at coordinate pk = (s - rest)/n_k,
-    // which is on the edge for s in [e_s0, e_s1]; the vertex's row a.p/|a|^2 is then the
-    // linear e_A s + e_B.  An edge almost parallel to the plane (|n_k| < 1e-9, where the
-    // linear form would cancel badly) contributes the rows of its whole length instead:
-    // e_A = 0, e_B its middle row, e_W its half-length in rows.
-    double e_A = 0.0, e_B = 0.0, e_W = 0.0, e_s0 = 1e300, e_s1 = -1e300;
-    {
-      const int ek = lane >> 2, ek1 = ek == 2 ? 0 : ek + 1, ek2 = ek == 0 ? 2 : ek - 1;
-      const double e_n = ek == 0 ? nrm[0] : (ek == 1 ? nrm[1] : nrm[2]);
-      const double e_n1 = ek1 == 0 ? nrm[0] : (ek1 == 1 ? nrm[1] : nrm[2]);
-      const double e_n2 = ek2 == 0 ? nrm[0] : (ek2 == 1 ? nrm[1] : nrm[2]);
-      const double e_a = ek == 0 ? av[0] : (ek == 1 ? av[1] : av[2]);
-      const double e_a1 = ek1 == 0 ? av[0] : (ek1 == 1 ? av[1] : av[2]);
-      const double e_a2 = ek2 == 0 ? av[0] : (ek2 == 1 ? av[1] : av[2]);
-      const double e_lo = ek == 0 ? lo[0] : (ek == 1 ? lo[1] : lo[2]);
-      const double e_hi = ek == 0 ? hi[0] : (ek == 1 ? hi[1] : hi[2]);
-      const double e_p1 = (lane & 1) ? (ek1 == 0 ? hi[0] : (ek1 == 1 ? hi[1] : hi[2]))
-                                     : (ek1 == 0 ? lo[0] : (ek1 == 1 ? lo[1] : lo[2]));
-      const double e_p2 = (lane & 2) ? (ek2 == 0 ? hi[0] : (ek2 == 1 ? hi[1] : hi[2]))
-                                     : (ek2 == 0 ? lo[0] : (ek2 == 1 ? lo[1] : lo[2]));
-      if (lane < 12 && e_n != 0.0) {
-        const double tol = 1e-9 * dc;
-        const double rest = e_n1 * e_p1 + e_n2 * e_p2, arest = e_a1 * e_p1 + e_a2 * e_p2;
-        const double u = rest + (e_lo - tol) * e_n, v = rest + (e_hi + tol) * e_n;
-        e_s0 = fmin(u, v);
-        e_s1 = fmax(u, v);
-        if (fabs(e_n) >= 1e-9) {
-          e_A = e_a / e_n * iaa;
-          e_B = (arest - e_a * rest / e_n) * iaa;
-        } else {
-          const double r0 = (e_a * e_lo + arest) * iaa, r1 = (e_a * e_hi + arest) * iaa;
-          e_B = 0.5 * (r0 + r1);
-          e_W = 0.5 * fabs(r1 - r0);
-        }
-      }
-    }
-    const double ia = 1.0 / sqrt(aa);
+    const bool anyzb = zb[0] || zb[1] || zb[2];
+    const float aaf = uniform_f32((float)aa), ibsf = uniform_f32((float)(1.0 / sqrt(bb)));
+    const float C0m1 = (float)(C0 - 1), C1p1 = (float)(C1 + 1);  // exact: |C| <= 2n
+    const float kab = uniform_f32((float)(-(av[0] * bv[0] + av[1] * bv[1] + av[2] * bv[2]) / bb));
+    const float knb = uniform_f32((float)(-(nrm[0] * bv[0] + nrm[1] * bv[1] + nrm[2] * bv[2]) / bb));
     // Fixed-point steps of the cell coordinate q_c = (x_c + 1) h (h = n/2, 32 fraction bits)
     // per row (a_c h) and per column (b_c h); |a_c h|, |b_c h| <= 1/2, clamped into int32
     // (error at most 2^-32 each).  Eight columns for the column walk.
@@ -378,6 +370,13 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
           (int)fmin(fmax(rint(bv[c] * h * 4294967296.0), -2147483648.0), 2147483647.0));
       D8[c] = 8ll * Dfx[c];
     }
+    // Lanes 0, 1, 2 form the fixed-point base of axes x, y, z at (row R0, column C0) for each
+    // section: these are their per-direction terms a_c R0, b_c C0 and b0_c.
+    const int ax = lane < 3 ? lane : 0;
+    const double l_n = ax == 0 ? nrm[0] : (ax == 1 ? nrm[1] : nrm[2]);
+    const double l_aR = (ax == 0 ? av[0] : (ax == 1 ? av[1] : av[2])) * (double)R0;
+    const double l_bC = (ax == 0 ? bv[0] : (ax == 1 ? bv[1] : bv[2])) * (double)C0;
+    const double l_b0 = ax == 0 ? fb0x : (ax == 1 ? fb0y : fb0z);
     for (int j = j0; j <= j1; ++j) {
       const int jo = a.sidx[j];  // the plane's row in the caller's shift order
       unsigned long long gq = 0ull;  // transpose: the section's quantized value
@@ -388,125 +387,110 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
         if (gq == 0ull) continue;
       }
       const double s = a.shift[j];
-      // the plane meets the support sphere in a circle of radius sqrt(rho2 - s^2) about the
-      // foot point s*n, where row 0 passes: rows farther than that (+1) are empty
-      const double rc2 = a.rho2 - s * s;
-      if (rc2 < 0.0) continue;
-      const double S[3] = {uniform_f64(s * nrm[0]), uniform_f64(s * nrm[1]),
-                           uniform_f64(s * nrm[2])};
-      const float Sf[3] = {uniform_f32((float)S[0]), uniform_f32((float)S[1]),
-                           uniform_f32((float)S[2])};
-      // rows of this section: the section polygon's vertices lie on the box edges; the row
-      // of a point p of the plane is a.p / |a|^2 (a is orthogonal to n and b), +-2 rows margin
-      int r_lo = INT_MAX, r_hi = INT_MIN;
-      if (s >= e_s0 && s <= e_s1) {
-        const double r = e_A * s + e_B;
-        r_lo = (int)fmax(floor(r - e_W) - 2.0, (double)L0);
-        r_hi = (int)fmin(ceil(r + e_W) + 2.0, (double)L1);
+      // the plane meets the support sphere only for s^2 <= rho^2
+      if (a.rho2 - s * s < 0.0) continue;
+      const float sf = (float)s;
+      float aL[3], aU[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        aL[c] = fmaf(-sf, kn[c], kL[c]);
+        aU[c] = fmaf(-sf, kn[c], kU[c]);
       }
-      r_lo = __builtin_amdgcn_readlane(row_min16(r_lo), 0);
-      r_hi = __builtin_amdgcn_readlane(row_max16(r_hi), 0);
-      int SR0 = max(R0, r_lo), SR1 = min(R1, r_hi);
-      if (!inside) {  // a brick inside the sphere: its rows lie within the circle
-        const double rr = sqrt(rc2) * ia + 1.0;
-        SR0 = max(SR0, (int)floor(-rr));
-        SR1 = min(SR1, (int)ceil(rr));
-      }
-      if (SR0 > SR1) continue;
-      const int nrows = SR1 - SR0 + 1;  // <= 122 by geometry (brick diameter + margins)
-      const int nbands = (nrows + 7) >> 3;
-      // conservative column interval of every row (rows SR0 + lane and SR0 + 64 + lane): first
-      // column into the wave's row table, 8-column step count reduced to a per-band maximum
+      const float rc2f = (float)(rho2w - s * s), tcs = sf * knb;
+      // Candidate column interval of every row R0 + t of the window (t = lane, lane + 64):
+      // first and last column into the wave's row table, the 8-column step count reduced to
+      // a per-band maximum (bands of the rows 8k .. 8k + 7 of the window), and the rows that
+      // meet the brick and the sphere as a ballot
       int bsteps[2];
+      unsigned long long live[2];
 #pragma unroll
       for (int half = 0; half < 2; ++half) {
-        if (64 * half >= nrows) {  // uniform: the section has at most 64 rows
+        if (64 * half >= nrow) {  // uniform: the window has at most 64 rows
           bsteps[half] = 0;
+          live[half] = 0ull;
           continue;
         }
         const int t = lane + 64 * half;
-        int rl = C0, rh = C0 - 1, nst = 0;
-        if (t < nrows) {
-          const float f1f = (float)(SR0 + t);  // exact: |row| <= 2n
-          float tlo = -3.0e38f, thi = 3.0e38f;
-          bool empty = false;
-          // the row's points p(t) = w + b t within the support sphere |p|^2 <= rho2 (every
-          // sample outside it reads a zero cell, so clipping the row there changes no sum);
-          // nothing to clip when the whole brick lies inside the sphere
-          if (!inside) {
-            const float wx = fmaf(avf[0], f1f, Sf[0]), wy = fmaf(avf[1], f1f, Sf[1]),
-                        wz = fmaf(avf[2], f1f, Sf[2]);
-            const float qb = wx * bvf[0] + wy * bvf[1] + wz * bvf[2];
-            const float qc = wx * wx + wy * wy + wz * wz - rho2f;
-            const float disc = qb * qb - bbf * qc;
-            if (disc < 0.0f) {
-              empty = true;
-            } else {
-              const float sq = sqrtf(disc);
-              tlo = (-qb - sq) * ibbf;
-              thi = (-qb + sq) * ibbf;
-            }
-          }
+        // branch-free except for the rare cases: every lane forms its row's bounds, and
+        // rows past the window or empty take the empty interval [C0, C0 - 1]
+        const float rf = (float)(R0 + t);  // exact: |row| <= 2n
+        float tlo = fmaxf(fmaxf(fmaf(beta[0], rf, aL[0]), fmaf(beta[1], rf, aL[1])),
+                          fmaf(beta[2], rf, aL[2]));
+        float thi = fminf(fminf(fmaf(beta[0], rf, aU[0]), fmaf(beta[1], rf, aU[1])),
+                          fmaf(beta[2], rf, aU[2]));
+        bool live_row = t < nrow;
+        if (anyzb) {  // rare: a basis component vanishes (structured directions)
 #pragma unroll
           for (int c = 0; c < 3; ++c) {
-            const float w = fmaf(avf[c], f1f, Sf[c]);
-            if (ibf[c] == 0.0f) {
-              if (w < lof[c] || w > hcf[c]) empty = true;
-            } else if (bvf[c] > 0.0f) {
-              tlo = fmaxf(tlo, (lof[c] - w) * ibf[c]);
-              thi = fminf(thi, (hcf[c] - w) * ibf[c]);
-            } else {
-              tlo = fmaxf(tlo, (hcf[c] - w) * ibf[c]);
-              thi = fminf(thi, (lof[c] - w) * ibf[c]);
-            }
-          }
-          if (!empty && tlo <= thi + 2.0f) {
-            tlo = fmaxf(tlo, (float)(C0 - 1));
-            thi = fminf(thi, (float)(C1 + 1));
-            rl = max(C0, (int)ceilf(tlo - RI_COLS));
-            rh = min(C1, (int)floorf(thi + RI_COLS));
-            if (BND) {
-              const double f1 = (double)(SR0 + t);
-#pragma unroll
-              for (int c = 0; c < 3; ++c)
-                if (flo[c] || fhi[c]) cube_ends(S[c] + av[c] * f1, bv[c], flo[c], fhi[c], rl, rh);
-            }
-            // steps of 8 columns
-            nst = rh >= rl ? ((rh - rl + 8) >> 3) : 0;
+            const float w = fmaf(avf[c], rf, sf * nf[c]);
+            if (zb[c] && (w < lof[c] || w > hcf[c])) live_row = false;
           }
         }
+        // nothing to clip when the whole brick lies inside the sphere
+        if (!inside) {
+          const float u = fmaf(-aaf * rf, rf, rc2f);
+          live_row = live_row && u >= 0.0f;
+          // the hardware square root (1 ulp): far inside the slack
+          const float hw = __builtin_amdgcn_sqrtf(fmaxf(u, 0.0f)) * ibsf;
+          const float tc = fmaf(rf, kab, tcs);
+          tlo = fmaxf(tlo, tc - hw);
+          thi = fminf(thi, tc + hw);
+        }
+        live_row = live_row && tlo <= thi + 2.0f;
+        // clamped into [C0 - 1, C1 + 1] (one med3 each), so the conversions stay in range
+        int rl = max(C0, (int)ceilf(__builtin_amdgcn_fmed3f(tlo, C0m1, C1p1) - RI_COLS));
+        int rh = min(C1, (int)floorf(__builtin_amdgcn_fmed3f(thi, C0m1, C1p1) + RI_COLS));
+        if (BND && live_row) {
+          const double f1 = (double)(R0 + t);
+#pragma unroll
+          for (int c = 0; c < 3; ++c)
+            if (flo[c] || fhi[c])
+              cube_ends(s * nrm[c] + av[c] * f1, bv[c], flo[c], fhi[c], rl, rh);
+        }
+        rl = live_row ? rl : C0;
+        rh = live_row ? rh : C0 - 1;
+        // steps of 8 columns
+        const int nst = rh >= rl ? ((rh - rl + 8) >> 3) : 0;
         // first and last column, 16 bits each (|i2| <= 2n < 2^15)
         rowlo[wv][t] = (int)(((unsigned)rl & 0xffffu) | ((unsigned)rh << 16));
         // max over the 8 consecutive rows of a band (lanes 8b .. 8b+7)
         bsteps[half] = group8_max(nst);
+        live[half] = __ballot(nst != 0);
       }
-      if (__ballot((bsteps[0] | bsteps[1]) != 0) == 0)
-        continue;  // no row of the section meets the brick and the sphere
+      if ((live[0] | live[1]) == 0ull) continue;  // no row meets the brick and the sphere
+      // the bands that hold the first and the last live row
+      const int t_first = live[0] ? __builtin_ctzll(live[0]) : 64 + __builtin_ctzll(live[1]);
+      const int t_last = live[1] ? 127 - __builtin_clzll(live[1]) : 63 - __builtin_clzll(live[0]);
+      const int k_first = t_first >> 3, k_last = t_last >> 3;
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       // Cell decisions in fixed point.  Q_c holds (x_c + 1) h - b0_c with 32 fraction bits,
       // where x_c is the sample's coordinate: Q = Qb + t A + (col - C0) D, Qb the value at
-      // (row SR0, column C0) converted from binary64.  Against the reference's quotient q
+      // (row R0, column C0) converted from binary64.  Against the reference's quotient q
       // (x rounded as radon_grid/src/integral.c:47-51 rounds it, then (x + 1) / (2/n)) the
-      // error is at most (1 + t + col - C0) 2^-32 (t < 128 rows; col - C0 < 160: a brick's
+      // error is at most (1 + t + col - C0) 2^-32 (t < 118 rows; col - C0 < 160: a brick's
       // section spans at most 64 sqrt(3) + 6 columns, plus a step's overshoot) plus the
       // roundings of q and of Qb (below 1e-10 for n < 16384): under 2^-23.  Q carries a bias
       // of 2^-22 (1024 units), so wherever its fraction is at least 2^-21 the reference's
       // quotient lies inside (k + 2^-23, k + 1 - 2^-23) for the integer part k of Q, which
       // is then the reference's truncated index (less b0); the few samples nearer a cell
-      // boundary take the exact binary64 decision instead.
+      // boundary take the exact binary64 decision instead.  Lane c (0..2) converts axis c;
+      // q lies within 160 cells of the brick, so its integer part fits int32 and its fraction
+      // rounded to 32 bits (at most 2^32 - 1 after the clamp) uint32.
       long long Qb[3];
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const double b0c = c == 0 ? fb0x : (c == 1 ? fb0y : fb0z);
-        const double q = ((S[c] + av[c] * (double)SR0) + bv[c] * (double)C0 + 1.0) * h - b0c;
+      {
+        const double q = (((s * l_n + l_aR) + l_bC) + 1.0) * h - l_b0;
         const double fl = floor(q);
         const double fr = fmin(rint((q - fl) * 4294967296.0), 4294967295.0);
-        const long long qb = (long long)fl * 4294967296ll + (long long)fr + 1024ll;
-        Qb[c] = (long long)(((unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane(
-                                 (int)(qb >> 32)) << 32) |
-                            (unsigned)__builtin_amdgcn_readfirstlane((int)qb));
+        const int qi = (int)fl;
+        const unsigned qf = (unsigned)fr;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const unsigned hi32 = (unsigned)__builtin_amdgcn_readlane(qi, c);
+          const unsigned lo32 = (unsigned)__builtin_amdgcn_readlane((int)qf, c);
+          Qb[c] = (long long)((((unsigned long long)hi32) << 32) | lo32) + 1024ll;
+        }
       }
       // One sample: Q the lane's fixed-point coordinates, row its row and c0 + step*m its
       // column (for the exact decision), mlast the last step inside the row (face bricks).
@@ -542,8 +526,13 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
           // rounds delta*v2*i2); the step index through readfirstlane keeps the column out
           // of the common path
           const double f2 = (double)(c0 + step * __builtin_amdgcn_readfirstlane(m));
-          const double f1 = (double)row;
-          const double Rx = S[0] + av[0] * f1, Ry = S[1] + av[1] * f1, Rz = S[2] + av[2] * f1;
+          // the row through an empty asm, so that the compiler does not hoist the row's
+          // binary64 terms below out of the sample loop into every band's setup
+          int rowv = row;
+          asm volatile("" : "+v"(rowv));
+          const double f1 = (double)rowv;
+          const double Rx = s * nrm[0] + av[0] * f1, Ry = s * nrm[1] + av[1] * f1,
+                       Rz = s * nrm[2] + av[2] * f1;
           const double x = Rx + bv[0] * f2, y = Ry + bv[1] * f2, z = Rz + bv[2] * f2;
           double qx, qy, qz;
           if (POW2) {
@@ -586,17 +575,16 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
       // cell at once (the LDS integer atomic serialises those, which costs time, not
       // results).  Measured at 256^3: 2.66 s against 2.92 s for the forward's columns and
       // 2.90 s for bands of 16 rows taken 2 apart (their longer rows' steps waste more).
-      int rlo_next = rowlo[wv][r_lane];
-      for (int k = 0; k < nbands; ++k) {
-        const int t = 8 * k + r_lane;  // this lane's row within the section
+      int rlo_next = rowlo[wv][8 * k_first + r_lane];
+      for (int k = k_first; k <= k_last; ++k) {
+        const int t = 8 * k + r_lane;  // this lane's row within the window
         const int rlo = (rlo_next << 16) >> 16;
-        const double frh = (double)(rlo_next >> 16);  // the row's last column
-        if (k + 1 < nbands) rlo_next = rowlo[wv][t + 8];
+        const int rhi = rlo_next >> 16;  // the row's last column
+        if (k < k_last) rlo_next = rowlo[wv][t + 8];
         const int steps = (k < 8) ? __builtin_amdgcn_readlane(bsteps[0], 8 * k)
                                   : __builtin_amdgcn_readlane(bsteps[1], 8 * (k - 8));
         if (steps == 0) continue;
-        const int row = SR0 + t;
-        const int rhi = (int)frh;
+        const int row = R0 + t;
         float bs = 0.f;
         if constexpr (T) {
           const int steps2 = (steps + 1) >> 1;  // 16-column steps cover the 8-column ones
