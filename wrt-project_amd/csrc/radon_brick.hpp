@@ -319,8 +319,10 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
     const int C0 = max(L0, (int)floor(pb0 / bb) - 2), C1 = min(L1, (int)ceil(pb1 / bb) + 2);
     if (R0 > R1 || C0 > C1) continue;
     const int nrow = R1 - R0 + 1;
-    // shifts with s in [pn0 - eps, pn1 + eps] (the ascending copy of the grid, bisected)
-    const double eps = 4.0 * delta;
+    // shifts with s in [pn0 - eps, pn1 + eps] (the ascending copy of the grid, bisected).  A
+    // plane that misses the closed box has every lattice point outside it, and a point's
+    // binary64 rounding (below 1e-15 here) is all that could bring its cell into the brick.
+    const double eps = 1e-9;
     int j0, j1;
     {
       int l = 0, r = a.nshift;
