@@ -19,8 +19,62 @@
 // interpolation of P from LDS.  Larger volumes read both planes from the FP32 [z][x][y] copy
 // in HBM.  Values and weights are FP32 (lerps reordered: z first), the sum along a ray is
 // Kahan-compensated in FP32, times delta in FP64 at the end.
-#include "radon_brick.hpp"  // cube_ends, div_rn
+#include "radon_brick.hpp"  // div_rn
 #include "wrt_dev.h"
+
+// Exact cube test of one axis on a row: x(c) = w + b*c in binary64 (the reference's
+// rounding) is monotone in c, so {c : x >= -1} and {c : x <= 1} are a prefix or a suffix of
+// the row.  Narrows [rl, rh] to the columns passing the tests that lo / hi ask for.  The
+// candidate interval comes from continuous bounds widened by a small slack, so the first
+// probe or its neighbour usually settles it; a bisection covers rows almost parallel to the
+// face.
+__device__ __forceinline__ void cube_ends(double w, double b, bool lo, bool hi, int& rl,
+                                          int& rh) {
+  if (rl > rh) return;
+  for (int side = 0; side < 2; ++side) {
+    if (side == 0 ? !lo : !hi) continue;
+    // P(c): the test holds at column c; suffix-type when x moves into the cube as c grows
+    const bool suffix = side == 0 ? b > 0.0 : b < 0.0;
+    auto P = [&](int c) {
+      const double x = w + b * (double)c;
+      return side == 0 ? x >= -1.0 : x <= 1.0;
+    };
+    if (b == 0.0) {
+      if (!P(rl)) rh = rl - 1;
+    } else if (suffix) {
+      if (!P(rl)) {
+        if (rl < rh && P(rl + 1)) {
+          rl = rl + 1;  // the candidate interval's slack added one column
+        } else if (!P(rh)) {
+          rh = rl - 1;
+        } else {
+          int l = rl, h = rh;  // P(l) false, P(h) true
+          while (h - l > 1) {
+            const int m = (l + h) >> 1;
+            if (P(m)) h = m; else l = m;
+          }
+          rl = h;
+        }
+      }
+    } else {
+      if (!P(rh)) {
+        if (rl < rh && P(rh - 1)) {
+          rh = rh - 1;
+        } else if (!P(rl)) {
+          rh = rl - 1;
+        } else {
+          int l = rl, h = rh;  // P(l) true, P(h) false
+          while (h - l > 1) {
+            const int m = (l + h) >> 1;
+            if (P(m)) l = m; else h = m;
+          }
+          rh = l;
+        }
+      }
+    }
+    if (rl > rh) return;
+  }
+}
 
 #define RAY_THREADS 512
 #define RAY_LDS_N 181  // largest n whose n^2 FP32 plane fits the 128 KiB stage

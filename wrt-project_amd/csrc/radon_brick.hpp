@@ -66,25 +66,6 @@ __device__ __forceinline__ double dpp_move(double v) {
   const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWMASK, 0xf, true);
   return __hiloint2double(hi, lo);
 }
-// min / max over the 16 lanes of each DPP row (row_ror 1, 2, 4, 8): every lane of a row ends
-// with its row's extremum
-template <int CTRL>
-__device__ __forceinline__ int dpp_ror(int v) {
-  return __builtin_amdgcn_update_dpp(v, v, CTRL, 0xf, 0xf, false);
-}
-__device__ __forceinline__ int row_min16(int v) {
-  v = min(v, dpp_ror<0x121>(v));
-  v = min(v, dpp_ror<0x122>(v));
-  v = min(v, dpp_ror<0x124>(v));
-  return min(v, dpp_ror<0x128>(v));
-}
-__device__ __forceinline__ int row_max16(int v) {
-  v = max(v, dpp_ror<0x121>(v));
-  v = max(v, dpp_ror<0x122>(v));
-  v = max(v, dpp_ror<0x124>(v));
-  return max(v, dpp_ror<0x128>(v));
-}
-
 // max over each aligned group of 8 lanes, every lane of the group ends with it: quad_perm
 // [1,0,3,2] (xor 1), quad_perm [2,3,0,1] (xor 2), then row_half_mirror (lane i of a half
 // row with lane 7 - i, which lies in the other quad).  Every lane has a source lane in these
@@ -123,60 +104,6 @@ __device__ __forceinline__ float uniform_f32(float v) {
 
 __device__ __forceinline__ float bf16_to_f32(unsigned short v) {
   return __uint_as_float(((unsigned)v) << 16);
-}
-
-// Exact cube test of one axis on a row: x(c) = w + b*c in binary64 (the reference's
-// rounding) is monotone in c, so {c : x >= -1} and {c : x <= 1} are a prefix or a suffix of
-// the row.  Narrows [rl, rh] to the columns passing the tests that lo / hi ask for.  The
-// candidate interval comes from continuous bounds widened by a small slack, so the first
-// probe or its neighbour usually settles it; a bisection covers rows almost parallel to the
-// face.
-__device__ __forceinline__ void cube_ends(double w, double b, bool lo, bool hi, int& rl,
-                                          int& rh) {
-  if (rl > rh) return;
-  for (int side = 0; side < 2; ++side) {
-    if (side == 0 ? !lo : !hi) continue;
-    // P(c): the test holds at column c; suffix-type when x moves into the cube as c grows
-    const bool suffix = side == 0 ? b > 0.0 : b < 0.0;
-    auto P = [&](int c) {
-      const double x = w + b * (double)c;
-      return side == 0 ? x >= -1.0 : x <= 1.0;
-    };
-    if (b == 0.0) {
-      if (!P(rl)) rh = rl - 1;
-    } else if (suffix) {
-      if (!P(rl)) {
-        if (rl < rh && P(rl + 1)) {
-          rl = rl + 1;  // the candidate interval's slack added one column
-        } else if (!P(rh)) {
-          rh = rl - 1;
-        } else {
-          int l = rl, h = rh;  // P(l) false, P(h) true
-          while (h - l > 1) {
-            const int m = (l + h) >> 1;
-            if (P(m)) h = m; else l = m;
-          }
-          rl = h;
-        }
-      }
-    } else {
-      if (!P(rh)) {
-        if (rl < rh && P(rh - 1)) {
-          rh = rh - 1;
-        } else if (!P(rl)) {
-          rh = rl - 1;
-        } else {
-          int l = rl, h = rh;  // P(l) true, P(h) false
-          while (h - l > 1) {
-            const int m = (l + h) >> 1;
-            if (P(m)) l = m; else h = m;
-          }
-          rh = l;
-        }
-      }
-    }
-    if (rl > rh) return;
-  }
 }
 
 // The LDS a brick task works in: the brick in the skewed layout plus one zero slot that
@@ -247,11 +174,14 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
   const double fb0x = (double)b0x, fb0y = (double)b0y, fb0z = (double)b0z;
   const int L0 = -2 * n, L1 = 2 * n - 1;  // lattice index range of i1 and i2 (bounds of R0..R1, C0..C1)
   // Faces of the cube the brick touches.  Only along those axes can a sample whose cell is
-  // in the brick fail the reference's cube test |x| <= 1 (elsewhere x < -1 truncates to cell
-  // 0, outside the brick, and x > 1 lands past it).  Face bricks apply the cube test exactly
-  // at the ends of every row (x is monotone along a row) and mask the lanes past a row's
-  // end; x == 1 exactly gives index n, which the reference maps to n - 1.
-  const bool flo[3] = {b0x == 0, b0y == 0, b0z == 0};
+  // in the brick fail the reference's cube test |x| <= 1.  The fixed-point decisions below
+  // floor the cell coordinate, so a sample with x < -1 gets cell -1 or less, outside every
+  // brick, and one with x > 1 gets cell n or more, past the brick or in its zero padding
+  // (cells past n hold 0 in the forward and are never read back in the transpose): both add
+  // nothing, as the reference's cube test gives.  Only the exact binary64 decision near a
+  // cell boundary truncates as zero_interp does, where -1 < q < 0 (x just below -1) would
+  // give cell 0, so on face bricks it applies the cube test itself; there x == 1 exactly
+  // gives index n, which the reference maps to n - 1.
   const bool fhi[3] = {b0x + FB >= n, b0y + FB >= n, b0z + ZN >= n};
   const int cxh = fhi[0] ? n - 1 : INT_MAX, cyh = fhi[1] ? n - 1 : INT_MAX,
             czh = fhi[2] ? n - 1 : INT_MAX;
@@ -440,17 +370,8 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
         }
         live_row = live_row && tlo <= thi + 2.0f;
         // clamped into [C0 - 1, C1 + 1] (one med3 each), so the conversions stay in range
-        int rl = max(C0, (int)ceilf(__builtin_amdgcn_fmed3f(tlo, C0m1, C1p1) - RI_COLS));
-        int rh = min(C1, (int)floorf(__builtin_amdgcn_fmed3f(thi, C0m1, C1p1) + RI_COLS));
-        if (BND && live_row) {
-          const double f1 = (double)(R0 + t);
-#pragma unroll
-          for (int c = 0; c < 3; ++c)
-            if (flo[c] || fhi[c])
-              cube_ends(s * nrm[c] + av[c] * f1, bv[c], flo[c], fhi[c], rl, rh);
-        }
-        rl = live_row ? rl : C0;
-        rh = live_row ? rh : C0 - 1;
+        const int rl = live_row ? max(C0, (int)ceilf(__builtin_amdgcn_fmed3f(tlo, C0m1, C1p1) - RI_COLS)) : C0;
+        const int rh = live_row ? min(C1, (int)floorf(__builtin_amdgcn_fmed3f(thi, C0m1, C1p1) + RI_COLS)) : C0 - 1;
         // steps of 8 columns
         const int nst = rh >= rl ? ((rh - rl + 8) >> 3) : 0;
         // first and last column, 16 bits each (|i2| <= 2n < 2^15)
@@ -495,17 +416,14 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
         }
       }
       // One sample: Q the lane's fixed-point coordinates, row its row and c0 + step*m its
-      // column (for the exact decision), mlast the last step inside the row (face bricks).
-      // Forward: adds the cell's value to bs; transpose: adds the section's quantized value
-      // to the cell.
+      // column (for the exact decision).  Forward: adds the cell's value to bs; transpose:
+      // adds the section's quantized value to the cell.
       // The cell of brick offsets (lx, ly, lz): all three in [0, 32) (z: [0, 16) in transpose
-      // mode, tested as 2*lz < 32; a negative offset sets high bits of the OR) and, on a face
-      // brick, the step within its row.  ok and the cell's byte offset in the LDS brick
-      // (forward: slot 0 holds the zero that rejected samples read, the brick starts at
-      // slot 1, and a rejected sample gets offset 0).
-      auto cell = [&](int lx, int ly, int lz, int m, int mlast, bool& ok, unsigned& off) {
+      // mode, tested as 2*lz < 32; a negative offset sets high bits of the OR).  ok and the
+      // cell's byte offset in the LDS brick (forward: slot 0 holds the zero that rejected
+      // samples read, the brick starts at slot 1, and a rejected sample gets offset 0).
+      auto cell = [&](int lx, int ly, int lz, bool& ok, unsigned& off) {
         ok = ((unsigned)lx | (unsigned)ly | ((unsigned)lz << (T ? 1 : 0))) < (unsigned)FB;
-        if (BND) ok = ok && m <= mlast;
         if constexpr (T) {
           off = mad_u24((unsigned)lx, 8u * FT_PX, mad_u24((unsigned)ly, 8u * FT_PY, (unsigned)lz * 8u));
         } else {
@@ -513,15 +431,14 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
           off = ok ? off : 0u;
         }
       };
-      auto sample = [&](const long long (&Q)[3], int row, int c0, int step, int m, int mlast,
-                        float& bs) {
+      auto sample = [&](const long long (&Q)[3], int row, int c0, int step, int m, float& bs) {
         const unsigned u = min(min((unsigned)Q[0], (unsigned)Q[1]), (unsigned)Q[2]);
         // the fixed point's cell (away from a boundary the fixed point floors where the
-        // reference truncates: they differ only for -1 < q < 0, x < -1, which an interior
-        // brick rejects either way and a face brick's row never holds) ...
+        // reference truncates: they differ only for -1 < q < 0, x < -1, where the floor's
+        // cell -1 lies outside every brick and the reference's cube test gives 0) ...
         bool ok;
         unsigned off;
-        cell((int)(Q[0] >> 32), (int)(Q[1] >> 32), (int)(Q[2] >> 32), m, mlast, ok, off);
+        cell((int)(Q[0] >> 32), (int)(Q[1] >> 32), (int)(Q[2] >> 32), ok, off);
         if (u < 2048u) {
           // ... replaced near a cell boundary (fraction of Q below 2^-21) by the reference's
           // rounding, in binary64 (the per-column products b*i2 rounded as the reference
@@ -548,15 +465,19 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
           }
           // truncation as in zero_interp; for an interior brick a negative quotient (x < -1)
           // truncates to 0 <= b0 - 32 and a quotient above n lands past the brick, so the
-          // brick test also is the cube test.  Face bricks: the row ends hold the cube test,
-          // and x == 1 (index n) maps to n - 1.
+          // brick test also is the cube test.  Face bricks apply the cube test itself, and
+          // x == 1 (index n) maps to n - 1.
           int ix = (int)qx, iy = (int)qy, iz = (int)qz;
           if (BND) {
             ix = min(ix, cxh);
             iy = min(iy, cyh);
             iz = min(iz, czh);
           }
-          cell(ix - b0x, iy - b0y, iz - b0z, m, mlast, ok, off);
+          cell(ix - b0x, iy - b0y, iz - b0z, ok, off);
+          if (BND && (fabs(x) > 1.0 || fabs(y) > 1.0 || fabs(z) > 1.0)) {
+            ok = false;
+            if constexpr (!T) off = 0u;
+          }
         }
         if constexpr (T) {
           if (ok)
@@ -581,7 +502,6 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
       for (int k = k_first; k <= k_last; ++k) {
         const int t = 8 * k + r_lane;  // this lane's row within the window
         const int rlo = (rlo_next << 16) >> 16;
-        const int rhi = rlo_next >> 16;  // the row's last column
         if (k < k_last) rlo_next = rowlo[wv][t + 8];
         const int steps = (k < 8) ? __builtin_amdgcn_readlane(bsteps[0], 8 * k)
                                   : __builtin_amdgcn_readlane(bsteps[1], 8 * (k - 8));
@@ -598,9 +518,8 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
 #pragma unroll
             for (int c = 0; c < 3; ++c)
               Q[c] = Qb[c] + (long long)tq * Afx[c] + (long long)(c0 - C0) * Dfx[c];
-            const int mlast = (rhi - c0) >> 4;  // floor: negative when the row ends before
             for (int m = 0; m < steps2; ++m) {
-              sample(Q, row, c0, 16, m, mlast, bs);
+              sample(Q, row, c0, 16, m, bs);
 #pragma unroll
               for (int c = 0; c < 3; ++c) Q[c] += 2 * D8[c];
             }
@@ -616,9 +535,8 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
 #pragma unroll
           for (int c = 0; c < 3; ++c)
             Q[c] = Qb[c] + (long long)tq * Afx[c] + (long long)(c0 - C0) * Dfx[c];
-          const int mlast = (rhi - c0) >> 3;
           for (int m = 0; m < steps; ++m) {
-            sample(Q, row, c0, 8, m, mlast, bs);
+            sample(Q, row, c0, 8, m, bs);
 #pragma unroll
             for (int c = 0; c < 3; ++c) Q[c] += D8[c];
           }
