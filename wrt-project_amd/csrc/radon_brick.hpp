@@ -13,8 +13,8 @@
 // power-of-two scaling or a corrected reciprocal), so only the value rounding (FP32/BF16)
 // and the summation order differ from the plain form.  Samples whose cell lies in the
 // brick add its FP32 value to a per-band FP32 partial, folded into an FP64 lane sum; a
-// section's lane sums are reduced over the wavefront in FP64 and added to the
-// plane's FP64 accumulator (one atomic per section and brick).
+// section's lane sums are reduced over the wavefront in FP32 and added to the plane's FP64
+// accumulator (one atomic per section and brick).
 //
 // Transpose mode works on half bricks (32 x 32 x 16 cells) of 64-bit integer accumulators:
 // every sample whose cell lies in the half brick adds the section's value, quantized to a
@@ -57,15 +57,6 @@ __device__ __forceinline__ unsigned mad_u24(unsigned a, unsigned b, unsigned c) 
   return r;
 }
 
-// Wavefront sum of a double with DPP lane moves (no LDS round trips): Hillis-Steele scan
-// within 16-lane rows (row_shr 1, 2, 4, 8, zero fill), then row broadcasts 15 and 31; the
-// total ends in lane 63 and is returned to every lane.
-template <int CTRL, int ROWMASK>
-__device__ __forceinline__ double dpp_move(double v) {
-  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWMASK, 0xf, true);
-  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWMASK, 0xf, true);
-  return __hiloint2double(hi, lo);
-}
 // max over each aligned group of 8 lanes, every lane of the group ends with it: quad_perm
 // [1,0,3,2] (xor 1), quad_perm [2,3,0,1] (xor 2), then row_half_mirror (lane i of a half
 // row with lane 7 - i, which lies in the other quad).  Every lane has a source lane in these
@@ -80,16 +71,22 @@ __device__ __forceinline__ int group8_max(int v) {
   return max(v, dpp_perm<0x141>(v));
 }
 
-__device__ __forceinline__ double wave_sum_dpp(double v) {
-  v += dpp_move<0x111, 0xf>(v);  // row_shr:1
-  v += dpp_move<0x112, 0xf>(v);  // row_shr:2
-  v += dpp_move<0x114, 0xf>(v);  // row_shr:4
-  v += dpp_move<0x118, 0xf>(v);  // row_shr:8
-  v += dpp_move<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
-  v += dpp_move<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
-  const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
-  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
-  return __hiloint2double(hi, lo);
+// Wavefront sum of a float with DPP lane moves (no LDS round trips): Hillis-Steele scan
+// within 16-lane rows (row_shr 1, 2, 4, 8, zero fill), then row broadcasts 15 and 31; the
+// total ends in lane 63.  Each step is one DPP add; six roundings of 2^-24 on partial sums.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ float dpp_move_f32(float v) {
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWMASK, 0xf, true));
+}
+__device__ __forceinline__ float wave_sum_f32(float v) {
+  v += dpp_move_f32<0x111, 0xf>(v);  // row_shr:1
+  v += dpp_move_f32<0x112, 0xf>(v);  // row_shr:2
+  v += dpp_move_f32<0x114, 0xf>(v);  // row_shr:4
+  v += dpp_move_f32<0x118, 0xf>(v);  // row_shr:8
+  v += dpp_move_f32<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+  v += dpp_move_f32<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 
 // a wave-uniform double moved into scalar registers (frees vector registers)
@@ -544,7 +541,9 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
         }
       }
       if constexpr (!T) {
-        const double tot = wave_sum_dpp(ks);
+        // the section's lane sums reduced in FP32 (one DPP add per step), its total added to
+        // the plane's FP64 accumulator
+        const double tot = (double)wave_sum_f32((float)ks);
         if (lane == 0 && tot != 0.0)
           unsafeAtomicAdd(a.acc + (size_t)jo * a.ndir + d, tot * delta * delta);
       }
