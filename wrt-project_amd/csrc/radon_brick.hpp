@@ -57,6 +57,18 @@ __device__ __forceinline__ unsigned mad_u24(unsigned a, unsigned b, unsigned c) 
   return r;
 }
 
+// floor(x) and floor(-x) converted to int32 in one instruction each (x within int32 range)
+__device__ __forceinline__ int cvt_floor(float x) {
+  int r;
+  asm("v_cvt_flr_i32_f32_e32 %0, %1" : "=v"(r) : "v"(x));
+  return r;
+}
+__device__ __forceinline__ int cvt_floor_neg(float x) {
+  int r;
+  asm("v_cvt_flr_i32_f32_e64 %0, -%1" : "=v"(r) : "v"(x));
+  return r;
+}
+
 // max over each aligned group of 8 lanes, every lane of the group ends with it: quad_perm
 // [1,0,3,2] (xor 1), quad_perm [2,3,0,1] (xor 2), then row_half_mirror (lane i of a half
 // row with lane 7 - i, which lies in the other quad).  Every lane has a source lane in these
@@ -191,6 +203,8 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
     for (int c = 0; c < 3; ++c) f2c += fmax(lo[c] * lo[c], hi[c] * hi[c]);
     inside = f2c * (1.0 + 1e-12) < a.rho2;
   }
+  // as a scalar, so that branches on it stay scalar compares
+  const int inside_s = __builtin_amdgcn_readfirstlane((int)inside);
   // The rows' candidate column intervals are computed in FP32 against a slab widened by
   // RI_TOL in coordinates and a support sphere whose rho^2 is widened by 1e-4 (1 + rho^2),
   // and then widened by RI_COLS columns.  A slab face F of axis c bounds the columns of row r
@@ -281,22 +295,24 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
       avf[c] = uniform_f32((float)av[c]);
       nf[c] = uniform_f32((float)nrm[c]);
     }
-    const bool anyzb = zb[0] || zb[1] || zb[2];
+    const int anyzb_s = __builtin_amdgcn_readfirstlane((int)(zb[0] || zb[1] || zb[2]));
     const float aaf = uniform_f32((float)aa), ibsf = uniform_f32((float)(1.0 / sqrt(bb)));
-    const float C0m1 = (float)(C0 - 1), C1p1 = (float)(C1 + 1);  // exact: |C| <= 2n
+    // exact: |C| <= 2n + 1
+    const float C0f = (float)C0, C1f = (float)C1, C0m1 = (float)(C0 - 1), C1p1 = (float)(C1 + 1);
     const float kab = uniform_f32((float)(-(av[0] * bv[0] + av[1] * bv[1] + av[2] * bv[2]) / bb));
     const float knb = uniform_f32((float)(-(nrm[0] * bv[0] + nrm[1] * bv[1] + nrm[2] * bv[2]) / bb));
     // Fixed-point steps of the cell coordinate q_c = (x_c + 1) h (h = n/2, 32 fraction bits)
     // per row (a_c h) and per column (b_c h); |a_c h|, |b_c h| <= 1/2, clamped into int32
     // (error at most 2^-32 each).  Eight columns for the column walk.
-    int Afx[3], Dfx[3];
+    int Afx[3], Dfx[3], nDfx[3];
     long long D8[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {  // wave-uniform: held in scalar registers
       Afx[c] = __builtin_amdgcn_readfirstlane(
           (int)fmin(fmax(rint(av[c] * h * 4294967296.0), -2147483648.0), 2147483647.0));
       Dfx[c] = __builtin_amdgcn_readfirstlane(
-          (int)fmin(fmax(rint(bv[c] * h * 4294967296.0), -2147483648.0), 2147483647.0));
+          (int)fmin(fmax(rint(bv[c] * h * 4294967296.0), -2147483647.0), 2147483647.0));
+      nDfx[c] = -Dfx[c];
       D8[c] = 8ll * Dfx[c];
     }
     // Lanes 0, 1, 2 form the fixed-point base of axes x, y, z at (row R0, column C0) for each
@@ -341,39 +357,45 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
           continue;
         }
         const int t = lane + 64 * half;
-        // branch-free except for the rare cases: every lane forms its row's bounds, and
-        // rows past the window or empty take the empty interval [C0, C0 - 1]
-        const float rf = (float)(R0 + t);  // exact: |row| <= 2n
+        // branch-free except for the rare cases: every lane forms its row's bounds.  A row
+        // past the window has an empty interval or columns whose samples all lie outside
+        // the brick (the window holds the box's rows), so it needs no mask: a walked sample
+        // is tested exactly whatever the interval.
+        const float rf = (float)(R0 + t);  // exact: |row| <= 2n + 130
         float tlo = fmaxf(fmaxf(fmaf(beta[0], rf, aL[0]), fmaf(beta[1], rf, aL[1])),
                           fmaf(beta[2], rf, aL[2]));
         float thi = fminf(fminf(fmaf(beta[0], rf, aU[0]), fmaf(beta[1], rf, aU[1])),
                           fmaf(beta[2], rf, aU[2]));
-        bool live_row = t < nrow;
-        if (anyzb) {  // rare: a basis component vanishes (structured directions)
+        bool dead = false;
+        if (anyzb_s) {  // rare: a basis component vanishes (structured directions)
 #pragma unroll
           for (int c = 0; c < 3; ++c) {
             const float w = fmaf(avf[c], rf, sf * nf[c]);
-            if (zb[c] && (w < lof[c] || w > hcf[c])) live_row = false;
+            if (zb[c] && (w < lof[c] || w > hcf[c])) dead = true;
           }
         }
         // nothing to clip when the whole brick lies inside the sphere
-        if (!inside) {
+        if (!inside_s) {
           const float u = fmaf(-aaf * rf, rf, rc2f);
-          live_row = live_row && u >= 0.0f;
+          dead = dead || u < 0.0f;
           // the hardware square root (1 ulp): far inside the slack
           const float hw = __builtin_amdgcn_sqrtf(fmaxf(u, 0.0f)) * ibsf;
           const float tc = fmaf(rf, kab, tcs);
-          tlo = fmaxf(tlo, tc - hw);
-          thi = fminf(thi, tc + hw);
+          // max and min as med3 against +-3e38 (fmaxf would first quiet its operands)
+          tlo = __builtin_amdgcn_fmed3f(tlo, tc - hw, 3.0e38f);
+          thi = __builtin_amdgcn_fmed3f(thi, tc + hw, -3.0e38f);
         }
-        live_row = live_row && tlo <= thi + 2.0f;
-        // clamped into [C0 - 1, C1 + 1] (one med3 each), so the conversions stay in range
-        const int rl = live_row ? max(C0, (int)ceilf(__builtin_amdgcn_fmed3f(tlo, C0m1, C1p1) - RI_COLS)) : C0;
-        const int rh = live_row ? min(C1, (int)floorf(__builtin_amdgcn_fmed3f(thi, C0m1, C1p1) + RI_COLS)) : C0 - 1;
-        // steps of 8 columns
-        const int nst = rh >= rl ? ((rh - rl + 8) >> 3) : 0;
-        // first and last column, 16 bits each (|i2| <= 2n < 2^15)
-        rowlo[wv][t] = (int)(((unsigned)rl & 0xffffu) | ((unsigned)rh << 16));
+        thi = dead ? -3.0e38f : thi;
+        // First and last column: rl = ceil of the lower bound less RI_COLS, clamped into
+        // [C0, C1 + 1], as its negation -rl = floor(-x) (one conversion); rh = floor of the
+        // upper bound plus RI_COLS, clamped into [C0 - 1, C1].  An empty row ends with rh < rl.
+        const int nrl = cvt_floor_neg(__builtin_amdgcn_fmed3f(tlo - RI_COLS, C0f, C1p1));
+        const int rh = cvt_floor(__builtin_amdgcn_fmed3f(thi + RI_COLS, C0m1, C1f));
+        // steps of 8 columns (rh - rl >= -(C1 - C0) - 2: the max keeps the shift's operand
+        // non-negative)
+        const int nst = max(rh + nrl + 8, 0) >> 3;
+        // -rl and rh, 16 bits each (|i2| <= 2n + 1 < 2^15)
+        rowlo[wv][t] = (int)__builtin_amdgcn_perm((unsigned)rh, (unsigned)nrl, 0x05040100u);
         // max over the 8 consecutive rows of a band (lanes 8b .. 8b+7)
         bsteps[half] = group8_max(nst);
         live[half] = __ballot(nst != 0);
@@ -491,7 +513,7 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
       // the brick test can reject a sample.
       float bs = 0.f;  // forward: the lane's FP32 sum over the section's samples
       // Forward: the lane's fixed-point coordinates at row R0 and column k_lane (Qb + (k_lane
-      // - C0) D); a band adds t A and rlo D.
+      // - C0) D); a band adds t A and rl D (as (-rl)(-D), the row table's form).
       long long Tb[3];
       if constexpr (!T) {
 #pragma unroll
@@ -502,8 +524,9 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
           asm volatile("" : "+v"(Tb[c]));
         }
       }
-      // Bands of 8 rows: lane (r, k) takes row 8k' + r of band k' and the columns rlo + k +
-      // 8m.  Transpose mode takes the columns rlo + dc + 2k + 16m in two phases dc = 0, 1:
+      // Bands of 8 rows: lane (r, k) takes row 8k' + r of band k' and the columns rl + k + 8m
+      // (rl the row's first column).  Transpose mode takes the columns rl + dc + 2k + 16m in
+      // two phases dc = 0, 1:
       // the lanes of one instruction then lie 2 columns apart and fewer of them add to one
       // cell at once (the LDS integer atomic serialises those, which costs time, not
       // results).  Measured at 256^3: 2.66 s against 2.92 s for the forward's columns and
@@ -511,7 +534,7 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
       int rlo_next = rowlo[wv][8 * k_first + r_lane];
       for (int k = k_first; k <= k_last; ++k) {
         const int t = 8 * k + r_lane;  // this lane's row within the window
-        const int rlo = (rlo_next << 16) >> 16;
+        const int nrl = (rlo_next << 16) >> 16;  // minus the band row's first column
         if (k < k_last) rlo_next = rowlo[wv][t + 8];
         const int steps = (k < 8) ? __builtin_amdgcn_readlane(bsteps[0], 8 * k)
                                   : __builtin_amdgcn_readlane(bsteps[1], 8 * (k - 8));
@@ -520,7 +543,7 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
         if constexpr (T) {
           const int steps2 = (steps + 1) >> 1;  // 16-column steps cover the 8-column ones
           for (int dcol = 0; dcol < 2; ++dcol) {
-            const int c0 = rlo + dcol + 2 * k_lane;  // first column i2 of the lane
+            const int c0 = dcol + 2 * k_lane - nrl;  // first column i2 of the lane
             int tq = t;  // as in the forward walk below
             asm volatile("" : "+v"(tq));
             long long Q[3];
@@ -534,7 +557,7 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
             }
           }
         } else {
-          const int c0 = rlo + k_lane;  // first column i2 of the lane
+          const int c0 = k_lane - nrl;  // first column i2 of the lane
           // t passes through an empty asm so that its products stay signed 32 x 32 -> 64
           // multiply-adds (one instruction each) instead of the two unsigned ones the
           // compiler picks for a known non-negative factor
@@ -543,7 +566,7 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
           long long Q[3];
 #pragma unroll
           for (int c = 0; c < 3; ++c)
-            Q[c] = Tb[c] + (long long)tq * Afx[c] + (long long)rlo * Dfx[c];
+            Q[c] = Tb[c] + (long long)tq * Afx[c] + (long long)nrl * nDfx[c];
           for (int m = 0; m < steps; ++m) {
             sample(Q, row, c0, 8, m, bs);
 #pragma unroll
