@@ -206,26 +206,29 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
   // as a scalar, so that branches on it stay scalar compares
   const int inside_s = __builtin_amdgcn_readfirstlane((int)inside);
   // The rows' candidate column intervals are computed in FP32 against a slab widened by
-  // RI_TOL in coordinates and a support sphere whose rho^2 is widened by 1e-4 (1 + rho^2),
-  // and then widened by RI_COLS columns.  A slab face F of axis c bounds the columns of row r
-  // at (F - (s n_c + a_c r)) / b_c, evaluated as kF - s kn + beta r with the per-direction
-  // products kF = F / b_c, kn = n_c / b_c, beta = -a_c / b_c rounded to FP32: each term's
-  // rounding, times |b_c|, is a coordinate error of at most 2^-24 times |F|, |s n_c| or
-  // |a_c r| (all below 3), so together under 6e-7 < RI_TOL whatever b_c, and the bound's own
-  // rounding (|t| <= 4n + 2, n <= 4096) moves it by under 5e-3 < RI_COLS columns.  The
-  // sphere |p|^2 <= rho^2 meets row r (p = s n + a r + b t, with n, a, b orthogonal) in the
-  // columns |t - t_c| <= sqrt(rho^2 - s^2 - |a|^2 r^2) / |b|, t_c = -(s n.b + r a.b) / |b|^2
-  // (zero up to the rounding of the basis).  The intervals therefore hold every column whose
-  // sample can fall in the brick and the sphere; which samples count is decided per sample,
-  // exactly.
-  constexpr float RI_TOL = 4e-6f, RI_COLS = 1.0f / 64.0f;
+  // RI_TOL in coordinates and a support sphere whose rho^2 (<= 3) is widened by
+  // 1e-4 (1 + rho^2).  Errors are measured in coordinates: a column error times |b_c| for a
+  // slab bound, times |b| for the sphere's, both at most 1/n.  A slab face F of axis c bounds
+  // the columns of row r at (F - (s n_c + a_c r)) / b_c, evaluated as kF - s kn + beta r with
+  // the per-direction products kF = F / b_c, kn = n_c / b_c, beta = -a_c / b_c rounded to FP32
+  // and two FMAs: each rounding is at most 2^-24 times one of |F|, |s n_c|, |a_c r|,
+  // |F - s n_c| or the bound's own |t b_c| (|t| <= 4n + 2 within the clamps below), all at
+  // most 6, so together under 2e-6 < RI_TOL whatever b_c.  The sphere |p|^2 <= rho^2 meets
+  // row r (p = s n + a r + b t, with n, a, b orthogonal) in the columns |t - t_c| <= sqrt(u)
+  // / |b|, u = rho^2 - s^2 - |a|^2 r^2, t_c = -(s n.b + r a.b) / |b|^2 (zero up to the
+  // rounding of the basis).  Where u >= 0, FP32 moves u by under 2e-6, much less than the
+  // widening, which therefore lengthens the half-width by at least (1e-4 - 2e-6) /
+  // (2 sqrt(3.001)) > 2.5e-5, beyond every rounding of t_c, of the half-width and of the ends
+  // (each under 2e-6).  The intervals therefore hold every column whose sample can fall in
+  // the brick and the sphere; which samples count is decided per sample, exactly.
+  constexpr float RI_TOL = 4e-6f;
   float lof[3], hcf[3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     lof[c] = uniform_f32((float)lo[c] - RI_TOL);
     hcf[c] = uniform_f32((float)fmin(hi[c], 1.0) + RI_TOL);
   }
-  const double rho2w = a.rho2 + 1e-4 * (1.0 + a.rho2);
+  const float rho2wf = uniform_f32((float)(a.rho2 + 1e-4 * (1.0 + a.rho2)));
 
   const int d_end = min((grp + 1) * a.dgroup, a.ndir);
   // waves take the group's directions one at a time from a shared counter, so that they
@@ -342,7 +345,11 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
         aL[c] = fmaf(-sf, kn[c], kL[c]);
         aU[c] = fmaf(-sf, kn[c], kU[c]);
       }
-      const float rc2f = (float)(rho2w - s * s), tcs = sf * knb;
+      const float rc2f = fmaf(-sf, sf, rho2wf), tcs = sf * knb;
+      // the flags re-read as scalars in the loop, so that their tests stay scalar compares
+      // (hoisted out of the loop, the compiler keeps them as lane masks rebuilt per section)
+      int ins = inside_s, azb = anyzb_s;
+      asm volatile("" : "+s"(ins), "+s"(azb));
       // Candidate column interval of every row R0 + t of the window (t = lane, lane + 64):
       // first and last column into the wave's row table, the 8-column step count reduced to
       // a per-band maximum (bands of the rows 8k .. 8k + 7 of the window), and the rows that
@@ -352,7 +359,7 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
 #pragma unroll
       for (int half = 0; half < 2; ++half) {
         if (64 * half >= nrow) {  // uniform: the window has at most 64 rows
-          bsteps[half] = 0;
+          bsteps[half] = -1;
           live[half] = 0ull;
           continue;
         }
@@ -366,18 +373,19 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
                           fmaf(beta[2], rf, aL[2]));
         float thi = fminf(fminf(fmaf(beta[0], rf, aU[0]), fmaf(beta[1], rf, aU[1])),
                           fmaf(beta[2], rf, aU[2]));
-        bool dead = false;
-        if (anyzb_s) {  // rare: a basis component vanishes (structured directions)
+        if (azb) {  // rare: a basis component vanishes (structured directions)
+          bool dead = false;
 #pragma unroll
           for (int c = 0; c < 3; ++c) {
             const float w = fmaf(avf[c], rf, sf * nf[c]);
             if (zb[c] && (w < lof[c] || w > hcf[c])) dead = true;
           }
+          if (dead) thi = -3.0e38f;
         }
-        // nothing to clip when the whole brick lies inside the sphere
-        if (!inside_s) {
+        // Nothing to clip when the whole brick lies inside the sphere.  A row that misses the
+        // sphere (u < 0) gets the half-width 0: at most the one column at t_c, harmlessly.
+        if (!ins) {
           const float u = fmaf(-aaf * rf, rf, rc2f);
-          dead = dead || u < 0.0f;
           // the hardware square root (1 ulp): far inside the slack
           const float hw = __builtin_amdgcn_sqrtf(fmaxf(u, 0.0f)) * ibsf;
           const float tc = fmaf(rf, kab, tcs);
@@ -385,20 +393,20 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
           tlo = __builtin_amdgcn_fmed3f(tlo, tc - hw, 3.0e38f);
           thi = __builtin_amdgcn_fmed3f(thi, tc + hw, -3.0e38f);
         }
-        thi = dead ? -3.0e38f : thi;
-        // First and last column: rl = ceil of the lower bound less RI_COLS, clamped into
-        // [C0, C1 + 1], as its negation -rl = floor(-x) (one conversion); rh = floor of the
-        // upper bound plus RI_COLS, clamped into [C0 - 1, C1].  An empty row ends with rh < rl.
-        const int nrl = cvt_floor_neg(__builtin_amdgcn_fmed3f(tlo - RI_COLS, C0f, C1p1));
-        const int rh = cvt_floor(__builtin_amdgcn_fmed3f(thi + RI_COLS, C0m1, C1f));
-        // steps of 8 columns (rh - rl >= -(C1 - C0) - 2: the max keeps the shift's operand
-        // non-negative)
-        const int nst = max(rh + nrl + 8, 0) >> 3;
+        // First and last column: rl = ceil of the lower bound clamped into [C0, C1 + 1], as
+        // its negation -rl = floor(-x) (one conversion); rh = floor of the upper bound clamped
+        // into [C0 - 1, C1].  An empty row ends with rh < rl.
+        const int nrl = cvt_floor_neg(__builtin_amdgcn_fmed3f(tlo, C0f, C1p1));
+        const int rh = cvt_floor(__builtin_amdgcn_fmed3f(thi, C0m1, C1f));
+        // rh - rl, and the row's steps of 8 columns less one, (rh - rl) >> 3: negative for an
+        // empty row (the bands add the one back)
+        const int span = rh + nrl;
+        const int nst = span >> 3;
         // -rl and rh, 16 bits each (|i2| <= 2n + 1 < 2^15)
         rowlo[wv][t] = (int)__builtin_amdgcn_perm((unsigned)rh, (unsigned)nrl, 0x05040100u);
         // max over the 8 consecutive rows of a band (lanes 8b .. 8b+7)
         bsteps[half] = group8_max(nst);
-        live[half] = __ballot(nst != 0);
+        live[half] = __ballot(span >= 0);
       }
       if ((live[0] | live[1]) == 0ull) continue;  // no row meets the brick and the sphere
       // the bands that hold the first and the last live row
@@ -536,9 +544,9 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
         const int t = 8 * k + r_lane;  // this lane's row within the window
         const int nrl = (rlo_next << 16) >> 16;  // minus the band row's first column
         if (k < k_last) rlo_next = rowlo[wv][t + 8];
-        const int steps = (k < 8) ? __builtin_amdgcn_readlane(bsteps[0], 8 * k)
-                                  : __builtin_amdgcn_readlane(bsteps[1], 8 * (k - 8));
-        if (steps == 0) continue;
+        const int steps = 1 + ((k < 8) ? __builtin_amdgcn_readlane(bsteps[0], 8 * k)
+                                       : __builtin_amdgcn_readlane(bsteps[1], 8 * (k - 8)));
+        if (steps <= 0) continue;
         const int row = R0 + t;
         if constexpr (T) {
           const int steps2 = (steps + 1) >> 1;  // 16-column steps cover the 8-column ones
