@@ -413,6 +413,9 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
       const int t_first = live[0] ? __builtin_ctzll(live[0]) : 64 + __builtin_ctzll(live[1]);
       const int t_last = live[1] ? 127 - __builtin_clzll(live[1]) : 63 - __builtin_clzll(live[0]);
       const int k_first = t_first >> 3, k_last = t_last >> 3;
+      // both halves' band maxima in one register: band b of half 0 at lane 8b, of half 1 at
+      // lane 8b + 1 (a band's 8 lanes hold the same maximum)
+      const int bsteps01 = (lane & 7) == 1 ? bsteps[1] : bsteps[0];
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -443,8 +446,8 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
           Qb[c] = (long long)((((unsigned long long)hi32) << 32) | lo32) + 1024ll;
         }
       }
-      // One sample: Q the lane's fixed-point coordinates, row its row and c0 + step*m its
-      // column (for the exact decision).  Forward: adds the cell's value to bs; transpose:
+      // One sample: Q the lane's fixed-point coordinates, R0 + t its row and cb - nrl + step*m
+      // its column (for the exact decision).  Forward: adds the cell's value to bs; transpose:
       // adds the section's quantized value to the cell.
       // The cell of brick offsets (lx, ly, lz): all three in [0, 32) (z: [0, 16) in transpose
       // mode, tested as 2*lz < 32; a negative offset sets high bits of the OR).  ok and the
@@ -459,7 +462,8 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
           off = ok ? off : 0u;
         }
       };
-      auto sample = [&](const long long (&Q)[3], int row, int c0, int step, int m, float& bs) {
+      auto sample = [&](const long long (&Q)[3], int t, int cb, int nrl, int step, int m,
+                        float& bs) {
         const unsigned u = min(min((unsigned)Q[0], (unsigned)Q[1]), (unsigned)Q[2]);
         // the fixed point's cell (away from a boundary the fixed point floors where the
         // reference truncates: they differ only for -1 < q < 0, x < -1, where the floor's
@@ -471,13 +475,13 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
           // ... replaced near a cell boundary (fraction of Q below 2^-21) by the reference's
           // rounding, in binary64 (the per-column products b*i2 rounded as the reference
           // rounds delta*v2*i2); the step index through readfirstlane keeps the column out
-          // of the common path
-          const double f2 = (double)(c0 + step * __builtin_amdgcn_readfirstlane(m));
-          // the row through an empty asm, so that the compiler does not hoist the row's
-          // binary64 terms below out of the sample loop into every band's setup
-          int rowv = row;
-          asm volatile("" : "+v"(rowv));
-          const double f1 = (double)rowv;
+          // of the common path.  Row and column are formed here from t and nrl passed through
+          // an empty asm, so that the compiler does not hoist them, or the row's binary64
+          // terms below, out of the sample loop into every band's setup.
+          int tv = t, nv = nrl;
+          asm volatile("" : "+v"(tv), "+v"(nv));
+          const double f2 = (double)(cb - nv + step * __builtin_amdgcn_readfirstlane(m));
+          const double f1 = (double)(R0 + tv);
           const double Rx = s * nrm[0] + av[0] * f1, Ry = s * nrm[1] + av[1] * f1,
                        Rz = s * nrm[2] + av[2] * f1;
           const double x = Rx + bv[0] * f2, y = Ry + bv[1] * f2, z = Rz + bv[2] * f2;
@@ -544,14 +548,12 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
         const int t = 8 * k + r_lane;  // this lane's row within the window
         const int nrl = (rlo_next << 16) >> 16;  // minus the band row's first column
         if (k < k_last) rlo_next = rowlo[wv][t + 8];
-        const int steps = 1 + ((k < 8) ? __builtin_amdgcn_readlane(bsteps[0], 8 * k)
-                                       : __builtin_amdgcn_readlane(bsteps[1], 8 * (k - 8)));
+        const int steps = 1 + __builtin_amdgcn_readlane(bsteps01, 8 * (k & 7) + (k >> 3));
         if (steps <= 0) continue;
-        const int row = R0 + t;
         if constexpr (T) {
           const int steps2 = (steps + 1) >> 1;  // 16-column steps cover the 8-column ones
           for (int dcol = 0; dcol < 2; ++dcol) {
-            const int c0 = dcol + 2 * k_lane - nrl;  // first column i2 of the lane
+            const int cb = dcol + 2 * k_lane, c0 = cb - nrl;  // c0: first column i2 of the lane
             int tq = t;  // as in the forward walk below
             asm volatile("" : "+v"(tq));
             long long Q[3];
@@ -559,13 +561,12 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
             for (int c = 0; c < 3; ++c)
               Q[c] = Qb[c] + (long long)tq * Afx[c] + (long long)(c0 - C0) * Dfx[c];
             for (int m = 0; m < steps2; ++m) {
-              sample(Q, row, c0, 16, m, bs);
+              sample(Q, t, cb, nrl, 16, m, bs);
 #pragma unroll
               for (int c = 0; c < 3; ++c) Q[c] += 2 * D8[c];
             }
           }
         } else {
-          const int c0 = k_lane - nrl;  // first column i2 of the lane
           // t passes through an empty asm so that its products stay signed 32 x 32 -> 64
           // multiply-adds (one instruction each) instead of the two unsigned ones the
           // compiler picks for a known non-negative factor
@@ -576,7 +577,7 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
           for (int c = 0; c < 3; ++c)
             Q[c] = Tb[c] + (long long)tq * Afx[c] + (long long)nrl * nDfx[c];
           for (int m = 0; m < steps; ++m) {
-            sample(Q, row, c0, 8, m, bs);
+            sample(Q, t, k_lane, nrl, 8, m, bs);
 #pragma unroll
             for (int c = 0; c < 3; ++c) Q[c] += D8[c];
           }
