@@ -346,12 +346,18 @@ int wrt_batch_dgroup(wrt_ctx* ctx);
  * operation, a cube edge e and one of `ndatasets` seeded datasets of that (op, e), on the
  * grids (e/2) phi x (e/4) GL theta x (e+1) shifts (REDUCTION: rays on e z-slices with the
  * same phi and shift grids).  RADON and ADJOINT run in FP32, REDUCTION in FP64.
- * Admission: a request's dataset becomes resident (job created and loaded) when its device
- * bytes fit the HBM budget, evicting idle resident datasets least recently used first; a
- * resident dataset serves later requests without reloading.  Up to max_inflight requests
- * run at once, each on its dataset's own stream, so requests that arrive while others are
- * on the GPU start as soon as a slot and memory are free and overlap with them.
- * Latency = completion - arrival, on the host's monotonic clock. */
+ * Admission: a request's dataset becomes resident (job created and loaded) when the free
+ * device memory hipMemGetInfo reports at that moment holds its bytes plus 1 GiB, and the
+ * optional budget allows, evicting idle resident datasets least recently used first; a
+ * resident dataset serves later requests without reloading.  The fast requests (RADON and
+ * ADJOINT in FP32) are batched continuously: their tasks (bricks x direction groups, voxel
+ * blocks) go into rounds of a few milliseconds of one persistent kernel, small requests
+ * whole and large ones in chunks, so a request arriving while a large one runs starts
+ * within a round or two; rounds alternate between two streams so that one fills the CUs the
+ * other frees.  The others (REDUCTION, and every request in plain mode) run on their
+ * dataset's own stream, up to max_inflight at once.  A request whose dataset serves an
+ * earlier request waits for it.  Latency = completion - arrival, on the host's monotonic
+ * clock. */
 typedef struct wrt_serve_cfg {
   double qps;          /* mean arrival rate (1/s); <= 0: every request arrives at t = 0 */
   int nreq;            /* number of requests */
@@ -361,7 +367,7 @@ typedef struct wrt_serve_cfg {
   int nedges;
   int ndatasets;       /* distinct datasets per (op, edge), >= 1 */
   size_t hbm_budget;   /* device bytes for resident datasets; 0: 90% of the free HBM */
-  int max_inflight;    /* concurrent requests on the device, >= 1 */
+  int max_inflight;    /* concurrent requests on their own streams (not batched), >= 1 */
   int plain;           /* 1: every request runs the plain FP64 form, one at a time (the
                           per-request baseline the served form is compared with) */
 } wrt_serve_cfg;
@@ -382,6 +388,8 @@ typedef struct wrt_serve_report {
   double samples, samples_per_s; /* logical samples of all requests, per wall second */
   int loads, evictions, peak_inflight;
   size_t peak_bytes;             /* largest device bytes held by resident datasets */
+  int batched;                   /* requests served through the rounds */
+  int rounds;                    /* rounds launched */
 } wrt_serve_report;
 
 /* request i of the stream cfg describes (deterministic in cfg) */

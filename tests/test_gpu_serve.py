@@ -1,7 +1,8 @@
 """GPU tests of the serve loop: every request's result is the same in the mix as alone
 (FP32 forms: relative L2 <= 1e-6, the FP64 atomics' summation order being the only
 difference; FP64 reduction: bitwise), admission under a small HBM budget evicts and still
-serves every request, and the latency report is consistent."""
+serves every request, the latency report is consistent, and the continuous batching runs a
+small request beside a large one that arrived before it."""
 import numpy as np
 import pytest
 
@@ -58,3 +59,21 @@ def test_serve_admission_evicts_under_budget(ctx):
     assert rep["evictions"] > 0 and rep["peak_bytes"] <= budget
     assert 0 < rep["p50_ms"] <= rep["p99_ms"] <= rep["max_ms"]
     assert abs(rep["p99_ms"] - np.sort(lat)[int(np.ceil(0.99 * 24)) - 1]) < 1e-9
+
+
+def test_serve_rounds_interleave_small_with_large(ctx):
+    """Continuous batching: a 64^3 plane-integral request queued right behind a 256^3 one (both
+    at t = 0, the large one first) runs in the rounds beside it and finishes long before it,
+    where a first-come, first-served queue would finish it after; both equal their alone
+    runs."""
+    seed = next(sd for sd in range(1, 400)
+                if [wrt.serve_describe(wrt.serve_cfg(qps=0.0, nreq=2, seed=sd, edges=(64, 256),
+                                                     mix=(1.0, 0.0, 0.0)), i).edge
+                    for i in range(2)] == [256, 64])
+    cfg = wrt.serve_cfg(qps=0.0, nreq=2, seed=seed, edges=(64, 256), mix=(1.0, 0.0, 0.0))
+    rep, lat, outs = ctx.serve(cfg, keep=range(2))
+    assert rep["completed"] == 2 and rep["batched"] == 2 and rep["rounds"] >= 2
+    assert lat[1] < 0.5 * lat[0], lat
+    for i in range(2):
+        want = _alone(ctx, wrt.serve_describe(cfg, i))
+        assert orc.rel_l2(outs[i], want) <= 1e-6, i

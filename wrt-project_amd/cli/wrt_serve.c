@@ -87,10 +87,11 @@ int main(int argc, char** argv) {
   }
   printf("{\"requests\": %d, \"form\": \"%s\", \"qps\": %g, \"wall_s\": %.4f, \"p50_ms\": %.3f, \"p99_ms\": %.3f, "
          "\"mean_ms\": %.3f, \"max_ms\": %.3f, \"samples\": %.6g, \"samples_per_s\": %.6g, "
-         "\"loads\": %d, \"evictions\": %d, \"peak_inflight\": %d, \"peak_bytes\": %zu}\n",
+         "\"loads\": %d, \"evictions\": %d, \"peak_inflight\": %d, \"peak_bytes\": %zu, "
+         "\"batched\": %d, \"rounds\": %d}\n",
          rep.completed, cfg.plain ? "plain f64, one at a time" : "served", cfg.qps, rep.wall_s, rep.p50_ms, rep.p99_ms, rep.mean_ms, rep.max_ms,
          rep.samples, rep.samples_per_s, rep.loads, rep.evictions, rep.peak_inflight,
-         rep.peak_bytes);
+         rep.peak_bytes, rep.batched, rep.rounds);
   wrt_free(ctx);
   return EXIT_SUCCESS;
 }

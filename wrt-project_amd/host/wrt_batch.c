@@ -28,12 +28,6 @@
     if ((x) != hipSuccess) return WRT_E_HIP; \
   } while (0)
 
-/* device time estimates (CU-seconds), measured on MI355X: a full 32^3 brick against one
- * direction over a shift grid of spacing 2/n, and one (voxel, direction) of the FP32
- * backprojection.  Only their ratio and the ordering they induce matter. */
-#define COST_BRICK_DIR 26.6e-6
-#define COST_VOXEL_DIR 2.1e-10
-
 typedef struct {
   int njobs;
   wrt_job** jobs;
@@ -86,59 +80,56 @@ static int batchable(const wrt_job* j) {
          (j->d.op == WRT_OP_RADON || (j->d.op == WRT_OP_ADJOINT && j->d.n <= 1024));
 }
 
-typedef struct {
-  double cost;
-  wrt_batch_task t;
-} costed_task;
-
-static int by_cost_desc(const void* a, const void* b) {
-  const double x = ((const costed_task*)a)->cost, y = ((const costed_task*)b)->cost;
+int wrt_costed_desc(const void* a, const void* b) {
+  const double x = ((const wrt_costed_task*)a)->cost, y = ((const wrt_costed_task*)b)->cost;
   return x < y ? 1 : (x > y ? -1 : 0);
+}
+
+long long wrt_item_tasks(const wrt_fast_desc* fd, int item, int G, wrt_costed_task* ct) {
+  const wrt_batch_item* it = &fd->item;
+  if (it->kind == WRT_ITEM_RADON) {
+    const int ndir = it->ra.ndir, n = it->ra.n;
+    const long long count = (long long)fd->nbrick * ((ndir + G - 1) / G);
+    if (!ct) return count;
+    const double shifts = (double)(it->ra.nshift - 1) / n; /* shift density vs 2/n */
+    long long k = 0;
+    for (int b = 0; b < fd->nbrick; ++b)
+      for (int g = 0; g * G < ndir; ++g) {
+        const int nd = (g + 1) * G <= ndir ? G : ndir - g * G;
+        ct[k].cost = WRT_COST_BRICK_DIR * nd * shifts * (fd->weight[b] / 32768.0);
+        ct[k].t.item = item;
+        ct[k].t.a = fd->brick[b];
+        ct[k].t.b = g;
+        ct[k].t.c = fd->bnd[b];
+        ++k;
+      }
+    return k;
+  }
+  const long long count = (fd->nvox + 2047) / 2048;
+  if (!ct) return count;
+  long long k = 0;
+  for (long long v = 0; v < fd->nvox; v += 2048) {
+    const int cnt = (int)(fd->nvox - v < 2048 ? fd->nvox - v : 2048);
+    ct[k].cost = WRT_COST_VOXEL_DIR * cnt * it->andir;
+    ct[k].t.item = item;
+    ct[k].t.a = (int)v;
+    ct[k].t.b = cnt;
+    ct[k].t.c = 0;
+    ++k;
+  }
+  return k;
 }
 
 /* the task list for direction groups of G, longest first; returns the count or < 0 */
 static int build_tasks(const wrt_fast_desc* fd, int nitems, int G, wrt_batch_task** out) {
   long long total = 0;
-  for (int i = 0; i < nitems; ++i) {
-    if (fd[i].item.kind == WRT_ITEM_RADON) {
-      const int ndir = fd[i].item.ra.ndir;
-      total += (long long)fd[i].nbrick * ((ndir + G - 1) / G);
-    } else {
-      total += (fd[i].nvox + 2047) / 2048;
-    }
-  }
+  for (int i = 0; i < nitems; ++i) total += wrt_item_tasks(&fd[i], i, G, NULL);
   if (total > 0x7fffffff) return WRT_E_ARG;
-  costed_task* ct = (costed_task*)malloc(sizeof(costed_task) * (size_t)(total + 1));
+  wrt_costed_task* ct = (wrt_costed_task*)malloc(sizeof(wrt_costed_task) * (size_t)(total + 1));
   if (!ct) return WRT_E_NOMEM;
   long long k = 0;
-  for (int i = 0; i < nitems; ++i) {
-    const wrt_batch_item* it = &fd[i].item;
-    if (it->kind == WRT_ITEM_RADON) {
-      const int ndir = it->ra.ndir, n = it->ra.n;
-      const double shifts = (double)(it->ra.nshift - 1) / n; /* shift density vs 2/n */
-      for (int b = 0; b < fd[i].nbrick; ++b)
-        for (int g = 0; g * G < ndir; ++g) {
-          const int nd = (g + 1) * G <= ndir ? G : ndir - g * G;
-          ct[k].cost = COST_BRICK_DIR * nd * shifts * (fd[i].weight[b] / 32768.0);
-          ct[k].t.item = i;
-          ct[k].t.a = fd[i].brick[b];
-          ct[k].t.b = g;
-          ct[k].t.c = fd[i].bnd[b];
-          ++k;
-        }
-    } else {
-      for (long long v = 0; v < fd[i].nvox; v += 2048) {
-        const int cnt = (int)(fd[i].nvox - v < 2048 ? fd[i].nvox - v : 2048);
-        ct[k].cost = COST_VOXEL_DIR * cnt * it->andir;
-        ct[k].t.item = i;
-        ct[k].t.a = (int)v;
-        ct[k].t.b = cnt;
-        ct[k].t.c = 0;
-        ++k;
-      }
-    }
-  }
-  qsort(ct, (size_t)k, sizeof(costed_task), by_cost_desc);
+  for (int i = 0; i < nitems; ++i) k += wrt_item_tasks(&fd[i], i, G, ct + k);
+  qsort(ct, (size_t)k, sizeof(wrt_costed_task), wrt_costed_desc);
   wrt_batch_task* t = (wrt_batch_task*)malloc(sizeof(wrt_batch_task) * (size_t)(k + 1));
   if (!t) {
     free(ct);

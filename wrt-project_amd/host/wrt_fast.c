@@ -942,8 +942,23 @@ int wrt_fast_fetch(wrt_job* j, double* out) {
   }
 }
 
+/* the FP32 plane values of a fast radon job from its FP64 accumulators, i.e. the rounding
+ * the per-job launch does on the device, done here for the rounds of the serve loop (which
+ * leave the FP32 buffer alone) */
+int wrt_fast_fetch_acc(wrt_job* j, double* out) {
+  if (!j || !j->fast || !out || j->d.op != WRT_OP_RADON) return WRT_E_ARG;
+  const wrt_job_desc* d = &j->d;
+  const size_t count = (size_t)d->nshift * d->nphi * d->ntheta;
+  HIPCHK(hipMemcpyAsync(out, ((radon_fast*)j->fast)->acc, count * sizeof(double),
+                        hipMemcpyDeviceToHost, j->st));
+  HIPCHK(hipStreamSynchronize(j->st));
+  for (size_t i = 0; i < count; ++i) out[i] = (double)(float)out[i];
+  return WRT_OK;
+}
+
 /* a loaded fast job as an item of the persistent batch launch (ra.dgroup left to the
- * planner), with the work the planner splits into tasks */
+ * planner; fd->dgroup is the per-job launch's choice), with the work the planner splits
+ * into tasks */
 int wrt_fast_describe(wrt_job* j, wrt_fast_desc* fd) {
   if (!j || !j->fast || !fd) return WRT_E_ARG;
   const wrt_job_desc* d = &j->d;
@@ -971,6 +986,7 @@ int wrt_fast_describe(wrt_job* j, wrt_fast_desc* fd) {
     fd->weight = f->ne_count;
     fd->bnd = f->ne_bnd;
     fd->nbrick = f->n_ne;
+    fd->dgroup = f->dgroup;
     fd->acc = f->acc;
     fd->out = f->out;
     fd->nout = (long long)d->nshift * d->nphi * d->ntheta;

@@ -71,9 +71,29 @@ typedef struct {
   float* out;
   long long nout;
   long long nvox;             /* adjoint: n^3 */
+  int dgroup;                 /* radon: direction group of the per-job launch */
 } wrt_fast_desc;
 int wrt_fast_supported(const wrt_job_desc* d);
 int wrt_fast_describe(wrt_job* j, wrt_fast_desc* fd);
+/* radon: the FP64 accumulators rounded to FP32, as doubles (the form's output) */
+int wrt_fast_fetch_acc(wrt_job* j, double* out);
+
+/* A task of the persistent launch with its estimated device time in CU-seconds: a full
+ * 32^3 brick against one direction over a shift grid of spacing 2/n, and one (voxel,
+ * direction) of the FP32 backprojection, measured on MI355X (a 256^3 plane-integral pass,
+ * a 256^3 backprojection of 8192 directions).  Only their ratio, the order they induce and
+ * the rough length of a serve round depend on them. */
+#define WRT_COST_BRICK_DIR 20e-6
+#define WRT_COST_VOXEL_DIR 1.1e-10
+typedef struct {
+  double cost;
+  wrt_batch_task t;
+} wrt_costed_task;
+/* the tasks of item fd (index `item` in its launch; radon: direction groups of G): with
+ * ct NULL the count, else written to ct (that many entries), heaviest brick first */
+long long wrt_item_tasks(const wrt_fast_desc* fd, int item, int G, wrt_costed_task* ct);
+/* order by falling cost (qsort) */
+int wrt_costed_desc(const void* a, const void* b);
 
 /* host-side tables (C library sin/cos, the reference's bits) */
 void wrt_host_radon_dirs(int n, const double* phi, int nphi, const double* theta, int ntheta,

@@ -62,7 +62,8 @@ class ServeReport(C.Structure):
     _fields_ = [("completed", C.c_int), ("wall_s", C.c_double), ("p50_ms", C.c_double),
                 ("p99_ms", C.c_double), ("mean_ms", C.c_double), ("max_ms", C.c_double),
                 ("samples", C.c_double), ("samples_per_s", C.c_double), ("loads", C.c_int),
-                ("evictions", C.c_int), ("peak_inflight", C.c_int), ("peak_bytes", C.c_size_t)]
+                ("evictions", C.c_int), ("peak_inflight", C.c_int), ("peak_bytes", C.c_size_t),
+                ("batched", C.c_int), ("rounds", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
