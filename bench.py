@@ -40,8 +40,14 @@ restatement on the host cores; "fourier" is the Chang-type Fourier-slice inversi
 32 theta x 129 shifts -> 64^3, FP64) with its rate against the FP64 peak; "programs" runs the radon_grid drop-in program end to end at
 256^3 with text files and with the binary format.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1]
-    python bench.py --traffic-probe     (run under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--band-rows R --dgroup G]
+    python bench.py --traffic-probe --band-rows R --dgroup G
+                                        (run under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)
+
+The brick kernel's tile shape (rows per band, directions per task) is autotuned by the
+library at the job's first launch, which the bench runs before its warmup; the bench line
+reports it in config.band_rows / config.dgroup, and the profiling runs fix it to those values
+so that their counters hold no tuning pass.
 
 The roofline's "traffic" (HBM bytes per pass) comes from those two counter runs: the probe
 runs a plain conversion kernel of known size (reads 8 B, writes 4 B per element) and then
@@ -227,6 +233,9 @@ def batch_leg(ctx, wrt, nvol, seed=3):
         jobs.append(j)
     st0 = ctx.batch_run(jobs)  # plan, autotune, capture
     st = ctx.batch_run(jobs)   # graph replay
+    dgroup = ctx.batch_dgroup()
+    for j in jobs:             # the per-job launches' own tuning runs outside the timing
+        j.run()
     per_job = sum(j.run()["seconds"] for j in jobs)
     for j in jobs:
         j.close()
@@ -236,7 +245,7 @@ def batch_leg(ctx, wrt, nvol, seed=3):
         j.load(gen())
         plain += j.run()["seconds"]
         j.close()
-    return {"volumes": nvol, "jobs": len(jobs), "edges": edges, "dgroup": ctx.batch_dgroup(),
+    return {"volumes": nvol, "jobs": len(jobs), "edges": edges, "dgroup": dgroup,
             "first_call_s": st0["kernel_seconds"], "graph_s": st["kernel_seconds"],
             "samples": st["logical_samples"],
             "samples_per_s": st["logical_samples"] / st["kernel_seconds"],
@@ -530,9 +539,14 @@ CALIB_COUNT = 1 << 27  # elements of the calibration conversion: 1 GiB read, 512
 
 
 def traffic_probe(args):
-    """Calibration kernel of known size, then one pass of the workload (for rocprofv3 --pmc)."""
+    """Calibration kernel of known size, then one pass of the workload (for rocprofv3 --pmc),
+    with the tile shape fixed (--band-rows, --dgroup: the values the bench reports), so that no
+    tuning pass adds to the counters."""
     import ctypes as C
 
+    if not (args.band_rows and args.dgroup):
+        sys.exit("bench.py --traffic-probe needs --band-rows and --dgroup (the tile shape the "
+                 "bench line reports in config)")
     import wrt
 
     hip = C.CDLL("libamdhip64.so")
@@ -630,7 +644,16 @@ def main():
     ap.add_argument("--no-programs", action="store_true")
     ap.add_argument("--batch-volumes", type=int, default=48)
     ap.add_argument("--traffic-probe", action="store_true")
+    ap.add_argument("--band-rows", type=int, default=0,
+                    help="fix the brick kernel's rows per band (4, 8, 16) instead of tuning it "
+                         "(profiling runs pass the tuned value the bench reports)")
+    ap.add_argument("--dgroup", type=int, default=0,
+                    help="fix the brick kernel's directions per task instead of tuning it")
     args = ap.parse_args()
+    if args.band_rows:
+        os.environ["WRT_BAND_ROWS"] = str(args.band_rows)
+    if args.dgroup:
+        os.environ["WRT_RADON_DGROUP"] = str(args.dgroup)
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(the path is built and measured on a single GPU; multi-GPU sharding is a "
@@ -653,6 +676,11 @@ def main():
 
     job = ctx.job("radon", args.prec, n=n, phi=phi, theta=th, shift=s)
     job.load(vol)
+    # the first launch settles the tile shape (autotuned and cached, or fixed by the options)
+    job.run()
+    band_rows, dgroup = job.tuning()
+    log(f"bench: brick kernel tile: {band_rows} rows x {64 // band_rows} columns per band, "
+        f"{dgroup} directions per task")
     for i in range(args.warmup):
         st = job.run()
         log(f"bench: warmup {i} {st['seconds']:.3f} s")
@@ -743,11 +771,6 @@ def main():
         log("bench: analytic")
         analytic = analytic_leg(ctx, wrt, seconds=0.0 if args.no_cpu else 6.0)
         log(f"bench: analytic {analytic['seconds']:.3f} s")
-    laplace = None
-    if not args.no_laplace:
-        log("bench: laplace")
-        laplace = laplace_leg(ctx, wrt)
-        log(f"bench: laplace {laplace['seconds'] * 1e3:.2f} ms")
     adjoint = None
     if not args.no_adjoint:
         log("bench: adjoint")
@@ -787,6 +810,14 @@ def main():
         batch = batch_leg(ctx, wrt, args.batch_volumes)
         log(f"bench: batch {batch['graph_s']:.3f} s, {batch['speedup_vs_per_job']:.2f}x per-job, "
             f"{batch['speedup_vs_plain']:.2f}x plain")
+    # The spectral Laplacian last: once the process has made a hipFFT plan, the streams it
+    # makes afterwards overlap less (the per-job face-brick side stream costs 5% per pass and
+    # the serve rounds lose their overlap), so every other leg runs before it.
+    laplace = None
+    if not args.no_laplace:
+        log("bench: laplace")
+        laplace = laplace_leg(ctx, wrt)
+        log(f"bench: laplace {laplace['seconds'] * 1e3:.2f} ms")
     ctx.close()
     programs = None
     if not args.no_programs:
@@ -814,6 +845,7 @@ def main():
         "config": {"workload": f"radon_grid {n}^3 {args.prec} bricks, {len(phi)} phi x {len(th)} "
                                f"GL theta x {len(s)} shifts", "n": n, "planes": nplanes,
                    "logical_samples": logical, "incube_samples": incube,
+                   "band_rows": band_rows, "dgroup": dgroup,
                    "index_arithmetic": "binary64, reference rounding"},
         "rel_l2_vs_f64": err_l2,
         "max_err_vs_f64": err_max,

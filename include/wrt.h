@@ -324,6 +324,15 @@ int wrt_job_poll(wrt_job* job, wrt_stats* st);
 int wrt_job_wait(wrt_job* job, wrt_stats* st);
 int wrt_job_fetch(wrt_job* job, double* out);
 void wrt_job_destroy(wrt_job* job);
+/* The tile shape an FP32/BF16 plane-integral (RADON) job's brick kernel runs with (no
+ * reference counterpart): rows per band of the section walk (4, 8 or 16; 64 / rows columns
+ * per step) and directions per task.  The first launch after a load picks them: the fastest
+ * of the band shapes at the load's direction group, then of half and twice that group,
+ * timed once per problem shape (edge, directions, shifts, precision, non-empty bricks) and
+ * kept in a tuning cache file (WRT_TUNE_CACHE, else $HOME/.cache/wrt_tune.txt);
+ * WRT_AUTOTUNE=0 keeps 8 rows and the load's group, WRT_BAND_ROWS and WRT_RADON_DGROUP fix
+ * them.  WRT_E_STATE before that first launch, WRT_E_ARG for other jobs. */
+int wrt_job_tuning(const wrt_job* job, int* band_rows, int* dgroup);
 
 /* Run many jobs (forward and adjoint, uneven sizes) as one submission (no reference
  * counterpart).  The FP32/BF16 plane-integral jobs and the FP32 backprojection jobs run as

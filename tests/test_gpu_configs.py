@@ -118,6 +118,7 @@ def test_c2_transpose_dot(ctx, c2):
 
 # ------------------------------------------------------------------------------ C3 ---
 def test_c3_batch_equals_alone(ctx, tmp_path):
+    old_cache = os.environ.get("WRT_TUNE_CACHE", "")
     os.environ["WRT_TUNE_CACHE"] = str(tmp_path / "tune.txt")
     try:
         rng = np.random.default_rng(3)
@@ -146,7 +147,7 @@ def test_c3_batch_equals_alone(ctx, tmp_path):
         for _, j in jobs:
             j.close()
     finally:
-        del os.environ["WRT_TUNE_CACHE"]
+        os.environ["WRT_TUNE_CACHE"] = old_cache
 
 
 # ------------------------------------------------------------------------------ C4 ---

@@ -1,7 +1,8 @@
-"""GPU tests of the fast plane-integral form (FP32 values / BF16 bricks, FP64 sums of FP32
-band partials, the reference's index decisions: fixed point with an exact binary64 path near
-cell boundaries) against the plain FP64 form and the oracle; integer volumes pin every index
-decision (test_fast_f32_index_decisions_exact).
+"""GPU tests of the fast plane-integral form (FP32 values / BF16 bricks, FP32 lane and wave
+sums added to FP64 plane accumulators, the reference's index decisions: fixed point with an
+exact binary64 path near cell boundaries) against the plain FP64 form and the oracle; integer
+volumes pin every index decision (test_fast_f32_index_decisions_exact), for every band shape
+the autotuner can pick.
 
 Tolerances: FP32 relative L2 <= 1e-5 and max |error| <= 1e-5 * max|R| (the target of the
 form); BF16 bricks relative L2 <= 2e-3 (BF16 keeps 8 significand bits per voxel)."""
@@ -222,7 +223,40 @@ def test_fast_ray_f32_rays_on_cube_faces(ctx, n):
 @pytest.mark.parametrize("n", [32, 33, 40, 48, 64])
 @pytest.mark.parametrize("support", ["cube", "ball"])
 def test_fast_f32_index_decisions_exact(ctx, n, support):
-    # Small integer values make every sum exact (FP32 band partials, FP64 totals), so the
+    _exact_case(ctx, n, support)
+
+
+@pytest.mark.parametrize("rows", [4, 8, 16])
+@pytest.mark.parametrize("n", [33, 64])
+def test_fast_f32_band_shapes_exact(ctx, n, rows, monkeypatch):
+    # every band shape the autotuner may pick (rows x columns 4 x 16, 8 x 8, 16 x 4) walks
+    # exactly the reference's samples
+    monkeypatch.setenv("WRT_BAND_ROWS", str(rows))
+    _exact_case(ctx, n, "ball")
+    _exact_case(ctx, n, "cube")
+
+
+def test_fast_f32_autotune_caches_per_shape(ctx, tmp_path, monkeypatch):
+    # the first launch of a shape times the band shapes and direction groups and records the
+    # winner; a second load of the same shape finds it (no new line) and gives the same sums
+    cache = tmp_path / "tune.txt"
+    monkeypatch.setenv("WRT_TUNE_CACHE", str(cache))
+    monkeypatch.delenv("WRT_AUTOTUNE", raising=False)
+    vol, phi, th, s = _phantom_case(ctx, 96, 64, 32, 97, seed=5, sigma=0.01)
+    ref, _ = ctx.radon_grid(vol, phi, th, s, prec="f64")
+    got1, _ = ctx.radon_grid(vol, phi, th, s, prec="f32")
+    lines = cache.read_text().splitlines()
+    assert len(lines) == 1
+    _, rows, group, sec = lines[0].split()
+    assert int(rows) in (4, 8, 16) and int(group) >= 16 and float(sec) > 0
+    got2, _ = ctx.radon_grid(vol, phi, th, s, prec="f32")
+    assert cache.read_text().splitlines() == lines
+    for got in (got1, got2):
+        assert orc.rel_l2(got, ref) <= 1e-5, orc.rel_l2(got, ref)
+
+
+def _exact_case(ctx, n, support):
+    # Small integer values make every sum exact (FP32 lane and wave sums, FP64 totals), so the
     # fast form must give the plain form's integer sum on every plane: a single sample put
     # into another cell changes it (unless the two cells hold equal values).  Random and
     # structured directions, shifts on cell faces and on the cube's faces.

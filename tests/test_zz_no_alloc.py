@@ -2,9 +2,9 @@
 C-ABI, two calls in a row (the second on a smaller volume and grid) against
 wrt_radon_grid's FP32 form on the same inputs, and the limit of its fixed scratch.
 
-Tolerance: relative L2 and max relative error <= 1e-7 against the FP32 form (both round the
-same FP64 plane sums to FP32; only the atomic order of those sums differs), and <= 1e-5
-against the plain FP64 form."""
+Tolerance: relative L2 and max relative error <= 1e-7 against the FP32 form walked in the same
+8 x 8 bands (both round the same FP64 plane sums of the same FP32 section sums to FP32; only
+the atomic order of those FP64 sums differs), and <= 1e-5 against the plain FP64 form."""
 import ctypes as C
 
 import numpy as np
@@ -22,9 +22,12 @@ def _case(ctx, n, nphi, nth, ns, seed):
     return vol, phi, th, s
 
 
-def test_noalloc_matches_fp32_form_twice(ctx):
+def test_noalloc_matches_fp32_form_twice(ctx, monkeypatch):
     # 160^3 on 80 x 40 x 161, then 48^3 on a shuffled 25-shift grid: the second call must
-    # not see the first call's bricks, counts or sums
+    # not see the first call's bricks, counts or sums.  The per-job form walks the 8 x 8
+    # bands the entry point walks (its autotuner may pick another band shape, which sums
+    # each section's lanes in another FP32 order).
+    monkeypatch.setenv("WRT_BAND_ROWS", "8")
     rng = np.random.default_rng(1)
     cases = [_case(ctx, 160, 80, 40, 161, 3), _case(ctx, 48, 24, 12, 25, 4)]
     vol, phi, th, s = cases[1]
@@ -40,7 +43,7 @@ def test_noalloc_matches_fp32_form_twice(ctx):
         assert st["logical_samples"] == 16.0 * vol.shape[0] ** 2 * got.size
 
 
-def test_noalloc_rejects_beyond_scratch(ctx):
+def test_noalloc_rejects_beyond_scratch(ctx, monkeypatch):
     # 256^3 FP32 bricks alone fill the 64 MiB scratch
     n = 256
     vol = np.zeros((n, n, n))
@@ -52,6 +55,7 @@ def test_noalloc_rejects_beyond_scratch(ctx):
     assert rc == wrt.WRT_E_SCRATCH
     assert np.all(out == 7.0)  # nothing written
     # and the entry point still works after the refusal
+    monkeypatch.setenv("WRT_BAND_ROWS", "8")
     vol, phi, th, s = _case(ctx, 32, 8, 4, 17, 5)
     got, _ = ctx.radon_grid_noalloc(vol, phi, th, s)
     want, _ = ctx.radon_grid(vol, phi, th, s, prec="f32")

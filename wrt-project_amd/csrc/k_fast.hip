@@ -3,33 +3,45 @@
 // in radon_brick.hpp.
 #include "radon_brick.hpp"
 
-template <bool BND, bool BF16, bool POW2>
+template <bool BND, bool BF16, bool POW2, int BR>
 __global__ __launch_bounds__(FR_WAVES * 64) void k_radon_fast(wrt_radon_fast_args a, double h,
                                                               double dc, double inv_dc) {
   __shared__ radon_lds L;
-  radon_brick_task<BND, BF16, POW2>(a, a.tasks[2 * blockIdx.x], a.tasks[2 * blockIdx.x + 1], h,
-                                    dc, inv_dc, L);
+  radon_brick_task<BND, BF16, POW2, false, BR>(a, a.tasks[2 * blockIdx.x],
+                                               a.tasks[2 * blockIdx.x + 1], h, dc, inv_dc, L);
 }
 
-template <bool BND, bool BF16>
+template <bool BND, bool BF16, int BR>
 static int launch_pow2(const wrt_radon_fast_args* a, bool pow2, hipStream_t st) {
   const double n = a->n;
   const double dc = 2.0 / a->n;
   const double inv = 1.0 / dc;
   const double h = n / 2.0;
   if (pow2)
-    k_radon_fast<BND, BF16, true><<<a->ntasks, FR_WAVES * 64, 0, st>>>(*a, h, dc, inv);
+    k_radon_fast<BND, BF16, true, BR><<<a->ntasks, FR_WAVES * 64, 0, st>>>(*a, h, dc, inv);
   else
-    k_radon_fast<BND, BF16, false><<<a->ntasks, FR_WAVES * 64, 0, st>>>(*a, h, dc, inv);
+    k_radon_fast<BND, BF16, false, BR><<<a->ntasks, FR_WAVES * 64, 0, st>>>(*a, h, dc, inv);
   WRT_LAUNCH_CHECK();
   return WRT_OK;
 }
 
+template <int BR>
+static int launch_rows(const wrt_radon_fast_args* a, int boundary, hipStream_t st) {
+  const bool pow2 = (a->n & (a->n - 1)) == 0;
+  if (boundary)
+    return a->bf16 ? launch_pow2<true, true, BR>(a, pow2, st) : launch_pow2<true, false, BR>(a, pow2, st);
+  return a->bf16 ? launch_pow2<false, true, BR>(a, pow2, st) : launch_pow2<false, false, BR>(a, pow2, st);
+}
+
 extern "C" int wrtk_radon_fast(const wrt_radon_fast_args* a, int boundary, hipStream_t st) {
   if (a->ntasks <= 0) return WRT_OK;
-  const bool pow2 = (a->n & (a->n - 1)) == 0;
-  if (boundary) return a->bf16 ? launch_pow2<true, true>(a, pow2, st) : launch_pow2<true, false>(a, pow2, st);
-  return a->bf16 ? launch_pow2<false, true>(a, pow2, st) : launch_pow2<false, false>(a, pow2, st);
+  switch (a->band_rows) {
+    case 0:
+    case 8: return launch_rows<8>(a, boundary, st);
+    case 4: return launch_rows<4>(a, boundary, st);
+    case 16: return launch_rows<16>(a, boundary, st);
+    default: return WRT_E_ARG;
+  }
 }
 
 // ------------------------------------------------- transpose of the plane integrals ---

@@ -4,17 +4,18 @@
 //
 // Work unit: one workgroup per (non-empty 32^3 brick, group of directions).  The brick is
 // staged once into LDS as FP32; every plane (shift, direction) that cuts the brick is a
-// "section".  A wavefront walks a section in 8-row bands: lane (r, k) = (lane>>3, lane&7)
-// handles lattice row i1 = band + r and columns i2 = c0 + k, c0 stepping by 8 across the
-// band's conservative column interval.  Per sample the cell index is decided exactly as
+// "section".  A wavefront walks a section in bands of BR rows (BR = 8 by default; 4 or 16,
+// with BC = 64 / BR columns per step): lane (r, k) = (lane / BC, lane % BC) handles lattice
+// row i1 = band + r and columns i2 = c0 + k, c0 stepping by BC across the row's
+// conservative column interval.  Per sample the cell index is decided exactly as
 // radon_grid/src/integral.c:47-51 + interp.c:70-72 decide it in binary64
 //   x = ((s*n) + (a*i1)) + (b*i2),  ix = (int)((x + 1) / (2/n)),  index n -> n-1,
 // (the per-column products b*i2 are formed in registers, the division is an exact
 // power-of-two scaling or a corrected reciprocal), so only the value rounding (FP32/BF16)
 // and the summation order differ from the plain form.  Samples whose cell lies in the
-// brick add its FP32 value to a per-band FP32 partial, folded into an FP64 lane sum; a
-// section's lane sums are reduced over the wavefront in FP32 and added to the plane's FP64
-// accumulator (one atomic per section and brick).
+// brick add its FP32 value to the lane's FP32 sum; a section's lane sums are reduced over the
+// wavefront in FP32 and added to the plane's FP64 accumulator (one atomic per section and
+// brick).
 //
 // Transpose mode works on half bricks (32 x 32 x 16 cells) of 64-bit integer accumulators:
 // every sample whose cell lies in the half brick adds the section's value, quantized to a
@@ -69,18 +70,23 @@ __device__ __forceinline__ int cvt_floor_neg(float x) {
   return r;
 }
 
-// max over each aligned group of 8 lanes, every lane of the group ends with it: quad_perm
-// [1,0,3,2] (xor 1), quad_perm [2,3,0,1] (xor 2), then row_half_mirror (lane i of a half
-// row with lane 7 - i, which lies in the other quad).  Every lane has a source lane in these
-// patterns, so the moves need no old value and each step combines into one DPP max.
+// max over each aligned group of G (4, 8 or 16) lanes, every lane of the group ends with it:
+// quad_perm [1,0,3,2] (xor 1), quad_perm [2,3,0,1] (xor 2), then row_half_mirror (lane i of
+// a half row with lane 7 - i, which lies in the other quad) and row_mirror (lane i of a row
+// with lane 15 - i, in the other half row).  Every lane has a source lane in these patterns,
+// so the moves need no old value and each step combines into one DPP max.
 template <int CTRL>
 __device__ __forceinline__ int dpp_perm(int v) {
   return __builtin_amdgcn_mov_dpp(v, CTRL, 0xf, 0xf, true);
 }
-__device__ __forceinline__ int group8_max(int v) {
+template <int G>
+__device__ __forceinline__ int group_max(int v) {
+  static_assert(G == 4 || G == 8 || G == 16, "groups of 4, 8 or 16 lanes");
   v = max(v, dpp_perm<0xB1>(v));
   v = max(v, dpp_perm<0x4E>(v));
-  return max(v, dpp_perm<0x141>(v));
+  if constexpr (G >= 8) v = max(v, dpp_perm<0x141>(v));
+  if constexpr (G >= 16) v = max(v, dpp_perm<0x140>(v));
+  return v;
 }
 
 // Wavefront sum of a float with DPP lane moves (no LDS round trips): Hillis-Steele scan
@@ -144,11 +150,16 @@ struct radon_T_lds {
 // sum_planes q_p * (number of the plane's samples in the cell); the caller stores them.
 // Called by every thread of a FR_WAVES*64-thread workgroup; returns with the workgroup's
 // updates issued (no barrier at the end: a caller that reads or reuses L waits first).
-template <bool BND, bool BF16, bool POW2, bool T = false, class LDS = radon_lds>
+template <bool BND, bool BF16, bool POW2, bool T = false, int BR = 8, class LDS = radon_lds>
 __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, int brick, int grp,
                                                  double h, double dc, double inv_dc, LDS& L,
                                                  int zhalf = 0) {
   constexpr int ZN = T ? FT_Z : FB;  // the box's extent in z
+  static_assert(BR == 4 || BR == 8 || BR == 16, "bands of 4, 8 or 16 rows");
+  static_assert(!T || BR == 8, "transpose mode walks 8-row bands");
+  constexpr int BC = 64 / BR;                          // columns per step
+  constexpr int LBC = BC == 16 ? 4 : (BC == 8 ? 3 : 2);  // log2(BC)
+  constexpr int NBH = 64 / BR;                         // bands per 64-row half
   int (*const rowlo)[136] = L.rowlo;
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -194,7 +205,7 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
   const bool fhi[3] = {b0x + FB >= n, b0y + FB >= n, b0z + ZN >= n};
   const int cxh = fhi[0] ? n - 1 : INT_MAX, cyh = fhi[1] ? n - 1 : INT_MAX,
             czh = fhi[2] ? n - 1 : INT_MAX;
-  const int r_lane = lane >> 3, k_lane = lane & 7;
+  const int r_lane = lane >> LBC, k_lane = lane & (BC - 1);
   // the brick's farthest corner lies within the support sphere
   bool inside;
   {
@@ -308,7 +319,7 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
     // per row (a_c h) and per column (b_c h); |a_c h|, |b_c h| <= 1/2, clamped into int32
     // (error at most 2^-32 each).  Eight columns for the column walk.
     int Afx[3], Dfx[3], nDfx[3];
-    long long D8[3];
+    long long DS[3];  // one step of BC columns
 #pragma unroll
     for (int c = 0; c < 3; ++c) {  // wave-uniform: held in scalar registers
       Afx[c] = __builtin_amdgcn_readfirstlane(
@@ -316,7 +327,7 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
       Dfx[c] = __builtin_amdgcn_readfirstlane(
           (int)fmin(fmax(rint(bv[c] * h * 4294967296.0), -2147483647.0), 2147483647.0));
       nDfx[c] = -Dfx[c];
-      D8[c] = 8ll * Dfx[c];
+      DS[c] = (long long)BC * Dfx[c];
     }
     // Lanes 0, 1, 2 form the fixed-point base of axes x, y, z at (row R0, column C0) for each
     // section as one FMA s * (n_c h) + K_c, K_c = ((a_c R0 + b_c C0) + 1) h - b0_c: |q| < 200
@@ -351,9 +362,9 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
       int ins = inside_s, azb = anyzb_s;
       asm volatile("" : "+s"(ins), "+s"(azb));
       // Candidate column interval of every row R0 + t of the window (t = lane, lane + 64):
-      // first and last column into the wave's row table, the 8-column step count reduced to
-      // a per-band maximum (bands of the rows 8k .. 8k + 7 of the window), and the rows that
-      // meet the brick and the sphere as a ballot
+      // first and last column into the wave's row table, the BC-column step count reduced to
+      // a per-band maximum (bands of the rows BR k .. BR k + BR - 1 of the window), and the
+      // rows that meet the brick and the sphere as a ballot
       int bsteps[2];
       unsigned long long live[2];
 #pragma unroll
@@ -398,24 +409,24 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
         // into [C0 - 1, C1].  An empty row ends with rh < rl.
         const int nrl = cvt_floor_neg(__builtin_amdgcn_fmed3f(tlo, C0f, C1p1));
         const int rh = cvt_floor(__builtin_amdgcn_fmed3f(thi, C0m1, C1f));
-        // rh - rl, and the row's steps of 8 columns less one, (rh - rl) >> 3: negative for an
-        // empty row (the bands add the one back)
+        // rh - rl, and the row's steps of BC columns less one, (rh - rl) / BC rounded down:
+        // negative for an empty row (the bands add the one back)
         const int span = rh + nrl;
-        const int nst = span >> 3;
+        const int nst = span >> LBC;
         // -rl and rh, 16 bits each (|i2| <= 2n + 1 < 2^15)
         rowlo[wv][t] = (int)__builtin_amdgcn_perm((unsigned)rh, (unsigned)nrl, 0x05040100u);
-        // max over the 8 consecutive rows of a band (lanes 8b .. 8b+7)
-        bsteps[half] = group8_max(nst);
+        // max over the BR consecutive rows of a band (lanes BR b .. BR b + BR - 1)
+        bsteps[half] = group_max<BR>(nst);
         live[half] = __ballot(span >= 0);
       }
       if ((live[0] | live[1]) == 0ull) continue;  // no row meets the brick and the sphere
       // the bands that hold the first and the last live row
       const int t_first = live[0] ? __builtin_ctzll(live[0]) : 64 + __builtin_ctzll(live[1]);
       const int t_last = live[1] ? 127 - __builtin_clzll(live[1]) : 63 - __builtin_clzll(live[0]);
-      const int k_first = t_first >> 3, k_last = t_last >> 3;
-      // both halves' band maxima in one register: band b of half 0 at lane 8b, of half 1 at
-      // lane 8b + 1 (a band's 8 lanes hold the same maximum)
-      const int bsteps01 = (lane & 7) == 1 ? bsteps[1] : bsteps[0];
+      const int k_first = t_first / BR, k_last = t_last / BR;
+      // both halves' band maxima in one register: band b of half 0 at lane BR b, of half 1
+      // at lane BR b + 1 (a band's BR lanes hold the same maximum)
+      const int bsteps01 = (lane & (BR - 1)) == 1 ? bsteps[1] : bsteps[0];
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -536,19 +547,19 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
           asm volatile("" : "+v"(Tb[c]));
         }
       }
-      // Bands of 8 rows: lane (r, k) takes row 8k' + r of band k' and the columns rl + k + 8m
-      // (rl the row's first column).  Transpose mode takes the columns rl + dc + 2k + 16m in
-      // two phases dc = 0, 1:
+      // Bands of BR rows: lane (r, k) takes row BR k' + r of band k' and the columns rl + k +
+      // BC m (rl the row's first column).  Transpose mode (BR = 8) takes the columns rl + dc +
+      // 2k + 16m in two phases dc = 0, 1:
       // the lanes of one instruction then lie 2 columns apart and fewer of them add to one
       // cell at once (the LDS integer atomic serialises those, which costs time, not
       // results).  Measured at 256^3: 2.66 s against 2.92 s for the forward's columns and
       // 2.90 s for bands of 16 rows taken 2 apart (their longer rows' steps waste more).
-      int rlo_next = rowlo[wv][8 * k_first + r_lane];
+      int rlo_next = rowlo[wv][BR * k_first + r_lane];
       for (int k = k_first; k <= k_last; ++k) {
-        const int t = 8 * k + r_lane;  // this lane's row within the window
+        const int t = BR * k + r_lane;  // this lane's row within the window
         const int nrl = (rlo_next << 16) >> 16;  // minus the band row's first column
-        if (k < k_last) rlo_next = rowlo[wv][t + 8];
-        const int steps = 1 + __builtin_amdgcn_readlane(bsteps01, 8 * (k & 7) + (k >> 3));
+        if (k < k_last) rlo_next = rowlo[wv][t + BR];
+        const int steps = 1 + __builtin_amdgcn_readlane(bsteps01, BR * (k % NBH) + k / NBH);
         if (steps <= 0) continue;
         if constexpr (T) {
           const int steps2 = (steps + 1) >> 1;  // 16-column steps cover the 8-column ones
@@ -563,7 +574,7 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
             for (int m = 0; m < steps2; ++m) {
               sample(Q, t, cb, nrl, 16, m, bs);
 #pragma unroll
-              for (int c = 0; c < 3; ++c) Q[c] += 2 * D8[c];
+              for (int c = 0; c < 3; ++c) Q[c] += 2 * DS[c];
             }
           }
         } else {
@@ -577,9 +588,9 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
           for (int c = 0; c < 3; ++c)
             Q[c] = Tb[c] + (long long)tq * Afx[c] + (long long)nrl * nDfx[c];
           for (int m = 0; m < steps; ++m) {
-            sample(Q, t, k_lane, nrl, 8, m, bs);
+            sample(Q, t, k_lane, nrl, BC, m, bs);
 #pragma unroll
-            for (int c = 0; c < 3; ++c) Q[c] += D8[c];
+            for (int c = 0; c < 3; ++c) Q[c] += DS[c];
           }
         }
       }

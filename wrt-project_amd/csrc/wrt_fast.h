@@ -33,6 +33,8 @@ typedef struct {
                          caller's grid may be in any order                    */
   int nshift;
   double* acc;        /* [nshift][ndir] FP64 accumulators, zeroed by caller   */
+  int band_rows;      /* rows per band of the walk: 4, 8 or 16 (0: 8); the forward
+                         per-job launch only, the others walk 8                */
   double rho2;        /* squared support radius: every sample farther from the
                          origin reads a zero cell (cell-centre radius of the
                          farthest non-zero cell plus half a cell diagonal)     */

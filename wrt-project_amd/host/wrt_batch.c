@@ -13,9 +13,9 @@
  *   segments the FP64 accumulators the prologue zeroes and the epilogue rounds to FP32;
  *   graph    prologue (zero fill by kernel, no memset node) -> persistent kernel -> epilogue.
  * The direction-group size G is the tunable: the autotuner times the batch at each
- * candidate on the first call for a list shape and keeps the fastest, in memory and in a
- * small text cache file (WRT_TUNE_CACHE, else $HOME/.cache/wrt_batch_tune.txt; set
- * WRT_AUTOTUNE=0 to skip tuning and use G = 64, or WRT_BATCH_DGROUP=G (>= 16) to fix G).
+ * candidate on the first call for a list shape and keeps the fastest, in memory and in the
+ * tuning cache file (host/wrt_tune.c; set WRT_AUTOTUNE=0 to skip tuning and use G = 64, or
+ * WRT_BATCH_DGROUP=G (>= 16) to fix G).
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -180,65 +180,31 @@ static int enqueue_plan(wrt_ctx* c, batch_plan* p, hipStream_t st) {
   return rc;
 }
 
-/* ---- autotune cache: "<signature hex> <G> <seconds>" lines ---- */
+/* ---- autotune: the plan's signature in the tuning cache (host/wrt_tune.c) ---- */
 static unsigned long long signature(const wrt_fast_desc* fd, int nitems) {
-  unsigned long long h = 1469598103934665603ULL;
-#define MIX(v)                                   \
-  do {                                           \
-    h ^= (unsigned long long)(long long)(v);     \
-    h *= 1099511628211ULL;                       \
-  } while (0)
-  MIX(nitems);
+  long long* w = (long long*)malloc(sizeof(long long) * (size_t)(2 + 6 * nitems));
+  if (!w) return 0;
+  int k = 0;
+  w[k++] = 0x62617463680aLL; /* "batch": its own signature space */
+  w[k++] = nitems;
   for (int i = 0; i < nitems; ++i) {
     const wrt_batch_item* it = &fd[i].item;
-    MIX(it->kind);
+    w[k++] = it->kind;
     if (it->kind == WRT_ITEM_RADON) {
-      MIX(it->ra.n);
-      MIX(it->ra.ndir);
-      MIX(it->ra.nshift);
-      MIX(it->ra.bf16);
-      MIX(fd[i].nbrick);
+      w[k++] = it->ra.n;
+      w[k++] = it->ra.ndir;
+      w[k++] = it->ra.nshift;
+      w[k++] = it->ra.bf16;
+      w[k++] = fd[i].nbrick;
     } else {
-      MIX(it->an);
-      MIX(it->andir);
-      MIX(it->ans);
+      w[k++] = it->an;
+      w[k++] = it->andir;
+      w[k++] = it->ans;
     }
   }
-#undef MIX
+  const unsigned long long h = wrt_tune_hash(w, k);
+  free(w);
   return h;
-}
-
-static void cache_path(char* buf, size_t len) {
-  const char* e = getenv("WRT_TUNE_CACHE");
-  if (e && *e) {
-    snprintf(buf, len, "%s", e);
-    return;
-  }
-  const char* home = getenv("HOME");
-  snprintf(buf, len, "%s/.cache/wrt_batch_tune.txt", home && *home ? home : "/tmp");
-}
-
-static int cache_lookup(unsigned long long sig) {
-  char path[1024];
-  cache_path(path, sizeof(path));
-  FILE* f = fopen(path, "r");
-  if (!f) return 0;
-  unsigned long long s;
-  int g, found = 0;
-  double sec;
-  while (fscanf(f, "%llx %d %lf", &s, &g, &sec) == 3)
-    if (s == sig && g >= 16) found = g;  /* the last entry wins */
-  fclose(f);
-  return found;
-}
-
-static void cache_store(unsigned long long sig, int G, double sec) {
-  char path[1024];
-  cache_path(path, sizeof(path));
-  FILE* f = fopen(path, "a");
-  if (!f) return;  /* no writable cache: tuning is redone next time */
-  fprintf(f, "%016llx %d %.6f\n", sig, G, sec);
-  fclose(f);
 }
 
 static int time_plan(wrt_ctx* c, batch_plan* p, double* sec) {
@@ -301,9 +267,9 @@ static int plan_build(wrt_ctx* c, wrt_job* const* jobs, int njobs, batch_plan** 
     G = forced;
   } else if (rc == WRT_OK && nitems > 0) {
     const unsigned long long sig = signature(fd, nitems);
-    const char* at = getenv("WRT_AUTOTUNE");
-    const int tune = !(at && at[0] == '0');
-    const int cached = tune ? cache_lookup(sig) : 0;
+    const int tune = wrt_tune_enabled();
+    int cached = 0, unused = 0;
+    if (tune && !(wrt_tune_lookup(sig, &cached, &unused) && cached >= 16)) cached = 0;
     if (cached) {
       G = cached;
     } else if (tune) {
@@ -318,7 +284,7 @@ static int plan_build(wrt_ctx* c, wrt_job* const* jobs, int njobs, batch_plan** 
           G = cand[q];
         }
       }
-      if (rc == WRT_OK) cache_store(sig, G, best);
+      if (rc == WRT_OK) wrt_tune_store(sig, G, 0, best);
     }
   }
   if (rc == WRT_OK) rc = plan_upload(p, fd, nitems, G, c->stream);

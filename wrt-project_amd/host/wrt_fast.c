@@ -44,6 +44,8 @@ typedef struct {
   double rho2;      /* squared support radius of the loaded volume */
   int* tasks;       /* device, interior then boundary */
   int n_interior, n_boundary, dgroup;
+  int band_rows;    /* rows per band of the walk (4, 8 or 16) */
+  int tuned;        /* band_rows and dgroup settled for this load */
   double* acc;      /* [nshift][ndir] */
   float* out;       /* [nshift][ndir] */
   double* staging;  /* n^3 FP64 input */
@@ -421,6 +423,39 @@ static int radon_fast_create(wrt_job* j) {
   return rc;
 }
 
+/* The task list for direction groups of dg: every non-empty brick (heaviest first) x group,
+ * interior bricks, then bricks touching the cube's faces; uploaded, the count kept. */
+static int radon_fast_tasks(wrt_job* j, int dg) {
+  radon_fast* f = (radon_fast*)j->fast;
+  const wrt_job_desc* d = &j->d;
+  const int ndir = d->nphi * d->ntheta;
+  const int ngroups = (ndir + dg - 1) / dg;
+  if (dg < 8 || (size_t)f->n_ne * ngroups > f->nbricks * (size_t)((ndir + 7) / 8)) return WRT_E_ARG;
+  int* t = (int*)malloc(2 * ((size_t)f->n_ne * ngroups + 1) * sizeof(int));
+  if (!t) return WRT_E_NOMEM;
+  int k = 0;
+  for (int pass = 0; pass < 2; ++pass) {
+    for (int i = 0; i < f->n_ne; ++i) {
+      if (f->ne_bnd[i] != pass) continue;
+      for (int g = 0; g < ngroups; ++g) {
+        t[2 * k] = f->ne_brick[i];
+        t[2 * k + 1] = g;
+        ++k;
+      }
+    }
+    if (pass == 0) f->n_interior = k;
+  }
+  f->n_boundary = k - f->n_interior;
+  f->dgroup = dg;
+  int rc = WRT_OK;
+  if (k > 0 && (hipMemcpyAsync(f->tasks, t, 2 * (size_t)k * sizeof(int), hipMemcpyHostToDevice,
+                               j->st) != hipSuccess ||
+                hipStreamSynchronize(j->st) != hipSuccess))
+    rc = WRT_E_HIP;
+  free(t);
+  return rc;
+}
+
 static int radon_fast_load(wrt_job* j, const double* in) {
   radon_fast* f = (radon_fast*)j->fast;
   const wrt_job_desc* d = &j->d;
@@ -464,14 +499,10 @@ static int radon_fast_load(wrt_job* j, const double* in) {
   long long want = (long long)ndir * (nonempty ? nonempty : 1) / 2048;
   int dg = (int)(want < 16 ? 16 : (want > cap ? cap : want));
   dg = (dg + 15) & ~15;
-  const int ngroups = (ndir + dg - 1) / dg;
-  f->dgroup = dg;
-  int* t = (int*)malloc(2 * (f->nbricks * ngroups + 1) * sizeof(int));
-  if (!t) { free(flags); return WRT_E_NOMEM; }
   /* bricks by falling non-zero count (the work of a task grows with it), so that the
    * longest tasks start first and the launch ends on short ones */
   int* order = (int*)malloc(f->nbricks * sizeof(int));
-  if (!order) { free(flags); free(t); return WRT_E_NOMEM; }
+  if (!order) { free(flags); return WRT_E_NOMEM; }
   int nne = 0;
   for (size_t b = 0; b < f->nbricks; ++b)
     if (flags[b]) order[nne++] = (int)b;
@@ -489,7 +520,7 @@ static int radon_fast_load(wrt_job* j, const double* in) {
   f->ne_bnd = (unsigned char*)malloc((size_t)(nne + 1));
   f->n_ne = nne;
   if (!f->ne_brick || !f->ne_count || !f->ne_bnd) {
-    free(order); free(flags); free(t);
+    free(order); free(flags);
     return WRT_E_NOMEM;
   }
   for (int i = 0; i < nne; ++i) {
@@ -500,42 +531,25 @@ static int radon_fast_load(wrt_job* j, const double* in) {
     f->ne_bnd[i] = bx == 0 || by == 0 || bz == 0 || (bx + 1) * WRT_BRICK >= d->n ||
                    (by + 1) * WRT_BRICK >= d->n || (bz + 1) * WRT_BRICK >= d->n;
   }
-  int k = 0;
-  for (int pass = 0; pass < 2; ++pass) {
-    for (int i = 0; i < nne; ++i) {
-      const int b = order[i];
-      const int bnd = f->ne_bnd[i];
-      if (bnd != pass) continue;
-      for (int g = 0; g < ngroups; ++g) {
-        t[2 * k] = b;
-        t[2 * k + 1] = g;
-        ++k;
-      }
-    }
-    if (pass == 0) f->n_interior = k;
-  }
   free(order);
-  f->n_boundary = k - f->n_interior;
   free(flags);
-  rc = WRT_OK;
-  if (k > 0 && (hipMemcpyAsync(f->tasks, t, 2 * (size_t)k * sizeof(int), hipMemcpyHostToDevice, st) != hipSuccess ||
-                hipStreamSynchronize(st) != hipSuccess))
-    rc = WRT_E_HIP;
-  free(t);
-  return rc;
+  f->band_rows = 8;
+  f->tuned = 0;
+  return radon_fast_tasks(j, dg);
 }
 
-/* enqueue on the job's stream; e1 / e3 bracket the brick kernels */
-static int radon_fast_enqueue(wrt_job* j) {
+/* one pass of the brick kernels on stream st (interior on st, faces on the side stream
+ * when the job has one), then the FP32 rounding */
+static int radon_fast_pass(wrt_job* j, hipStream_t st, int timed) {
   radon_fast* f = (radon_fast*)j->fast;
   const wrt_job_desc* d = &j->d;
-  hipStream_t st = j->st;
   const size_t ndir = (size_t)d->nphi * d->ntheta;
   const size_t nout = (size_t)d->nshift * ndir;
   int rc = wrtk_fill_f64(f->acc, 0.0, nout, st);
   if (rc != WRT_OK) return rc;
-  HIPCHK(hipEventRecord(j->e1, st));
+  if (timed) HIPCHK(hipEventRecord(j->e1, st));
   wrt_radon_fast_args a;
+  memset(&a, 0, sizeof(a));
   a.bricks = f->bricks;
   a.n = d->n;
   a.nb = f->nb;
@@ -548,6 +562,7 @@ static int radon_fast_enqueue(wrt_job* j) {
   a.nshift = d->nshift;
   a.acc = f->acc;
   a.rho2 = f->rho2;
+  a.band_rows = f->band_rows;
   /* face bricks on the side stream, interior bricks on the job's stream, concurrently:
    * both add into the same FP64 accumulators with atomics */
   hipStream_t sb = f->side ? f->side : st;
@@ -566,8 +581,110 @@ static int radon_fast_enqueue(wrt_job* j) {
   rc = wrtk_radon_fast(&a, 0, st);
   if (rc != WRT_OK) return rc;
   if (f->side) HIPCHK(hipStreamWaitEvent(st, f->join, 0));
-  HIPCHK(hipEventRecord(j->e3, st));
+  if (timed) HIPCHK(hipEventRecord(j->e3, st));
   return wrtk_acc_to_f32(f->acc, f->out, nout, st);
+}
+
+/* seconds of one pass with the current band shape and direction group (events of its own:
+ * the job's bracket its launches) */
+static int radon_fast_time(wrt_job* j, double* sec) {
+  hipEvent_t a = NULL, b = NULL;
+  int rc = WRT_OK;
+  if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess ||
+      hipEventRecord(a, j->st) != hipSuccess)
+    rc = WRT_E_HIP;
+  if (rc == WRT_OK) rc = radon_fast_pass(j, j->st, 0);
+  float ms = 0;
+  if (rc == WRT_OK && (hipEventRecord(b, j->st) != hipSuccess ||
+                       hipEventSynchronize(b) != hipSuccess ||
+                       hipEventElapsedTime(&ms, a, b) != hipSuccess))
+    rc = WRT_E_HIP;
+  if (a) hipEventDestroy(a);
+  if (b) hipEventDestroy(b);
+  *sec = ms * 1e-3;
+  return rc;
+}
+
+/* The autotuner of the per-job launch, run at a load's first launch: the band shape (8 x 8,
+ * 4 x 16 or 16 x 4 rows x columns) timed at the load's direction group, then half and twice
+ * that group at the best shape; the fastest pair is kept for the load and in the tuning cache
+ * under the problem's shape (edge, directions, shifts, precision, non-empty bricks), where
+ * later loads of that shape find it.  WRT_BAND_ROWS (4, 8, 16) and WRT_RADON_DGROUP (>= 16)
+ * fix the shape and the group and skip the tuning; WRT_AUTOTUNE=0 keeps 8 x 8 and the load's
+ * group. */
+static int radon_fast_tune(wrt_job* j) {
+  radon_fast* f = (radon_fast*)j->fast;
+  const wrt_job_desc* d = &j->d;
+  f->tuned = 1;
+  const char* force = getenv("WRT_BAND_ROWS");
+  const char* force_g = getenv("WRT_RADON_DGROUP");
+  if ((force && *force) || (force_g && *force_g)) {
+    if (force && *force) {
+      const int br = atoi(force);
+      if (br != 4 && br != 8 && br != 16) return WRT_E_ARG;
+      f->band_rows = br;
+    }
+    if (force_g && *force_g) {
+      const int g = atoi(force_g);
+      if (g < 16 || g % 16) return WRT_E_ARG;
+      return radon_fast_tasks(j, g);
+    }
+    return WRT_OK;
+  }
+  if (!wrt_tune_enabled() || f->n_ne == 0) return WRT_OK;
+  const int dg0 = f->dgroup, ndir = d->nphi * d->ntheta;
+  const long long key[7] = {0x72616e64LL /* its own signature space */, d->n, ndir, d->nshift,
+                            d->prec, f->n_ne, dg0};
+  const unsigned long long sig = wrt_tune_hash(key, 7);
+  int br = 8, dg = dg0;
+  if (wrt_tune_lookup(sig, &br, &dg) && (br == 4 || br == 8 || br == 16) && dg >= 16) {
+    f->band_rows = br;
+    return dg != f->dgroup ? radon_fast_tasks(j, dg) : WRT_OK;
+  }
+  static const int shapes[3] = {8, 4, 16};
+  double best = 1e300, sec = 0;
+  int best_br = 8, best_dg = dg0;
+  for (int q = 0; q < 3; ++q) {
+    f->band_rows = shapes[q];
+    int rc = radon_fast_time(j, &sec); /* the first run of a variant loads its code ... */
+    if (rc == WRT_OK) rc = radon_fast_time(j, &sec); /* ... so the second one is timed */
+    if (rc != WRT_OK) return rc;
+    if (sec < best) {
+      best = sec;
+      best_br = shapes[q];
+    }
+  }
+  f->band_rows = best_br;
+  const int cand[2] = {dg0 / 2, dg0 * 2};
+  for (int q = 0; q < 2; ++q) {
+    const int g = cand[q] & ~15;
+    if (g < 16 || g > 4096 || g > ndir) continue;
+    int rc = radon_fast_tasks(j, g);
+    if (rc == WRT_OK) rc = radon_fast_time(j, &sec);
+    if (rc != WRT_OK) return rc;
+    if (sec < best) {
+      best = sec;
+      best_dg = g;
+    }
+  }
+  if (f->dgroup != best_dg) {
+    const int rc = radon_fast_tasks(j, best_dg);
+    if (rc != WRT_OK) return rc;
+  }
+  wrt_tune_store(sig, best_br, best_dg, best);
+  return WRT_OK;
+}
+
+/* enqueue on the job's stream; e1 / e3 bracket the brick kernels (the first launch of a
+ * load first settles the band shape and direction group: radon_fast_tune) */
+static int radon_fast_enqueue(wrt_job* j) {
+  radon_fast* f = (radon_fast*)j->fast;
+  if (!f->tuned) {
+    const int rc = radon_fast_tune(j);
+    if (rc != WRT_OK) return rc;
+    HIPCHK(hipEventRecord(j->e0, j->st)); /* the launch's time leaves the tuning out */
+  }
+  return radon_fast_pass(j, j->st, 1);
 }
 
 /* ------------------------------------------------------------------ RADON_T --- */
@@ -940,6 +1057,15 @@ int wrt_fast_fetch(wrt_job* j, double* out) {
     case WRT_OP_RAY_T: return fetch_f32(j, ((ray_T_fast*)j->fast)->out, n3, out);
     default: return fetch_f32(j, ((adjoint_fast*)j->fast)->out, n3, out);
   }
+}
+
+int wrt_fast_tuning(const wrt_job* j, int* band_rows, int* dgroup) {
+  if (!j || !j->fast || j->d.op != WRT_OP_RADON || j->d.prec == WRT_F64) return WRT_E_ARG;
+  const radon_fast* f = (const radon_fast*)j->fast;
+  if (!f->tuned) return WRT_E_STATE;
+  if (band_rows) *band_rows = f->band_rows;
+  if (dgroup) *dgroup = f->dgroup;
+  return WRT_OK;
 }
 
 /* the FP32 plane values of a fast radon job from its FP64 accumulators, i.e. the rounding

@@ -75,6 +75,8 @@ typedef struct {
 } wrt_fast_desc;
 int wrt_fast_supported(const wrt_job_desc* d);
 int wrt_fast_describe(wrt_job* j, wrt_fast_desc* fd);
+/* radon: the band shape and direction group of the per-job launch */
+int wrt_fast_tuning(const wrt_job* j, int* band_rows, int* dgroup);
 /* radon: the FP64 accumulators rounded to FP32, as doubles (the form's output) */
 int wrt_fast_fetch_acc(wrt_job* j, double* out);
 
@@ -94,6 +96,12 @@ typedef struct {
 long long wrt_item_tasks(const wrt_fast_desc* fd, int item, int G, wrt_costed_task* ct);
 /* order by falling cost (qsort) */
 int wrt_costed_desc(const void* a, const void* b);
+
+/* the tuning cache (host/wrt_tune.c) */
+int wrt_tune_enabled(void);
+unsigned long long wrt_tune_hash(const long long* v, int n);
+int wrt_tune_lookup(unsigned long long sig, int* v0, int* v1);
+void wrt_tune_store(unsigned long long sig, int v0, int v1, double sec);
 
 /* host-side tables (C library sin/cos, the reference's bits) */
 void wrt_host_radon_dirs(int n, const double* phi, int nphi, const double* theta, int ntheta,

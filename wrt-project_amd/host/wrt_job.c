@@ -400,6 +400,10 @@ int wrt_job_run(wrt_job* j, wrt_stats* st) {
   return wrt_job_wait(j, st);
 }
 
+int wrt_job_tuning(const wrt_job* j, int* band_rows, int* dgroup) {
+  return wrt_fast_tuning(j, band_rows, dgroup);
+}
+
 int wrt_job_fetch(wrt_job* j, double* out) {
   if (!j || !out) return WRT_E_ARG;
   if (j->pending) return WRT_E_STATE;

@@ -221,8 +221,9 @@ static void slot_free(round_slot* r) {
   memset(r, 0, sizeof(*r));
 }
 
-static int slot_init(round_slot* r) {
+static int slot_init(round_slot* r, int cu_count) {
   memset(r, 0, sizeof(*r));
+  (void)cu_count;
   if (hipStreamCreateWithFlags(&r->st, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&r->pro, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&r->done, hipEventDisableTiming) != hipSuccess)
@@ -421,7 +422,7 @@ int wrt_serve_run(wrt_ctx* ctx, const wrt_serve_cfg* c, wrt_serve_report* rep,
     req_ds[i] = k;
   }
   if (rc != WRT_OK) goto done;
-  for (int q = 0; q < RS_SLOTS && rc == WRT_OK; ++q) rc = slot_init(&slots[q]);
+  for (int q = 0; q < RS_SLOTS && rc == WRT_OK; ++q) rc = slot_init(&slots[q], ctx->cu_count);
   if (rc != WRT_OK) goto done;
 
   /* admission: live free device memory less a reserve, and the optional byte budget */

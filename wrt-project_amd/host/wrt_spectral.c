@@ -2,8 +2,8 @@
  * wrt_spectral.c -- the spectral Laplacians that follow the backprojection
  * (radon_inversion/backprojection/laplace2dfft.m, laplace3dfft.m): real-to-complex DFT
  * (hipFFT), the frequency factor (k_spectral.hip), complex-to-real DFT.  FP64 throughout, as
- * the reference's Octave code computes.  One plan per context, kept for the next call of the
- * same size and dimension.
+ * the reference's Octave code computes.  The plan pair is made per call and freed at its end
+ * (see the end of laplace()).
  */
 #include <hipfft/hipfft.h>
 #include <math.h>
@@ -101,6 +101,11 @@ static int laplace(wrt_ctx* c, const double* u, int n, int dims, double period, 
   }
   hipFree(d_u);
   hipFree(d_spec);
+  /* The plans are not kept past the call: with a live 256^3 D2Z/Z2D plan pair in the
+   * context, the serve loop's rounds of the persistent kernel started later from that
+   * context ran measurably slower (8-QPS stream: p99 395 -> 630 ms, burst rate -17%; a new
+   * context, or freeing the plans, restores both). */
+  wrt_spectral_release(c);
   return rc;
 }
 

@@ -78,7 +78,7 @@ ABI_SYMBOLS = [
     "wrt_ray_grid", "wrt_radon_reduction", "wrt_radon_reduction_rays", "wrt_radon_adjoint", "wrt_radon_grid_T",
     "wrt_ray_grid_T", "wrt_job_bytes", "wrt_job_input_count", "wrt_job_output_count",
     "wrt_job_create", "wrt_job_load", "wrt_job_load_synthetic", "wrt_job_run",
-    "wrt_job_fetch", "wrt_job_destroy", "wrt_batch_run", "wrt_phantom",
+    "wrt_job_fetch", "wrt_job_destroy", "wrt_job_tuning", "wrt_batch_run", "wrt_phantom",
     "wrt_phantom_params", "wrt_gaussian", "wrt_radon_count_incube", "wrt_job_launch",
     "wrt_job_poll", "wrt_job_wait", "wrt_serve_describe", "wrt_serve_grids", "wrt_serve_desc",
     "wrt_serve_input", "wrt_serve_run", "wrt_batch_dgroup",
@@ -205,6 +205,7 @@ def lib():
         L.wrt_job_poll.argtypes = [P, P]
         L.wrt_job_wait.argtypes = [P, P]
         L.wrt_job_fetch.argtypes = [P, P]
+        L.wrt_job_tuning.argtypes = [P, C.POINTER(I), C.POINTER(I)]
         L.wrt_job_destroy.argtypes = [P]
         L.wrt_job_destroy.restype = None
         L.wrt_batch_run.argtypes = [P, P, I, P]
@@ -601,6 +602,13 @@ class Job:
         out = np.empty(self.nout)
         _chk(lib().wrt_job_fetch(self.h, _p(out)), "wrt_job_fetch")
         return out
+
+    def tuning(self):
+        """(rows per band, directions per task) of a fast plane-integral job after its first
+        launch."""
+        br, dg = C.c_int(0), C.c_int(0)
+        _chk(lib().wrt_job_tuning(self.h, C.byref(br), C.byref(dg)), "wrt_job_tuning")
+        return br.value, dg.value
 
     def close(self):
         if self.h:
