@@ -9,7 +9,8 @@ import os
 import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-ORACLE = os.path.join(ROOT, "oracle", "liboracle.so")
+# ORACLE_LIB selects another build of the checker (the sanitizer build oracle/_san)
+ORACLE = os.environ.get("ORACLE_LIB") or os.path.join(ROOT, "oracle", "liboracle.so")
 REF_RADON = os.path.join(ROOT, "oracle", "_ref", "libref_radon_grid.so")
 REF_RAY = os.path.join(ROOT, "oracle", "_ref", "libref_ray_grid.so")
 

@@ -12,7 +12,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "libwrt.so")
+# WRT_LIB selects another build of the library (the CPU sanitizer build lib/san/libwrt.so)
+LIB_PATH = os.environ.get("WRT_LIB") or os.path.join(HERE, "lib", "libwrt.so")
 
 (WRT_OK, WRT_E_ARG, WRT_E_NODEV, WRT_E_HIP, WRT_E_NOMEM, WRT_E_ADMIT, WRT_E_IO, WRT_E_STATE,
  WRT_E_SCRATCH) = (0, -1, -2, -3, -4, -5, -6, -7, -8)
