@@ -482,7 +482,10 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
         bool ok;
         unsigned off;
         cell((int)(Q[0] >> 32), (int)(Q[1] >> 32), (int)(Q[2] >> 32), ok, off);
-        if (u < 2048u) {
+        const bool near = u < 2048u;
+        // a branch of the whole wave on the ballot (a scalar branch on VCC, no exec mask to
+        // save and restore in every step); each lane takes the exact result only if it is near
+        if (__builtin_amdgcn_ballot_w64(near) != 0ull) {
           // ... replaced near a cell boundary (fraction of Q below 2^-21) by the reference's
           // rounding, in binary64 (the per-column products b*i2 rounded as the reference
           // rounds delta*v2*i2); the step index through readfirstlane keeps the column out
@@ -516,11 +519,15 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
             iy = min(iy, cyh);
             iz = min(iz, czh);
           }
-          cell(ix - b0x, iy - b0y, iz - b0z, ok, off);
+          bool okx;
+          unsigned offx;
+          cell(ix - b0x, iy - b0y, iz - b0z, okx, offx);
           if (BND && (fabs(x) > 1.0 || fabs(y) > 1.0 || fabs(z) > 1.0)) {
-            ok = false;
-            if constexpr (!T) off = 0u;
+            okx = false;
+            if constexpr (!T) offx = 0u;
           }
+          ok = near ? okx : ok;
+          off = near ? offx : off;
         }
         if constexpr (T) {
           if (ok)
