@@ -590,9 +590,9 @@ def kernel_sources_sha():
     return h.hexdigest()
 
 
-def _profile_figure(path, key):
+def _profile_figure(path, key, tile=None):
     """the figure `key:` of a committed counter summary, if it was taken on the current
-    kernel sources (None otherwise)"""
+    kernel sources and, given tile = (band_rows, dgroup), with that tile (None otherwise)"""
     vals = {}
     try:
         for line in open(path):
@@ -603,17 +603,21 @@ def _profile_figure(path, key):
         return None
     if vals.get("kernel_sources_sha256") != kernel_sources_sha() or key not in vals:
         return None
+    if tile is not None and (vals.get("band_rows"), vals.get("dgroup")) != tuple(map(str, tile)):
+        return None
     return float(vals[key])
 
 
-def traffic_from_profile():
-    """HBM bytes per pass from the committed counter summary (None if absent or stale)."""
-    return _profile_figure(TRAFFIC_FILE, "traffic_bytes_per_pass")
+def traffic_from_profile(tile):
+    """HBM bytes per pass from the committed counter summary (None if absent, stale or
+    taken with another tile shape)."""
+    return _profile_figure(TRAFFIC_FILE, "traffic_bytes_per_pass", tile)
 
 
-def valu_from_profile():
-    """VALU wavefront instructions per pass from the committed counter summary"""
-    return _profile_figure(VALU_FILE, "valu_insts_per_pass")
+def valu_from_profile(tile):
+    """VALU wavefront instructions per pass from the committed counter summary (None if
+    absent, stale or taken with another tile shape)"""
+    return _profile_figure(VALU_FILE, "valu_insts_per_pass", tile)
 
 
 def main():
@@ -731,8 +735,8 @@ def main():
     # with the hash of the kernel sources it was taken on) over the pass time measured here.
     # The HBM figures beside it are the measured traffic and a byte model that prices every
     # in-cube sample's voxel read as an HBM read, which it is not.
-    valu = valu_from_profile()
-    traffic = traffic_from_profile()
+    valu = valu_from_profile((band_rows, dgroup))
+    traffic = traffic_from_profile((band_rows, dgroup))
     roofline = {
         "bound": "valu",
         "achieved": valu / kavg / 1e9 if valu else None,

@@ -611,7 +611,10 @@ static int radon_fast_time(wrt_job* j, double* sec) {
  * under the problem's shape (edge, directions, shifts, precision, non-empty bricks), where
  * later loads of that shape find it.  WRT_BAND_ROWS (4, 8, 16) and WRT_RADON_DGROUP (>= 16)
  * fix the shape and the group and skip the tuning; WRT_AUTOTUNE=0 keeps 8 x 8 and the load's
- * group. */
+ * group.  A candidate replaces the incumbent only when it is faster by more than
+ * WRT_TUNE_MARGIN, so that timing noise between near-equal shapes does not change the pick
+ * from one run to the next (the first candidate, 8 x 8 at the load's group, is the default). */
+#define WRT_TUNE_MARGIN 0.01
 static int radon_fast_tune(wrt_job* j) {
   radon_fast* f = (radon_fast*)j->fast;
   const wrt_job_desc* d = &j->d;
@@ -649,7 +652,7 @@ static int radon_fast_tune(wrt_job* j) {
     int rc = radon_fast_time(j, &sec); /* the first run of a variant loads its code ... */
     if (rc == WRT_OK) rc = radon_fast_time(j, &sec); /* ... so the second one is timed */
     if (rc != WRT_OK) return rc;
-    if (sec < best) {
+    if (sec < best * (1.0 - WRT_TUNE_MARGIN)) {
       best = sec;
       best_br = shapes[q];
     }
@@ -662,7 +665,7 @@ static int radon_fast_tune(wrt_job* j) {
     int rc = radon_fast_tasks(j, g);
     if (rc == WRT_OK) rc = radon_fast_time(j, &sec);
     if (rc != WRT_OK) return rc;
-    if (sec < best) {
+    if (sec < best * (1.0 - WRT_TUNE_MARGIN)) {
       best = sec;
       best_dg = g;
     }
