@@ -22,8 +22,10 @@ The "batch" object is the batched form: 48 seeded volumes with edges drawn from
 {32,48,64,96,128,160,192}, each with a forward pass (FP32 bricks, (e/2) phi x (e/4) GL theta
 x (e+1) shifts) and a backprojection (FP32) of a seeded Gaussian sinogram on the same grid,
 all in one persistent graph-captured launch (direction groups autotuned on the first call);
-the replay's time is compared with the same 96 jobs launched one by one, and with their
-plain FP64 forms.
+the replay's time is compared with the same 96 jobs launched one by one, with all 96 launched
+at once on their own streams, and with their plain FP64 forms.  Its "small_volumes" object
+is the same for 96 volumes with edges from {16,24,32,48,64}, where per-job launches are
+bound by launch and tail latency.
 
 The "bf16" object is the same pass from BF16 bricks (time, rate, its byte model, relL2
 against the plain FP64 subset): an accuracy-reduced storage form, not a faster one, since the
@@ -216,9 +218,9 @@ def rays_cpu(vol, z, s, phi, rays_in, rs, rphi, th, seconds):
     return out
 
 
-def batch_leg(ctx, wrt, nvol, seed=3):
+def batch_leg(ctx, wrt, nvol, seed=3, pool=(32, 48, 64, 96, 128, 160, 192)):
     rng = np.random.default_rng(seed)
-    edges = [int(e) for e in rng.choice([32, 48, 64, 96, 128, 160, 192], size=nvol)]
+    edges = [int(e) for e in rng.choice(list(pool), size=nvol)]
     specs = []
     for i, e in enumerate(edges):
         phi, th, w, s = ctx.radon_grid_init(e // 2, e // 4, e + 1)
@@ -236,7 +238,14 @@ def batch_leg(ctx, wrt, nvol, seed=3):
     dgroup = ctx.batch_dgroup()
     for j in jobs:             # the per-job launches' own tuning runs outside the timing
         j.run()
+    # one job at a time (each on the GPU alone), and every job at once on its own stream
     per_job = sum(j.run()["seconds"] for j in jobs)
+    t0 = time.perf_counter()
+    for j in jobs:
+        j.launch()
+    for j in jobs:
+        j.wait()
+    concurrent = time.perf_counter() - t0
     for j in jobs:
         j.close()
     plain = 0.0
@@ -249,8 +258,10 @@ def batch_leg(ctx, wrt, nvol, seed=3):
             "first_call_s": st0["kernel_seconds"], "graph_s": st["kernel_seconds"],
             "samples": st["logical_samples"],
             "samples_per_s": st["logical_samples"] / st["kernel_seconds"],
-            "per_job_fast_s": per_job, "plain_f64_s": plain,
+            "per_job_fast_s": per_job, "per_job_concurrent_s": concurrent,
+            "plain_f64_s": plain,
             "speedup_vs_per_job": per_job / st["kernel_seconds"],
+            "speedup_vs_concurrent": concurrent / st["kernel_seconds"],
             "speedup_vs_plain": plain / st["kernel_seconds"]}
 
 
@@ -813,7 +824,15 @@ def main():
         log("bench: batch")
         batch = batch_leg(ctx, wrt, args.batch_volumes)
         log(f"bench: batch {batch['graph_s']:.3f} s, {batch['speedup_vs_per_job']:.2f}x per-job, "
+            f"{batch['speedup_vs_concurrent']:.2f}x concurrent, "
             f"{batch['speedup_vs_plain']:.2f}x plain")
+        # many small volumes, where per-job launches are bound by launch and tail latency
+        batch["small_volumes"] = batch_leg(ctx, wrt, 2 * args.batch_volumes, seed=4,
+                                           pool=(16, 24, 32, 48, 64))
+        sv = batch["small_volumes"]
+        log(f"bench: batch of small volumes {sv['graph_s'] * 1e3:.2f} ms, "
+            f"{sv['speedup_vs_per_job']:.2f}x per-job, "
+            f"{sv['speedup_vs_concurrent']:.2f}x concurrent")
     # The spectral Laplacian last: once the process has made a hipFFT plan, the streams it
     # makes afterwards overlap less (the per-job face-brick side stream costs 5% per pass and
     # the serve rounds lose their overlap), so every other leg runs before it.

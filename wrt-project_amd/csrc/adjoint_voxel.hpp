@@ -21,11 +21,14 @@
 // result is unspecified and must not be stored); dirs / trig / cub as in wrt_batch_item.
 // Directions go in fours and every voxel's four cubic gathers are issued before any is
 // used, so a thread keeps 4V table loads in flight; each voxel's sum still runs in
-// direction order, so the result of a voxel does not depend on V.
-template <int V>
-__device__ __forceinline__ void adjoint_voxels_f32(const float4* __restrict__ dirs,
-                                                   const double4* __restrict__ trig, int ndir,
-                                                   const float4* __restrict__ cub, int ns,
+// direction order, so the result of a voxel does not depend on V.  The pointer types are
+// parameters so that a caller whose tables come from memory (the batch kernel's items) can
+// pass them in the constant and global address spaces, which the kernel arguments of the
+// per-job launch have by themselves: the wave-uniform direction reads then are scalar
+// loads and the cubic gathers global (not flat) loads.
+template <int V, class DirP, class TrigP, class CubP>
+__device__ __forceinline__ void adjoint_voxels_f32(DirP __restrict__ dirs, TrigP __restrict__ trig,
+                                                   int ndir, CubP __restrict__ cub, int ns,
                                                    float s0, float inv_ds, float ds, int n,
                                                    const long long* k, float* out) {
   double xd[V], yd[V], zd[V];

@@ -6,9 +6,11 @@
 // A workgroup takes the next task from a device counter, runs it, and takes another until
 // the list is exhausted, so uneven tasks balance across the CUs; the host orders the list
 // longest first.  A plane-integral task is the brick task of the per-job launch
-// (radon_brick.hpp) with the job's arguments; a backprojection task is up to 2048 voxels of
-// one job, two per thread (adjoint_voxel.hpp).  Results are the per-job launches' up to the
-// order of the FP64 accumulator atomics.
+// (radon_brick.hpp) with the job's arguments; a backprojection task is up to WRT_ADJ_TASK
+// voxels of one job, WRT_ADJ_VPT per thread (adjoint_voxel.hpp: more voxels per thread
+// than the per-job launch's two keep more gathers in flight at the one workgroup per CU the
+// brick task's LDS allows).  Results are the per-job launches' up to the order of the FP64
+// accumulator atomics.
 #include "adjoint_voxel.hpp"
 #include "radon_brick.hpp"
 
@@ -30,16 +32,6 @@ __global__ __launch_bounds__(256) void k_batch_epilogue(const wrt_batch_seg* __r
     sg.out[i] = (float)sg.acc[i];
 }
 
-template <bool BF16, bool POW2>
-__device__ __forceinline__ void radon_item(const wrt_batch_item& it, const wrt_batch_task& t,
-                                           radon_lds& L) {
-  const wrt_radon_fast_args ra = it.ra;
-  if (t.c)
-    radon_brick_task<true, BF16, POW2>(ra, t.a, t.b, it.h, it.dc, it.inv_dc, L);
-  else
-    radon_brick_task<false, BF16, POW2>(ra, t.a, t.b, it.h, it.dc, it.inv_dc, L);
-}
-
 __global__ __launch_bounds__(FR_WAVES * 64) void k_batch(const wrt_batch_item* __restrict__ items,
                                                          const wrt_batch_task* __restrict__ tasks,
                                                          int ntasks, int* __restrict__ counter) {
@@ -49,31 +41,35 @@ __global__ __launch_bounds__(FR_WAVES * 64) void k_batch(const wrt_batch_item* _
     __syncthreads();  // the previous task is done with L and s_task
     if (threadIdx.x == 0) s_task = atomicAdd(counter, 1);
     __syncthreads();
-    const int ti = s_task;
+    // wave-uniform, so that the task and its item are read into scalar registers
+    const int ti = __builtin_amdgcn_readfirstlane(s_task);
     if (ti >= ntasks) break;  // every workgroup reaches this: the counter only grows
     const wrt_batch_task t = tasks[ti];
     const wrt_batch_item& it = items[t.item];
     if (it.kind == WRT_ITEM_RADON) {
-      const bool bf16 = it.ra.bf16 != 0;
-      if (it.pow2) {
-        if (bf16) radon_item<true, true>(it, t, L);
-        else radon_item<false, true>(it, t, L);
-      } else {
-        if (bf16) radon_item<true, false>(it, t, L);
-        else radon_item<false, false>(it, t, L);
-      }
+      // one code path for every plane-integral task: face brick, BF16 and power-of-two edge
+      // are run-time flags (each inlined variant would add its own live ranges to the one
+      // register allocation of this kernel)
+      radon_brick_task<false>(it.ra, t.a, t.b, it.h, it.dc, it.inv_dc, L, t.c != 0,
+                              it.ra.bf16 != 0, it.pow2 != 0);
     } else {
-      // voxels a + tid and a + 1024 + tid of the task's (up to 2048) voxels
+      // voxels a + tid + 1024 v (v < WRT_ADJ_VPT) of the task's (up to WRT_ADJ_TASK)
+      // voxels: a thread's voxels past the count repeat its first, computed but not stored
       if ((int)threadIdx.x < t.b) {
-        long long k[2] = {(long long)t.a + threadIdx.x, (long long)t.a + 1024 + threadIdx.x};
-        const bool second = (int)threadIdx.x + 1024 < t.b;
-        if (!second) k[1] = k[0];  // computed twice, stored once
-        float r[2];
-        adjoint_voxels_f32<2>((const float4*)it.adirs, (const double4*)it.atrig, it.andir,
-                              (const float4*)it.acub, it.ans, (float)it.as0,
-                              (float)(1.0 / it.ads), (float)it.ads, it.an, k, r);
-        it.aout[k[0]] = r[0];
-        if (second) it.aout[k[1]] = r[1];
+        long long k[WRT_ADJ_VPT];
+#pragma unroll
+        for (int v = 0; v < WRT_ADJ_VPT; ++v)
+          k[v] = (long long)t.a + ((int)threadIdx.x + 1024 * v < t.b ? threadIdx.x + 1024 * v
+                                                                      : threadIdx.x);
+        float r[WRT_ADJ_VPT];
+        adjoint_voxels_f32<WRT_ADJ_VPT>(
+            (const WRT_CONST float4*)it.adirs, (const WRT_CONST double4*)it.atrig, it.andir,
+            (const WRT_GLOBAL float4*)it.acub, it.ans, (float)it.as0, (float)(1.0 / it.ads),
+            (float)it.ads, it.an, k, r);
+        WRT_GLOBAL float* const out = (WRT_GLOBAL float*)it.aout;
+#pragma unroll
+        for (int v = 0; v < WRT_ADJ_VPT; ++v)
+          if ((int)threadIdx.x + 1024 * v < t.b) out[k[v]] = r[v];
       }
     }
   }

@@ -7,8 +7,8 @@ template <bool BND, bool BF16, bool POW2, int BR>
 __global__ __launch_bounds__(FR_WAVES * 64) void k_radon_fast(wrt_radon_fast_args a, double h,
                                                               double dc, double inv_dc) {
   __shared__ radon_lds L;
-  radon_brick_task<BND, BF16, POW2, false, BR>(a, a.tasks[2 * blockIdx.x],
-                                               a.tasks[2 * blockIdx.x + 1], h, dc, inv_dc, L);
+  radon_brick_task<false, BR>(a, a.tasks[2 * blockIdx.x], a.tasks[2 * blockIdx.x + 1], h, dc,
+                              inv_dc, L, BND, BF16, POW2);
 }
 
 template <bool BND, bool BF16, int BR>
@@ -56,7 +56,7 @@ __global__ __launch_bounds__(FR_WAVES * 64) void k_radon_T_fast(wrt_radon_fast_a
   __shared__ radon_T_lds L;
   const int bh = a.tasks[2 * blockIdx.x], grp = a.tasks[2 * blockIdx.x + 1];
   const int brick = bh >> 1, zh = bh & 1;
-  radon_brick_task<BND, false, POW2, true>(a, brick, grp, h, dc, inv_dc, L, zh);
+  radon_brick_task<true>(a, brick, grp, h, dc, inv_dc, L, BND, false, POW2, zh);
   __syncthreads();
   const size_t nbr = (size_t)a.nb * a.nb * a.nb;
   long long* __restrict__ dst = a.part + ((size_t)grp * nbr + brick) * (FB * FB * FB) + FT_Z * zh;

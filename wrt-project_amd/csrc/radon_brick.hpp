@@ -117,10 +117,6 @@ __device__ __forceinline__ float uniform_f32(float v) {
   return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
 }
 
-__device__ __forceinline__ float bf16_to_f32(unsigned short v) {
-  return __uint_as_float(((unsigned)v) << 16);
-}
-
 // The LDS a brick task works in: the brick in the skewed layout plus one zero slot that
 // rejected samples read (branch-free accumulation), per wave the packed first/last column
 // of every row of its current section, and the workgroup's next direction.
@@ -150,10 +146,15 @@ struct radon_T_lds {
 // sum_planes q_p * (number of the plane's samples in the cell); the caller stores them.
 // Called by every thread of a FR_WAVES*64-thread workgroup; returns with the workgroup's
 // updates issued (no barrier at the end: a caller that reads or reuses L waits first).
-template <bool BND, bool BF16, bool POW2, bool T = false, int BR = 8, class LDS = radon_lds>
+// The variant flags are wave-uniform: BND the brick touches a cube face, BF16 the bricks
+// are stored as BF16, POW2 the edge n is a power of two.  The per-job launches pass them as
+// compile-time constants (one specialised kernel per variant); the persistent batch kernel
+// passes them per task at run time, which costs one scalar branch at the stage and in the
+// rare exact branch, so that one code path (one register allocation) serves every task.
+template <bool T = false, int BR = 8, class LDS = radon_lds>
 __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, int brick, int grp,
                                                  double h, double dc, double inv_dc, LDS& L,
-                                                 int zhalf = 0) {
+                                                 bool BND, bool BF16, bool POW2, int zhalf = 0) {
   constexpr int ZN = T ? FT_Z : FB;  // the box's extent in z
   static_assert(BR == 4 || BR == 8 || BR == 16, "bands of 4, 8 or 16 rows");
   static_assert(!T || BR == 8, "transpose mode walks 8-row bands");
@@ -163,6 +164,11 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
   int (*const rowlo)[136] = L.rowlo;
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  // the job's tables in the global address space (the batch kernel reads the argument
+  // block from memory, where the compiler does not know it)
+  const WRT_GLOBAL double* const g_dirs = (const WRT_GLOBAL double*)a.dirs;
+  const WRT_GLOBAL double* const g_shift = (const WRT_GLOBAL double*)a.shift;
+  const WRT_GLOBAL int* const g_sidx = (const WRT_GLOBAL int*)a.sidx;
   const int nb = a.nb, n = a.n;
   const int bxi = brick / (nb * nb), byi = (brick / nb) % nb, bzi = brick % nb;
   const int b0x = bxi * FB, b0y = byi * FB, b0z = bzi * FB + (T ? FT_Z * zhalf : 0);
@@ -172,14 +178,21 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
     for (int i = threadIdx.x; i < FB * FT_PX; i += blockDim.x) L.acc[i] = 0ull;
     if (threadIdx.x == 0) L.next_dir = grp * a.dgroup;
   } else {
+    // 32-bit words: one FP32 voxel, or two BF16 voxels (z even and odd of one row; the
+    // low half is the even one)
     float* const sb = L.sb;
-    const size_t base = (size_t)brick * (FB * FB * FB);
-    for (int i = threadIdx.x; i < FB * FB * FB; i += blockDim.x) {
-      const int lx = i >> 10, ly = (i >> 5) & 31, lz = i & 31;
-      float v;
-      if (BF16) v = bf16_to_f32(((const unsigned short*)a.bricks)[base + i]);
-      else v = ((const float*)a.bricks)[base + i];
-      sb[1 + lx * FR_PX + ly * FR_PY + lz] = v;
+    const int nw = BF16 ? FB * FB * FB / 2 : FB * FB * FB;
+    const WRT_GLOBAL unsigned* const src = (const WRT_GLOBAL unsigned*)a.bricks + (size_t)brick * nw;
+    for (int i = threadIdx.x; i < nw; i += blockDim.x) {
+      const unsigned w = src[i];
+      const int v = BF16 ? 2 * i : i;  // the (first) voxel's index in the brick
+      float* const p = sb + 1 + (v >> 10) * FR_PX + ((v >> 5) & 31) * FR_PY + (v & 31);
+      if (BF16) {
+        p[0] = __uint_as_float(w << 16);
+        p[1] = __uint_as_float(w & 0xffff0000u);
+      } else {
+        p[0] = __uint_as_float(w);
+      }
     }
     if (threadIdx.x == 0) {
       sb[0] = 0.f;
@@ -249,7 +262,7 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
     if (lane == 0) d = atomicAdd(&L.next_dir, 1);
     d = __builtin_amdgcn_readfirstlane(d);
     if (d >= d_end) break;
-    const double* D = a.dirs + 9 * (size_t)d;
+    const WRT_GLOBAL double* D = g_dirs + 9 * (size_t)d;
     const double nrm[3] = {uniform_f64(D[0]), uniform_f64(D[1]), uniform_f64(D[2])};
     const double av[3] = {uniform_f64(D[3]), uniform_f64(D[4]), uniform_f64(D[5])};
     const double bv[3] = {uniform_f64(D[6]), uniform_f64(D[7]), uniform_f64(D[8])};
@@ -281,10 +294,10 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
     int j0, j1;
     {
       int l = 0, r = a.nshift;
-      while (l < r) { const int m = (l + r) >> 1; if (a.shift[m] < pn0 - eps) l = m + 1; else r = m; }
+      while (l < r) { const int m = (l + r) >> 1; if (g_shift[m] < pn0 - eps) l = m + 1; else r = m; }
       j0 = l;
       l = 0; r = a.nshift;
-      while (l < r) { const int m = (l + r) >> 1; if (a.shift[m] <= pn1 + eps) l = m + 1; else r = m; }
+      while (l < r) { const int m = (l + r) >> 1; if (g_shift[m] <= pn1 + eps) l = m + 1; else r = m; }
       j1 = l - 1;
     }
     if (j0 > j1) continue;
@@ -338,15 +351,15 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
                          (ax == 0 ? bv[0] : (ax == 1 ? bv[1] : bv[2])) * (double)C0) + 1.0) * h -
                        (ax == 0 ? fb0x : (ax == 1 ? fb0y : fb0z));
     for (int j = j0; j <= j1; ++j) {
-      const int jo = a.sidx[j];  // the plane's row in the caller's shift order
+      const int jo = g_sidx[j];  // the plane's row in the caller's shift order
       unsigned long long gq = 0ull;  // transpose: the section's quantized value
       if constexpr (T) {
-        const float gv = a.g[(size_t)jo * a.ndir + d];
+        const float gv = ((const WRT_GLOBAL float*)a.g)[(size_t)jo * a.ndir + d];
         if (gv == 0.f) continue;
         gq = (unsigned long long)(long long)rint((double)gv * a.gscale);
         if (gq == 0ull) continue;
       }
-      const double s = a.shift[j];
+      const double s = g_shift[j];
       // the plane meets the support sphere only for s^2 <= rho^2
       if (a.rho2 - s * s < 0.0) continue;
       const float sf = (float)s;
@@ -570,6 +583,7 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
         if (steps <= 0) continue;
         if constexpr (T) {
           const int steps2 = (steps + 1) >> 1;  // 16-column steps cover the 8-column ones
+#pragma unroll
           for (int dcol = 0; dcol < 2; ++dcol) {
             const int cb = dcol + 2 * k_lane, c0 = cb - nrl;  // c0: first column i2 of the lane
             int tq = t;  // as in the forward walk below
@@ -606,7 +620,8 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
         // the plane's FP64 accumulator
         const double tot = (double)wave_sum_f32(bs);
         if (lane == 0 && tot != 0.0)
-          unsafeAtomicAdd(a.acc + (size_t)jo * a.ndir + d, tot * delta * delta);
+          __hip_atomic_fetch_add((WRT_GLOBAL double*)a.acc + (size_t)jo * a.ndir + d,
+                                 tot * delta * delta, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
     }
   }

@@ -10,6 +10,19 @@
 
 #define WRT_HOSTDEV __host__ __device__ __forceinline__
 
+// Address spaces for pointers a kernel reads from memory (the persistent batch kernel's
+// argument blocks), where the compiler cannot infer them as it does for kernel arguments:
+// global (vector loads and atomics without the flat path's extra waits) and constant
+// (wave-uniform reads of tables the kernel never writes become scalar loads).
+// (The host pass, which only checks device code, sees them as plain pointers.)
+#if defined(__HIP_DEVICE_COMPILE__)
+#define WRT_GLOBAL __attribute__((address_space(1)))
+#define WRT_CONST __attribute__((address_space(4)))
+#else
+#define WRT_GLOBAL
+#define WRT_CONST
+#endif
+
 // fmod(a, d) for a >= 0, d > 0, exact (equal to the C library's fmod).  q = trunc(a/d) is
 // the true quotient k or k+1 (a rounded-up quotient); a - k*d is representable, so the
 // fma that forms it is exact.

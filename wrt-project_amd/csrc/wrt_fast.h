@@ -98,8 +98,12 @@ typedef struct {
   double as0, ads;          /* adjoint: uniform shift grid s_i = s0 + i*ds               */
 } wrt_batch_item;
 
+/* voxels per thread of a backprojection task in the persistent kernel (1024 threads) */
+#define WRT_ADJ_VPT 3
+#define WRT_ADJ_TASK (1024 * WRT_ADJ_VPT)
+
 /* radon: (item, brick, direction group, 1 if the brick touches a cube face);
- * adjoint: (item, first voxel, voxel count <= 2048, 0) */
+ * adjoint: (item, first voxel, voxel count <= WRT_ADJ_TASK, 0) */
 typedef struct {
   int item, a, b, c;
 } wrt_batch_task;

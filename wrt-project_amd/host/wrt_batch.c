@@ -7,7 +7,7 @@
  * generation stay the same):
  *   items    one per batched job: its arguments (wrt_fast_describe);
  *   tasks    plane integrals: (job, non-empty brick, direction group of G directions);
- *            backprojections: (job, 2048 consecutive voxels);
+ *            backprojections: (job, WRT_ADJ_TASK consecutive voxels);
  *            ordered by an estimate of their device time, longest first, so that the
  *            persistent workgroups, which take tasks from a counter, finish together;
  *   segments the FP64 accumulators the prologue zeroes and the epilogue rounds to FP32;
@@ -105,11 +105,11 @@ long long wrt_item_tasks(const wrt_fast_desc* fd, int item, int G, wrt_costed_ta
       }
     return k;
   }
-  const long long count = (fd->nvox + 2047) / 2048;
+  const long long count = (fd->nvox + WRT_ADJ_TASK - 1) / WRT_ADJ_TASK;
   if (!ct) return count;
   long long k = 0;
-  for (long long v = 0; v < fd->nvox; v += 2048) {
-    const int cnt = (int)(fd->nvox - v < 2048 ? fd->nvox - v : 2048);
+  for (long long v = 0; v < fd->nvox; v += WRT_ADJ_TASK) {
+    const int cnt = (int)(fd->nvox - v < WRT_ADJ_TASK ? fd->nvox - v : WRT_ADJ_TASK);
     ct[k].cost = WRT_COST_VOXEL_DIR * cnt * it->andir;
     ct[k].t.item = item;
     ct[k].t.a = (int)v;
@@ -273,7 +273,7 @@ static int plan_build(wrt_ctx* c, wrt_job* const* jobs, int njobs, batch_plan** 
     if (cached) {
       G = cached;
     } else if (tune) {
-      static const int cand[] = {16, 32, 64, 128, 256};
+      static const int cand[] = {16, 32, 64, 128, 256, 512};
       double best = 1e300;
       for (int q = 0; rc == WRT_OK && q < (int)(sizeof(cand) / sizeof(cand[0])); ++q) {
         double sec = 0;
