@@ -17,37 +17,50 @@
 #define WRT_ADJOINT_VOXEL_HPP
 #include "wrt_common.hpp"
 
-// V voxels per thread, k[0..V-1] of the [z][x][y] output (valid[v] = 0 skips one: its
-// result is unspecified and must not be stored); dirs / trig / cub as in wrt_batch_item.
-// Directions go in fours and every voxel's four cubic gathers are issued before any is
-// used, so a thread keeps 4V table loads in flight; each voxel's sum still runs in
-// direction order, so the result of a voxel does not depend on V.  The pointer types are
-// parameters so that a caller whose tables come from memory (the batch kernel's items) can
-// pass them in the constant and global address spaces, which the kernel arguments of the
-// per-job launch have by themselves: the wave-uniform direction reads then are scalar
-// loads and the cubic gathers global (not flat) loads.
-template <int V, class DirP, class TrigP, class CubP>
-__device__ __forceinline__ void adjoint_voxels_f32(DirP __restrict__ dirs, TrigP __restrict__ trig,
-                                                   int ndir, CubP __restrict__ cub, int ns,
-                                                   float s0, float inv_ds, float ds, int n,
-                                                   const long long* k, float* out) {
-  double xd[V], yd[V], zd[V];
-  float x[V], y[V], z[V], acc[V], comp[V];
+// The voxels go to wavefronts in tiles of 4 (x) x 4 (y) x 4V (z): lane l takes x offset
+// (l >> 2) & 3, y offset l & 3 and z offsets (l >> 4) + 4v, v < V.  A tile is compact, so
+// for each direction its voxels' shifts fall in a few neighbouring spline intervals (the
+// wavefront's cubic gathers touch few cache lines), and most tiles lie wholly inside or
+// wholly outside the unit ball.  Tiles are numbered z-major like the output:
+// ((tz * ntx) + tx) * ntx + ty with ntx = ceil(n / 4) tiles along x and y and
+// ceil(n / 4V) along z (wrt_adj_tiles).  A tile past the volume's edge clamps its coordinates
+// into the volume (computed, not stored: valid[v] = false).
+template <int V>
+__device__ __forceinline__ void adjoint_tile_voxels(long long tile, int lane, int n, long long* k,
+                                                    bool* valid) {
+  const int ntx = (n + 3) >> 2;
+  const long long txy = (long long)ntx * ntx;
+  const int tz = (int)(tile / txy);
+  const int rem = (int)(tile - (long long)tz * txy);
+  const int tx = rem / ntx, ty = rem - tx * ntx;
+  const int ix = 4 * tx + ((lane >> 2) & 3), iy = 4 * ty + (lane & 3);
 #pragma unroll
   for (int v = 0; v < V; ++v) {
-    const int iy = (int)(k[v] % n);
-    const int ix = (int)((k[v] / n) % n);
-    const int iz = (int)(k[v] / ((long long)n * n));
-    xd[v] = (-1.0) + (1.0 / n) + ix * (2.0 / n);
-    yd[v] = (-1.0) + (1.0 / n) + iy * (2.0 / n);
-    zd[v] = (-1.0) + (1.0 / n) + iz * (2.0 / n);
-    x[v] = (float)xd[v];
-    y[v] = (float)yd[v];
-    z[v] = (float)zd[v];
+    const int iz = 4 * V * tz + (lane >> 4) + 4 * v;
+    valid[v] = ix < n && iy < n && iz < n;
+    k[v] = ((long long)min(iz, n - 1) * n + min(ix, n - 1)) * n + min(iy, n - 1);
+  }
+}
+
+// The sum over directions for V voxels per thread.  INSIDE: every voxel of the wavefront
+// lies within radius 1 - 1e-4 of the origin, where |s| <= 1 holds for every direction and
+// |s| is farther than 1e-5 from 1 (the FP32 s errs by under 1e-6), so the cut and its FP64
+// re-decision are skipped: the same result, bit for bit, with fewer instructions.
+template <int V, bool INSIDE, class DirP, class TrigP, class CubP>
+__device__ __forceinline__ void adjoint_dir_sum(DirP __restrict__ dirs, TrigP __restrict__ trig,
+                                                int ndir, CubP __restrict__ cub, int ns,
+                                                float s0, float inv_ds, const double* xd,
+                                                const double* yd, const double* zd,
+                                                const float* x, const float* y, const float* z,
+                                                float* out) {
+  float acc[V], comp[V];
+#pragma unroll
+  for (int v = 0; v < V; ++v) {
     acc[v] = 0.f;
     comp[v] = 0.f;
   }
   const int last = ns - 2;
+  const float lastf = (float)last;  // exact: ns < 2^24
   const float u0 = -s0 * inv_ds;  // u = s * inv_ds + u0 = (s - s0) / ds
   int d = 0;
   for (; d + 4 <= ndir; d += 4) {
@@ -63,23 +76,30 @@ __device__ __forceinline__ void adjoint_voxels_f32(DirP __restrict__ dirs, TrigP
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const float s = fmaf(x[v], D[u].x, fmaf(y[v], D[u].y, z[v] * D[u].z));
-        bool in = fabsf(s) <= 1.0f;
-        if (fabsf(fabsf(s) - 1.0f) <= 1e-5f) {
-          const double4 T = trig[d + u];
-          const double sd = xd[v] * T.x * T.y + yd[v] * T.x * T.z + zd[v] * T.w;
-          in = !(fabs(sd) > 1.0);
+        bool in = true;
+        if (!INSIDE) {
+          in = fabsf(s) <= 1.0f;
+          if (fabsf(fabsf(s) - 1.0f) <= 1e-5f) {
+            const double4 T = trig[d + u];
+            const double sd = xd[v] * T.x * T.y + yd[v] * T.x * T.z + zd[v] * T.w;
+            in = !(fabs(sd) > 1.0);
+          }
         }
         const float uu = fmaf(s, inv_ds, u0);
-        int i = (int)uu;
-        i = i < 0 ? 0 : (i > last ? last : i);  // a valid interval even where s is cut
+        // the interval, truncated and clamped into [0, last] (a valid interval even where s
+        // is cut): truncating the clamped uu is the same, in one med3 and one conversion
+        const int i = (int)__builtin_amdgcn_fmed3f(uu, 0.0f, lastf);
         sv[v][u] = uu - (float)i;  // tt in [0, 1] where the sample is kept
         iv[v][u] = i;
         inv[v][u] = in;
       }
+    // each direction's row of cubics from its scalar base, the interval as a 32-bit offset
 #pragma unroll
-    for (int v = 0; v < V; ++v)
+    for (int u = 0; u < 4; ++u) {
+      const auto row = cub + (long long)(d + u) * (ns - 1);
 #pragma unroll
-      for (int u = 0; u < 4; ++u) Cv[v][u] = cub[(long long)(d + u) * (ns - 1) + iv[v][u]];
+      for (int v = 0; v < V; ++v) Cv[v][u] = row[(unsigned)iv[v][u]];
+    }
 #pragma unroll
     for (int v = 0; v < V; ++v) {
       float g = 0.f;  // the four directions' terms, in direction order
@@ -88,7 +108,7 @@ __device__ __forceinline__ void adjoint_voxels_f32(DirP __restrict__ dirs, TrigP
         const float t = sv[v][u];
         const float4 C = Cv[v][u];
         const float val = fmaf(fmaf(fmaf(C.w, t, C.z), t, C.y), t, C.x);
-        g = fmaf(D[u].w, inv[v][u] ? val : 0.f, g);
+        g = fmaf(D[u].w, INSIDE || inv[v][u] ? val : 0.f, g);
       }
       const float yk = g - comp[v];
       const float tk = acc[v] + yk;
@@ -101,18 +121,20 @@ __device__ __forceinline__ void adjoint_voxels_f32(DirP __restrict__ dirs, TrigP
 #pragma unroll
     for (int v = 0; v < V; ++v) {
       const float s = fmaf(x[v], D.x, fmaf(y[v], D.y, z[v] * D.z));
-      bool in = fabsf(s) <= 1.0f;
-      if (fabsf(fabsf(s) - 1.0f) <= 1e-5f) {
-        const double4 T = trig[d];
-        const double sd = xd[v] * T.x * T.y + yd[v] * T.x * T.z + zd[v] * T.w;
-        in = !(fabs(sd) > 1.0);
+      bool in = true;
+      if (!INSIDE) {
+        in = fabsf(s) <= 1.0f;
+        if (fabsf(fabsf(s) - 1.0f) <= 1e-5f) {
+          const double4 T = trig[d];
+          const double sd = xd[v] * T.x * T.y + yd[v] * T.x * T.z + zd[v] * T.w;
+          in = !(fabs(sd) > 1.0);
+        }
       }
       if (in) {
         const float uu = fmaf(s, inv_ds, u0);
-        int i = (int)uu;
-        i = i < 0 ? 0 : (i > last ? last : i);
+        const int i = (int)__builtin_amdgcn_fmed3f(uu, 0.0f, lastf);
         const float t = uu - (float)i;
-        const float4 C = cub[(long long)d * (ns - 1) + i];
+        const float4 C = cub[(long long)d * (ns - 1) + (unsigned)i];
         const float val = fmaf(fmaf(fmaf(C.w, t, C.z), t, C.y), t, C.x);
         const float yk = fmaf(D.w, val, -comp[v]);
         const float tk = acc[v] + yk;
@@ -123,6 +145,43 @@ __device__ __forceinline__ void adjoint_voxels_f32(DirP __restrict__ dirs, TrigP
   }
 #pragma unroll
   for (int v = 0; v < V; ++v) out[v] = acc[v];
+}
+
+// V voxels per thread, k[0..V-1] of the [z][x][y] output; dirs / trig / cub as in
+// wrt_batch_item.  Called by all 64 lanes of a wavefront (the inside test is a ballot).
+// Directions go in fours and every voxel's four cubic gathers are issued before any is
+// used, so a thread keeps 4V table loads in flight; each voxel's sum still runs in
+// direction order, so the result of a voxel does not depend on V or on the tiling.  The
+// pointer types are parameters so that a caller whose tables come from memory (the batch
+// kernel's items) can pass them in the constant and global address spaces, which the
+// kernel arguments of the per-job launch have by themselves: the wave-uniform direction
+// reads then are scalar loads and the cubic gathers global (not flat) loads.
+template <int V, class DirP, class TrigP, class CubP>
+__device__ __forceinline__ void adjoint_voxels_f32(DirP __restrict__ dirs, TrigP __restrict__ trig,
+                                                   int ndir, CubP __restrict__ cub, int ns,
+                                                   float s0, float inv_ds, float ds, int n,
+                                                   const long long* k, float* out) {
+  (void)ds;
+  double xd[V], yd[V], zd[V];
+  float x[V], y[V], z[V];
+  bool inside = true;
+#pragma unroll
+  for (int v = 0; v < V; ++v) {
+    const int iy = (int)(k[v] % n);
+    const int ix = (int)((k[v] / n) % n);
+    const int iz = (int)(k[v] / ((long long)n * n));
+    xd[v] = (-1.0) + (1.0 / n) + ix * (2.0 / n);
+    yd[v] = (-1.0) + (1.0 / n) + iy * (2.0 / n);
+    zd[v] = (-1.0) + (1.0 / n) + iz * (2.0 / n);
+    x[v] = (float)xd[v];
+    y[v] = (float)yd[v];
+    z[v] = (float)zd[v];
+    inside = inside && xd[v] * xd[v] + yd[v] * yd[v] + zd[v] * zd[v] <= (1.0 - 1e-4) * (1.0 - 1e-4);
+  }
+  if (__builtin_amdgcn_ballot_w64(!inside) == 0ull)
+    adjoint_dir_sum<V, true>(dirs, trig, ndir, cub, ns, s0, inv_ds, xd, yd, zd, x, y, z, out);
+  else
+    adjoint_dir_sum<V, false>(dirs, trig, ndir, cub, ns, s0, inv_ds, xd, yd, zd, x, y, z, out);
 }
 
 #endif

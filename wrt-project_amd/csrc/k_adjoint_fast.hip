@@ -3,6 +3,7 @@
 // the FP64 spline rows, and the per-job launch of the voxel body (adjoint_voxel.hpp).
 #include "adjoint_voxel.hpp"
 #include "wrt_dev.h"
+#include "wrt_fast.h"
 
 // (y, c) packed FP64 rows [row][ns][2] -> per-interval FP32 cubics [row][ns-1] in the
 // interval's unit variable tt = (s - x_i) / h: y + (b h) tt + (c h^2) tt^2 + (d h^3) tt^3
@@ -34,31 +35,30 @@ extern "C" int wrtk_adjoint_cubics(const double* x, int ns, const double* tab, v
   return WRT_OK;
 }
 
-// two voxels per thread (k and k + half of the block's span) of the [z][x][y] output
+// one tile of 4 x 4 x 8 voxels per wavefront (two voxels per thread), four per workgroup
 __global__ __launch_bounds__(256) void k_adjoint_f32(const float4* __restrict__ dirs,
                                                      const double4* __restrict__ trig, int ndir,
                                                      const float4* __restrict__ cub, int ns,
                                                      float s0, float inv_ds, float ds, int n,
-                                                     float* __restrict__ out) {
-  const long long total = (long long)n * n * n;
-  const long long k0 = (long long)blockIdx.x * 512 + threadIdx.x;
-  long long k[2] = {k0, k0 + 256};
-  if (k[0] >= total) return;
-  const bool second = k[1] < total;
-  if (!second) k[1] = k[0];  // computed twice, stored once
+                                                     long long ntiles, float* __restrict__ out) {
+  const long long tile = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (tile >= ntiles) return;  // the whole wavefront
+  long long k[2];
+  bool valid[2];
+  adjoint_tile_voxels<2>(tile, threadIdx.x & 63, n, k, valid);
   float r[2];
   adjoint_voxels_f32<2>(dirs, trig, ndir, cub, ns, s0, inv_ds, ds, n, k, r);
-  out[k[0]] = r[0];
-  if (second) out[k[1]] = r[1];
+  if (valid[0]) out[k[0]] = r[0];
+  if (valid[1]) out[k[1]] = r[1];
 }
 
 extern "C" int wrtk_adjoint_f32(const void* dirs, const void* trig, int ndir, const void* cub,
                                 int ns, double s0, double ds, int n, float* out, hipStream_t st) {
-  const long long total = (long long)n * n * n;
-  if (total <= 0) return WRT_OK;
-  k_adjoint_f32<<<dim3((unsigned)((total + 511) / 512)), 256, 0, st>>>(
-      (const float4*)dirs, (const double4*)trig, ndir, (const float4*)cub, ns, (float)s0, (float)(1.0 / ds), (float)ds,
-      n, out);
+  if (n <= 0) return WRT_OK;
+  const long long ntiles = wrt_adj_tiles(n, 2);
+  k_adjoint_f32<<<dim3((unsigned)((ntiles + 3) / 4)), 256, 0, st>>>(
+      (const float4*)dirs, (const double4*)trig, ndir, (const float4*)cub, ns, (float)s0,
+      (float)(1.0 / ds), (float)ds, n, ntiles, out);
   WRT_LAUNCH_CHECK();
   return WRT_OK;
 }

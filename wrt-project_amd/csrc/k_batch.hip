@@ -6,10 +6,10 @@
 // A workgroup takes the next task from a device counter, runs it, and takes another until
 // the list is exhausted, so uneven tasks balance across the CUs; the host orders the list
 // longest first.  A plane-integral task is the brick task of the per-job launch
-// (radon_brick.hpp) with the job's arguments; a backprojection task is up to WRT_ADJ_TASK
-// voxels of one job, WRT_ADJ_VPT per thread (adjoint_voxel.hpp: more voxels per thread
-// than the per-job launch's two keep more gathers in flight at the one workgroup per CU the
-// brick task's LDS allows).  Results are the per-job launches' up to the order of the FP64
+// (radon_brick.hpp) with the job's arguments; a backprojection task is up to
+// WRT_ADJ_TASK_TILES voxel tiles of one job, one per wavefront, WRT_ADJ_VPT voxels per thread
+// (adjoint_voxel.hpp: more voxels per thread than the per-job launch's two keep more gathers
+// in flight at the one workgroup per CU the brick task's LDS allows).  Results are the per-job launches' up to the order of the FP64
 // accumulator atomics.
 #include "adjoint_voxel.hpp"
 #include "radon_brick.hpp"
@@ -53,14 +53,12 @@ __global__ __launch_bounds__(FR_WAVES * 64) void k_batch(const wrt_batch_item* _
       radon_brick_task<false>(it.ra, t.a, t.b, it.h, it.dc, it.inv_dc, L, t.c != 0,
                               it.ra.bf16 != 0, it.pow2 != 0);
     } else {
-      // voxels a + tid + 1024 v (v < WRT_ADJ_VPT) of the task's (up to WRT_ADJ_TASK)
-      // voxels: a thread's voxels past the count repeat its first, computed but not stored
-      if ((int)threadIdx.x < t.b) {
+      // tile a + w for wavefront w < b of the task's tiles (one tile per wavefront)
+      const int w = threadIdx.x >> 6;
+      if (w < t.b) {
         long long k[WRT_ADJ_VPT];
-#pragma unroll
-        for (int v = 0; v < WRT_ADJ_VPT; ++v)
-          k[v] = (long long)t.a + ((int)threadIdx.x + 1024 * v < t.b ? threadIdx.x + 1024 * v
-                                                                      : threadIdx.x);
+        bool valid[WRT_ADJ_VPT];
+        adjoint_tile_voxels<WRT_ADJ_VPT>((long long)t.a + w, threadIdx.x & 63, it.an, k, valid);
         float r[WRT_ADJ_VPT];
         adjoint_voxels_f32<WRT_ADJ_VPT>(
             (const WRT_CONST float4*)it.adirs, (const WRT_CONST double4*)it.atrig, it.andir,
@@ -69,7 +67,7 @@ __global__ __launch_bounds__(FR_WAVES * 64) void k_batch(const wrt_batch_item* _
         WRT_GLOBAL float* const out = (WRT_GLOBAL float*)it.aout;
 #pragma unroll
         for (int v = 0; v < WRT_ADJ_VPT; ++v)
-          if ((int)threadIdx.x + 1024 * v < t.b) out[k[v]] = r[v];
+          if (valid[v]) out[k[v]] = r[v];
       }
     }
   }

@@ -98,12 +98,19 @@ typedef struct {
   double as0, ads;          /* adjoint: uniform shift grid s_i = s0 + i*ds               */
 } wrt_batch_item;
 
-/* voxels per thread of a backprojection task in the persistent kernel (1024 threads) */
+/* Backprojection voxels go to wavefronts in tiles of 4 (x) x 4 (y) x 4*vpt (z) voxels,
+ * vpt voxels per thread (adjoint_voxel.hpp); the tile count of an n^3 volume: */
+static inline long long wrt_adj_tiles(int n, int vpt) {
+  const long long txy = (long long)((n + 3) / 4);
+  return txy * txy * (long long)((n + 4 * vpt - 1) / (4 * vpt));
+}
+/* voxels per thread of a backprojection task in the persistent kernel; a task is up to one
+ * tile per wavefront of its 1024-thread workgroup */
 #define WRT_ADJ_VPT 3
-#define WRT_ADJ_TASK (1024 * WRT_ADJ_VPT)
+#define WRT_ADJ_TASK_TILES 16
 
 /* radon: (item, brick, direction group, 1 if the brick touches a cube face);
- * adjoint: (item, first voxel, voxel count <= WRT_ADJ_TASK, 0) */
+ * adjoint: (item, first tile, tile count <= WRT_ADJ_TASK_TILES, 0) */
 typedef struct {
   int item, a, b, c;
 } wrt_batch_task;

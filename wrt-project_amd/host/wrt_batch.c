@@ -7,7 +7,7 @@
  * generation stay the same):
  *   items    one per batched job: its arguments (wrt_fast_describe);
  *   tasks    plane integrals: (job, non-empty brick, direction group of G directions);
- *            backprojections: (job, WRT_ADJ_TASK consecutive voxels);
+ *            backprojections: (job, up to WRT_ADJ_TASK_TILES voxel tiles);
  *            ordered by an estimate of their device time, longest first, so that the
  *            persistent workgroups, which take tasks from a counter, finish together;
  *   segments the FP64 accumulators the prologue zeroes and the epilogue rounds to FP32;
@@ -105,12 +105,13 @@ long long wrt_item_tasks(const wrt_fast_desc* fd, int item, int G, wrt_costed_ta
       }
     return k;
   }
-  const long long count = (fd->nvox + WRT_ADJ_TASK - 1) / WRT_ADJ_TASK;
+  const long long ntiles = wrt_adj_tiles(it->an, WRT_ADJ_VPT);
+  const long long count = (ntiles + WRT_ADJ_TASK_TILES - 1) / WRT_ADJ_TASK_TILES;
   if (!ct) return count;
   long long k = 0;
-  for (long long v = 0; v < fd->nvox; v += WRT_ADJ_TASK) {
-    const int cnt = (int)(fd->nvox - v < WRT_ADJ_TASK ? fd->nvox - v : WRT_ADJ_TASK);
-    ct[k].cost = WRT_COST_VOXEL_DIR * cnt * it->andir;
+  for (long long v = 0; v < ntiles; v += WRT_ADJ_TASK_TILES) {
+    const int cnt = (int)(ntiles - v < WRT_ADJ_TASK_TILES ? ntiles - v : WRT_ADJ_TASK_TILES);
+    ct[k].cost = WRT_COST_VOXEL_DIR * cnt * (64.0 * WRT_ADJ_VPT) * it->andir;
     ct[k].t.item = item;
     ct[k].t.a = (int)v;
     ct[k].t.b = cnt;
