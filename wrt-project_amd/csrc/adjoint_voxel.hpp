@@ -42,17 +42,21 @@ __device__ __forceinline__ void adjoint_tile_voxels(long long tile, int lane, in
   }
 }
 
-// The sum over directions for V voxels per thread.  INSIDE: every voxel of the wavefront
-// lies within radius 1 - 1e-4 of the origin, where |s| <= 1 holds for every direction and
-// |s| is farther than 1e-5 from 1 (the FP32 s errs by under 1e-6), so the cut and its FP64
-// re-decision are skipped: the same result, bit for bit, with fewer instructions.
-template <int V, bool INSIDE, class DirP, class TrigP, class CubP>
+// The sum over directions for V voxels per thread.  MODE 0: the general case.  MODE >= 1
+// (INSIDE): every voxel of the wavefront lies within radius 1 - 1e-4 of the origin, where
+// |s| <= 1 holds for every direction and |s| is farther than 1e-5 from 1 (the FP32 s errs by
+// under 1e-6), so the cut and its FP64 re-decision are skipped.  MODE 2: besides, the shift
+// grid covers [-1 + 5e-5, 1 - 5e-5], so u = (s - s0)/ds of such an s lies in [0, ns - 1) and
+// needs no clamp: the interval is trunc(u) and tt = fract(u) (exact: u - floor(u) is
+// representable).  Every mode gives the same result, bit for bit, with fewer instructions.
+template <int V, int MODE, class DirP, class TrigP, class CubP>
 __device__ __forceinline__ void adjoint_dir_sum(DirP __restrict__ dirs, TrigP __restrict__ trig,
                                                 int ndir, CubP __restrict__ cub, int ns,
                                                 float s0, float inv_ds, const double* xd,
                                                 const double* yd, const double* zd,
                                                 const float* x, const float* y, const float* z,
                                                 float* out) {
+  constexpr bool INSIDE = MODE >= 1;
   float acc[V], comp[V];
 #pragma unroll
   for (int v = 0; v < V; ++v) {
@@ -86,11 +90,16 @@ __device__ __forceinline__ void adjoint_dir_sum(DirP __restrict__ dirs, TrigP __
           }
         }
         const float uu = fmaf(s, inv_ds, u0);
-        // the interval, truncated and clamped into [0, last] (a valid interval even where s
-        // is cut): truncating the clamped uu is the same, in one med3 and one conversion
-        const int i = (int)__builtin_amdgcn_fmed3f(uu, 0.0f, lastf);
-        sv[v][u] = uu - (float)i;  // tt in [0, 1] where the sample is kept
-        iv[v][u] = i;
+        if (MODE == 2) {
+          iv[v][u] = (int)uu;
+          sv[v][u] = __builtin_amdgcn_fractf(uu);
+        } else {
+          // the interval, truncated and clamped into [0, last] (a valid interval even where
+          // s is cut): truncating the clamped uu is the same, in one med3 and one conversion
+          const int i = (int)__builtin_amdgcn_fmed3f(uu, 0.0f, lastf);
+          sv[v][u] = uu - (float)i;  // tt in [0, 1] where the sample is kept
+          iv[v][u] = i;
+        }
         inv[v][u] = in;
       }
     // each direction's row of cubics from its scalar base, the interval as a 32-bit offset
@@ -156,12 +165,13 @@ __device__ __forceinline__ void adjoint_dir_sum(DirP __restrict__ dirs, TrigP __
 // kernel's items) can pass them in the constant and global address spaces, which the
 // kernel arguments of the per-job launch have by themselves: the wave-uniform direction
 // reads then are scalar loads and the cubic gathers global (not flat) loads.
-template <int V, class DirP, class TrigP, class CubP>
+// MODE1 = false leaves out mode 1 (inside tiles on a grid that does not cover [-1, 1] take
+// the general mode), for a caller short of registers.
+template <int V, bool MODE1 = true, class DirP, class TrigP, class CubP>
 __device__ __forceinline__ void adjoint_voxels_f32(DirP __restrict__ dirs, TrigP __restrict__ trig,
                                                    int ndir, CubP __restrict__ cub, int ns,
                                                    float s0, float inv_ds, float ds, int n,
                                                    const long long* k, float* out) {
-  (void)ds;
   double xd[V], yd[V], zd[V];
   float x[V], y[V], z[V];
   bool inside = true;
@@ -178,10 +188,15 @@ __device__ __forceinline__ void adjoint_voxels_f32(DirP __restrict__ dirs, TrigP
     z[v] = (float)zd[v];
     inside = inside && xd[v] * xd[v] + yd[v] * yd[v] + zd[v] * zd[v] <= (1.0 - 1e-4) * (1.0 - 1e-4);
   }
-  if (__builtin_amdgcn_ballot_w64(!inside) == 0ull)
-    adjoint_dir_sum<V, true>(dirs, trig, ndir, cub, ns, s0, inv_ds, xd, yd, zd, x, y, z, out);
+  // the grid covers [-1 + 5e-5, 1 - 5e-5] (wave-uniform)
+  const bool covers = s0 <= -1.0f + 5e-5f && fmaf((float)(ns - 1), ds, s0) >= 1.0f - 5e-5f;
+  const bool all_inside = __builtin_amdgcn_ballot_w64(!inside) == 0ull;
+  if (all_inside && covers)
+    adjoint_dir_sum<V, 2>(dirs, trig, ndir, cub, ns, s0, inv_ds, xd, yd, zd, x, y, z, out);
+  else if (MODE1 && all_inside)
+    adjoint_dir_sum<V, 1>(dirs, trig, ndir, cub, ns, s0, inv_ds, xd, yd, zd, x, y, z, out);
   else
-    adjoint_dir_sum<V, false>(dirs, trig, ndir, cub, ns, s0, inv_ds, xd, yd, zd, x, y, z, out);
+    adjoint_dir_sum<V, 0>(dirs, trig, ndir, cub, ns, s0, inv_ds, xd, yd, zd, x, y, z, out);
 }
 
 #endif

@@ -60,7 +60,7 @@ __global__ __launch_bounds__(FR_WAVES * 64) void k_batch(const wrt_batch_item* _
         bool valid[WRT_ADJ_VPT];
         adjoint_tile_voxels<WRT_ADJ_VPT>((long long)t.a + w, threadIdx.x & 63, it.an, k, valid);
         float r[WRT_ADJ_VPT];
-        adjoint_voxels_f32<WRT_ADJ_VPT>(
+        adjoint_voxels_f32<WRT_ADJ_VPT, false>(
             (const WRT_CONST float4*)it.adirs, (const WRT_CONST double4*)it.atrig, it.andir,
             (const WRT_GLOBAL float4*)it.acub, it.ans, (float)it.as0, (float)(1.0 / it.ads),
             (float)it.ads, it.an, k, r);
