@@ -585,10 +585,11 @@ def traffic_probe(args):
 
 
 VALU_FILE = os.path.join(ROOT, "profiles", "radon_fast_256_valu_pmc.md")
-# the sources the plane-integral kernel is built from: a counter summary holds their hash,
-# and its figures are used only while the sources are unchanged
+# the sources the plane-integral kernel is built from, and the build flags: a counter
+# summary holds their hash, and its figures are used only while they are unchanged
 KERNEL_SOURCES = [os.path.join(ROOT, "wrt-project_amd", "csrc", f)
-                  for f in ("radon_brick.hpp", "k_fast.hip", "wrt_fast.h")]
+                  for f in ("radon_brick.hpp", "k_fast.hip", "wrt_fast.h")] + [
+    os.path.join(ROOT, "wrt-project_amd", "Makefile")]
 
 
 def kernel_sources_sha():
