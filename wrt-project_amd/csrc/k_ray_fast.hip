@@ -77,8 +77,11 @@ __device__ __forceinline__ void cube_ends(double w, double b, bool lo, bool hi, 
 }
 
 #define RAY_THREADS 512
-#define RAY_LDS_N 181  // largest n whose n^2 FP32 plane fits the 128 KiB stage
-#define RAY_T_LDS_N 143  // largest n whose n^2 FP64 plane fits the 160 KiB LDS
+#define RAY_LDS_N 181  // largest n whose n x (n | 1) FP32 plane fits the 128 KiB stage
+
+// the LDS planes' row pitch: odd, so that rows start in different banks
+__host__ __device__ __forceinline__ int ray_pitch(int n) { return n | 1; }
+#define RAY_T_LDS_N 143  // largest n whose n x (n | 1) FP64 plane fits the 160 KiB LDS
 #define RAY_T_THREADS 512
 
 // one axis: the reference's index and fraction of coordinate x (|x| <= 1)
@@ -197,8 +200,22 @@ __global__ __launch_bounds__(RAY_THREADS) void k_ray_f32(const float* __restrict
   const size_t nn = (size_t)n * n;
   const float* __restrict__ v0 = vol32 + (size_t)kz * nn;
   const float* __restrict__ v1 = v0 + nn;
+  // The LDS plane's rows have an odd pitch (n | 1): lanes on neighbouring rays whose points
+  // differ in x then fall in different banks (with pitch n = 128 every row starts in bank 0)
+  const int P = ray_pitch(n);
   if (LDS) {
-    for (int k = threadIdx.x; k < (int)nn; k += RAY_THREADS) plane[k] = lerpf(cz, v0[k], v1[k]);
+    // element k = x n + y of the slice goes to x P + y; (x, y) steps by RAY_THREADS elements
+    const int dq = RAY_THREADS / n, dr = RAY_THREADS % n;
+    int x = threadIdx.x / n, y = threadIdx.x % n;
+    for (int k = threadIdx.x; k < (int)nn; k += RAY_THREADS) {
+      plane[x * P + y] = lerpf(cz, v0[k], v1[k]);
+      x += dq;
+      y += dr;
+      if (y >= n) {
+        y -= n;
+        ++x;
+      }
+    }
     __syncthreads();
   }
   if (t >= nrays) return;
@@ -221,13 +238,14 @@ __global__ __launch_bounds__(RAY_THREADS) void k_ray_f32(const float* __restrict
     float fx, fy;
     ray_point<POW2>(fixed, Qx, Qy, px, py, sx, sy, i, n, dg, inv_dg, h, ix, iy, fx, fy);
     float p00, p10, p01, p11;
-    const int o = ix * n + iy;
     if (LDS) {
+      const int o = ix * P + iy;
       p00 = plane[o];
-      p10 = plane[o + n];
+      p10 = plane[o + P];
       p01 = plane[o + 1];
-      p11 = plane[o + n + 1];
+      p11 = plane[o + P + 1];
     } else {
+      const int o = ix * n + iy;
       p00 = lerpf(cz, v0[o], v1[o]);
       p10 = lerpf(cz, v0[o + n], v1[o + n]);
       p01 = lerpf(cz, v0[o + 1], v1[o + 1]);
@@ -248,7 +266,7 @@ static void launch_ray(const float* vol32, int n, const int* zi, const double* z
                        hipStream_t st) {
   const long long nrays = (long long)nshift * nphi;
   const dim3 grid((unsigned)((nrays + RAY_THREADS - 1) / RAY_THREADS), (unsigned)nz);
-  const size_t lds = LDS ? (size_t)n * n * sizeof(float) : 0;
+  const size_t lds = LDS ? (size_t)n * ray_pitch(n) * sizeof(float) : 0;
   const int m = n - 1;
   if ((m & (m - 1)) == 0) {
     if (lds > 65536)
@@ -307,8 +325,9 @@ __global__ __launch_bounds__(RAY_T_THREADS) void k_ray_T_f32(const float* __rest
   const double czd = zc[iz];
   const double delta = (2.0 / (n - 1)) / (2.0);
   const size_t nn = (size_t)n * n;
+  const int P = ray_pitch(n);  // the LDS plane's odd row pitch, as in k_ray_f32
   if (LDS) {
-    for (int k = threadIdx.x; k < (int)nn; k += RAY_T_THREADS) tplane[k] = 0.0;
+    for (int k = threadIdx.x; k < n * P; k += RAY_T_THREADS) tplane[k] = 0.0;
     __syncthreads();
   }
   const long long nrays = (long long)nshift * nphi;
@@ -336,15 +355,16 @@ __global__ __launch_bounds__(RAY_T_THREADS) void k_ray_T_f32(const float* __rest
       // the forward's value: lerp(fy, lerp(fx, p00, p10), lerp(fx, p01, p11))
       const float gy0 = gv * (1.f - fy), gy1 = gv * fy;
       const float w00 = gy0 * (1.f - fx), w10 = gy0 * fx, w01 = gy1 * (1.f - fx), w11 = gy1 * fx;
-      const int o = ix * n + iy;
       if (LDS) {
         // LDS FP64 atomic adds: no returned value to wait for (the FP32 LDS atomic runs lane
         // by lane on gfx950)
+        const int o = ix * P + iy;
         unsafeAtomicAdd(&tplane[o], (double)w00);
-        unsafeAtomicAdd(&tplane[o + n], (double)w10);
+        unsafeAtomicAdd(&tplane[o + P], (double)w10);
         unsafeAtomicAdd(&tplane[o + 1], (double)w01);
-        unsafeAtomicAdd(&tplane[o + n + 1], (double)w11);
+        unsafeAtomicAdd(&tplane[o + P + 1], (double)w11);
       } else {
+        const int o = ix * n + iy;
         double* p = acc + (size_t)o * n + kz;  // [x][y][z]
         const float wv[4] = {w00, w10, w01, w11};
         const size_t off[4] = {0, nn, (size_t)n, nn + n};
@@ -359,8 +379,17 @@ __global__ __launch_bounds__(RAY_T_THREADS) void k_ray_T_f32(const float* __rest
   }
   if (LDS) {
     __syncthreads();
+    // element k = x n + y of the plane sits at x P + y; (x, y) steps by RAY_T_THREADS
+    const int dq = RAY_T_THREADS / n, dr = RAY_T_THREADS % n;
+    int x = threadIdx.x / n, y = threadIdx.x % n;
     for (int k = threadIdx.x; k < (int)nn; k += RAY_T_THREADS) {
-      const double v = tplane[k];
+      const double v = tplane[x * P + y];
+      x += dq;
+      y += dr;
+      if (y >= n) {
+        y -= n;
+        ++x;
+      }
       if (v == 0.0) continue;
       double* p = acc + (size_t)k * n + kz;  // k = x*n + y
       unsafeAtomicAdd(p, v * w0);
@@ -382,7 +411,7 @@ static void launch_ray_T(const float* g, int n, const int* zi, const double* zc,
   if (per > most) per = most;
   if (per < 1) per = 1;
   const dim3 grid((unsigned)per, (unsigned)nz);
-  const size_t lds = LDS ? (size_t)n * n * sizeof(double) : 0;
+  const size_t lds = LDS ? (size_t)n * ray_pitch(n) * sizeof(double) : 0;
   // the smallest stride >= 4 coprime to nshift (rays 4 shifts apart: at C1's 257 shifts
   // over 128 cells, 2 cells apart; a stride of 8 measured the same)
   int sstride = 4;
