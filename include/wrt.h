@@ -337,7 +337,8 @@ int wrt_job_tuning(const wrt_job* job, int* band_rows, int* dgroup);
 /* Run many jobs (forward and adjoint, uneven sizes) as one submission (no reference
  * counterpart).  The FP32/BF16 plane-integral jobs and the FP32 backprojection jobs run as
  * one persistent launch captured in a HIP graph: their tasks (brick x direction group,
- * 2048-voxel blocks) are ordered longest first and taken by the CUs from a device counter.
+ * blocks of 16 backprojection voxel tiles) are ordered longest first and taken by the CUs
+ * from a device counter.
  * The direction-group size is autotuned on the first call for a list shape (cached in
  * memory and in the file WRT_TUNE_CACHE or $HOME/.cache/wrt_batch_tune.txt; WRT_AUTOTUNE=0
  * disables tuning, WRT_BATCH_DGROUP=G >= 16 fixes the size).  The graph is replayed while the list and the jobs' loads stay the same;
