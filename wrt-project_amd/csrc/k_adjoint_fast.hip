@@ -35,7 +35,13 @@ extern "C" int wrtk_adjoint_cubics(const double* x, int ns, const double* tab, v
   return WRT_OK;
 }
 
-// one tile of 4 x 4 x 8 voxels per wavefront (two voxels per thread), four per workgroup
+// voxels per thread of the per-job launch
+#ifndef ADJ_VPT
+#define ADJ_VPT 2
+#endif
+
+// one tile of 4 x 4 x 4*ADJ_VPT voxels per wavefront (ADJ_VPT voxels per thread), four per
+// workgroup
 __global__ __launch_bounds__(256) void k_adjoint_f32(const float4* __restrict__ dirs,
                                                      const double4* __restrict__ trig, int ndir,
                                                      const float4* __restrict__ cub, int ns,
@@ -43,19 +49,20 @@ __global__ __launch_bounds__(256) void k_adjoint_f32(const float4* __restrict__ 
                                                      long long ntiles, float* __restrict__ out) {
   const long long tile = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (tile >= ntiles) return;  // the whole wavefront
-  long long k[2];
-  bool valid[2];
-  adjoint_tile_voxels<2>(tile, threadIdx.x & 63, n, k, valid);
-  float r[2];
-  adjoint_voxels_f32<2>(dirs, trig, ndir, cub, ns, s0, inv_ds, ds, n, k, r);
-  if (valid[0]) out[k[0]] = r[0];
-  if (valid[1]) out[k[1]] = r[1];
+  long long k[ADJ_VPT];
+  bool valid[ADJ_VPT];
+  adjoint_tile_voxels<ADJ_VPT>(tile, threadIdx.x & 63, n, k, valid);
+  float r[ADJ_VPT];
+  adjoint_voxels_f32<ADJ_VPT>(dirs, trig, ndir, cub, ns, s0, inv_ds, ds, n, k, r);
+#pragma unroll
+  for (int v = 0; v < ADJ_VPT; ++v)
+    if (valid[v]) out[k[v]] = r[v];
 }
 
 extern "C" int wrtk_adjoint_f32(const void* dirs, const void* trig, int ndir, const void* cub,
                                 int ns, double s0, double ds, int n, float* out, hipStream_t st) {
   if (n <= 0) return WRT_OK;
-  const long long ntiles = wrt_adj_tiles(n, 2);
+  const long long ntiles = wrt_adj_tiles(n, ADJ_VPT);
   k_adjoint_f32<<<dim3((unsigned)((ntiles + 3) / 4)), 256, 0, st>>>(
       (const float4*)dirs, (const double4*)trig, ndir, (const float4*)cub, ns, (float)s0,
       (float)(1.0 / ds), (float)ds, n, ntiles, out);
