@@ -108,13 +108,13 @@ int wrt_radon_grid(wrt_ctx* ctx, const double* vol, int n, const double* phi, in
  * harnesses that forbid allocation.  Every device buffer is carved from the library's fixed
  * scratch of 64 MiB per GPU, held from wrt_init to the end of the process; host tables use
  * static storage.  Limits: nphi * ntheta <= 65536, nshift <= 8192, and the call's bricks,
- * one 32-plane FP64 slab of the volume, its tables and FP64 plane sums within the scratch:
+ * one 32-plane FP64 slab of the volume, its tables and 64-bit plane sums within the scratch:
  * about 6.7e6 planes at n = 128, 3.5e6 at n = 192, and no n above 224 (the 32^3 FP32 bricks
  * of a 256^3 volume alone fill it); beyond them it returns WRT_E_SCRATCH before any device
  * work and writes nothing.  Same arguments, output
- * layout and values as wrt_radon_grid(..., WRT_F32, ...) up to the summation order of the
- * FP64 plane accumulators.  Synchronous; calls that overlap on one GPU, or from several host
- * threads, share the scratch and race. */
+ * layout and values as wrt_radon_grid(..., WRT_F32, ...), bit for bit when that call walks
+ * the same 8 x 8 bands (WRT_BAND_ROWS=8).  Synchronous; calls that overlap on one GPU, or
+ * from several host threads, share the scratch and race. */
 int wrt_radon_grid_noalloc(wrt_ctx* ctx, const double* vol, int n, const double* phi,
                            int nphi, const double* theta, int ntheta, const double* shift,
                            int nshift, double* out, wrt_stats* st);
@@ -340,12 +340,13 @@ int wrt_job_tuning(const wrt_job* job, int* band_rows, int* dgroup);
  * blocks of 16 backprojection voxel tiles) are ordered longest first and taken by the CUs
  * from a device counter.
  * The direction-group size is autotuned on the first call for a list shape (cached in
- * memory and in the file WRT_TUNE_CACHE or $HOME/.cache/wrt_batch_tune.txt; WRT_AUTOTUNE=0
- * disables tuning, WRT_BATCH_DGROUP=G >= 16 fixes the size).  The graph is replayed while the list and the jobs' loads stay the same;
- * any other job (plain FP64 forms, reductions) runs on its own stream beside the graph.
- * Each job's output then equals its wrt_job_run output up to the summation order of the
- * FP64 plane accumulators (relative L2 <= 1e-6).  st: kernel_seconds = the graph's device
- * time, seconds = the longest of it and the other jobs' device times. */
+ * memory and in the file WRT_TUNE_CACHE or $HOME/.cache/wrt_tune.txt; WRT_AUTOTUNE=0
+ * disables tuning, WRT_BATCH_DGROUP=G >= 16 fixes the size).  The graph is replayed while the
+ * list and the jobs' loads stay the same; any other job (plain FP64 forms, reductions) runs
+ * on its own stream beside the graph.  Each job's output then equals its wrt_job_run output,
+ * bit for bit when that run walks the persistent kernel's 8 x 8 bands (WRT_BAND_ROWS=8;
+ * plane sums are exact 64-bit fixed-point sums in any order).  st: kernel_seconds = the
+ * graph's device time, seconds = the longest of it and the other jobs' device times. */
 int wrt_batch_run(wrt_ctx* ctx, wrt_job* const* jobs, int njobs, wrt_stats* st);
 /* direction-group size of the context's current batch plan (0 if none) */
 int wrt_batch_dgroup(wrt_ctx* ctx);

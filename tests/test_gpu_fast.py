@@ -87,7 +87,9 @@ def test_fast_job_repeatable(ctx):
     j.run()
     b = j.fetch()
     j.close()
-    assert orc.rel_l2(a, b) <= 1e-12
+    # the same bits: plane sums are exact 64-bit fixed-point sums, whatever the order in
+    # which the concurrent interior and face launches add their sections
+    assert np.array_equal(a, b)
 
 
 def _adjoint_case(ctx, n, nphi, nth, ns, seed):

@@ -9,12 +9,13 @@
 // (radon_brick.hpp) with the job's arguments; a backprojection task is up to
 // WRT_ADJ_TASK_TILES voxel tiles of one job, one per wavefront, WRT_ADJ_VPT voxels per thread
 // (adjoint_voxel.hpp: more voxels per thread than the per-job launch's two keep more gathers
-// in flight at the one workgroup per CU the brick task's LDS allows).  Results are the per-job launches' up to the order of the FP64
-// accumulator atomics.
+// in flight at the one workgroup per CU the brick task's LDS allows).  Results are the
+// per-job launches', bit for bit: every voxel's sum runs in the same order, and the plane
+// sums are exact 64-bit fixed-point sums, whatever the order of their sections.
 #include "adjoint_voxel.hpp"
 #include "radon_brick.hpp"
 
-// zero the FP64 accumulators (segment blockIdx.y < nseg) and the task counter
+// zero the fixed-point plane sums (segment blockIdx.y < nseg) and the task counter
 __global__ __launch_bounds__(256) void k_batch_prologue(const wrt_batch_seg* __restrict__ segs,
                                                         int nseg, int* __restrict__ counter) {
   if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *counter = 0;
@@ -22,14 +23,14 @@ __global__ __launch_bounds__(256) void k_batch_prologue(const wrt_batch_seg* __r
   const wrt_batch_seg sg = segs[blockIdx.y];
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < sg.count;
        i += (long long)gridDim.x * blockDim.x)
-    sg.acc[i] = 0.0;
+    sg.acc[i] = 0ll;
 }
 
 __global__ __launch_bounds__(256) void k_batch_epilogue(const wrt_batch_seg* __restrict__ segs) {
   const wrt_batch_seg sg = segs[blockIdx.y];
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < sg.count;
        i += (long long)gridDim.x * blockDim.x)
-    sg.out[i] = (float)sg.acc[i];
+    sg.out[i] = (float)((double)sg.acc[i] * sg.qstep);
 }
 
 __global__ __launch_bounds__(FR_WAVES * 64) void k_batch(const wrt_batch_item* __restrict__ items,

@@ -129,6 +129,7 @@ __global__ __launch_bounds__(256) void k_brickify(const double* __restrict__ vol
   const size_t total = first + (size_t)nbx * nb * nb * (FB * FB * FB);
   const int x0 = bx0 * FB;
   double r2 = 0.0;
+  float vmax = 0.0f;
   for (size_t i = first + (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
        i += (size_t)gridDim.x * blockDim.x) {
     const size_t brick = i / (FB * FB * FB);
@@ -138,8 +139,14 @@ __global__ __launch_bounds__(256) void k_brickify(const double* __restrict__ vol
     double v = 0.0;
     if (x < n && y < n && z < n) v = vol[((size_t)(x - x0) * n + y) * n + z];
     const float f = (float)v;
-    if (bf16) ((unsigned short*)bricks)[i] = f32_to_bf16_rne(f);
-    else ((float*)bricks)[i] = f;
+    if (bf16) {
+      const unsigned short h = f32_to_bf16_rne(f);
+      ((unsigned short*)bricks)[i] = h;
+      vmax = fmaxf(vmax, fabsf(__uint_as_float((unsigned)h << 16)));
+    } else {
+      ((float*)bricks)[i] = f;
+      vmax = fmaxf(vmax, fabsf(f));
+    }
     // one count per wavefront: the 64 lanes of a wave always share a brick
     const unsigned long long nz = __ballot(v != 0.0);
     if ((threadIdx.x & 63) == 0 && nz) atomicAdd(&flags[brick], (int)__popcll(nz));
@@ -156,6 +163,13 @@ __global__ __launch_bounds__(256) void k_brickify(const double* __restrict__ vol
     b = o > b ? o : b;
   }
   if ((threadIdx.x & 63) == 0 && b) atomicMax(r2max, b);
+  // and the largest |value| (non-negative floats order like their bit patterns too)
+  unsigned vb = __float_as_uint(vmax);
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned o = __shfl_down(vb, off, 64);
+    vb = o > vb ? o : vb;
+  }
+  if ((threadIdx.x & 63) == 0 && vb) atomicMax(r2max + 1, (unsigned long long)vb);
 }
 
 extern "C" int wrtk_brickify(const double* vol, int n, int nb, int bx0, int nbx, int bf16,
@@ -173,6 +187,23 @@ __global__ void k_acc_to_f32(const double* __restrict__ acc, float* __restrict__
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
        i += (size_t)gridDim.x * blockDim.x)
     out[i] = (float)acc[i];
+}
+
+__global__ void k_fix_to_f32(const long long* __restrict__ acc, double qstep,
+                             float* __restrict__ out, size_t count) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+       i += (size_t)gridDim.x * blockDim.x)
+    out[i] = (float)((double)acc[i] * qstep);  // exact: |acc| < 2^53, qstep a power of two
+}
+
+extern "C" int wrtk_fix_to_f32(const long long* acc, double qstep, float* out, size_t count,
+                               hipStream_t st) {
+  if (count == 0) return WRT_OK;
+  size_t blocks = (count + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  k_fix_to_f32<<<(unsigned)blocks, 256, 0, st>>>(acc, qstep, out, count);
+  WRT_LAUNCH_CHECK();
+  return WRT_OK;
 }
 
 extern "C" int wrtk_acc_to_f32(const double* acc, float* out, size_t count, hipStream_t st) {

@@ -87,18 +87,20 @@ extern "C" int wrtk_noalloc_tasks(const int* flags, int n, int nb, int ngroups, 
   return WRT_OK;
 }
 
-// acc[i] = (double)(float)acc[i]: the FP32 form's output, kept in FP64 for the host copy
-__global__ __launch_bounds__(256) void k_na_round(double* __restrict__ acc, size_t count) {
+// the fixed-point plane sums (units of qstep) -> the FP32 form's output, kept as doubles in
+// the same 8-byte slots for the host copy
+__global__ __launch_bounds__(256) void k_na_round(long long* __restrict__ acc, double qstep,
+                                                  size_t count) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
        i += (size_t)gridDim.x * blockDim.x)
-    acc[i] = (double)(float)acc[i];
+    acc[i] = __double_as_longlong((double)(float)((double)acc[i] * qstep));
 }
 
-extern "C" int wrtk_noalloc_round(double* acc, size_t count, hipStream_t st) {
+extern "C" int wrtk_noalloc_round(long long* acc, double qstep, size_t count, hipStream_t st) {
   if (!count) return WRT_OK;
   size_t blocks = (count + 255) / 256;
   if (blocks > 4096) blocks = 4096;
-  k_na_round<<<(unsigned)blocks, 256, 0, st>>>(acc, count);
+  k_na_round<<<(unsigned)blocks, 256, 0, st>>>(acc, qstep, count);
   WRT_LAUNCH_CHECK();
   return WRT_OK;
 }

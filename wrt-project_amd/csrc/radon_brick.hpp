@@ -616,12 +616,19 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
         }
       }
       if constexpr (!T) {
-        // the section's lane sums reduced in FP32 (one DPP add per step), its total added to
-        // the plane's FP64 accumulator
-        const double tot = (double)wave_sum_f32(bs);
-        if (lane == 0 && tot != 0.0)
-          __hip_atomic_fetch_add((WRT_GLOBAL double*)a.acc + (size_t)jo * a.ndir + d,
-                                 tot * delta * delta, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // the section's lane sums reduced in FP32 (one DPP add per step); its total, in
+        // units of qstep, rounded to an integer by one FMA onto 1.5 * 2^52 (|tot * qmul| <
+        // 2^50, so the result is an integer-valued double whose bits less the constant's are
+        // that integer), added to the plane's 64-bit sum: integer adds are exact and
+        // associative, so the plane's sum does not depend on the order of the sections
+        const float tot = wave_sum_f32(bs);
+        if (lane == 0 && tot != 0.0f) {
+          const double y = fma((double)tot, a.qmul, 6755399441055744.0);
+          const long long q = __double_as_longlong(y) - 0x4338000000000000ll;
+          __hip_atomic_fetch_add((WRT_GLOBAL unsigned long long*)a.acc + (size_t)jo * a.ndir + d,
+                                 (unsigned long long)q, __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT);
+        }
       }
     }
   }

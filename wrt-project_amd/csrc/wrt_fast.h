@@ -32,7 +32,10 @@ typedef struct {
   const int* sidx;    /* [nshift] output row of the j-th smallest shift: the
                          caller's grid may be in any order                    */
   int nshift;
-  double* acc;        /* [nshift][ndir] FP64 accumulators, zeroed by caller   */
+  long long* acc;     /* [nshift][ndir] plane sums in 64-bit fixed point, units
+                         of qstep (wrt_plane_qstep), zeroed by the caller      */
+  double qmul;        /* delta^2 / qstep: a section adds rint(its FP32 sample
+                         sum * qmul)                                          */
   int band_rows;      /* rows per band of the walk: 4, 8 or 16 (0: 8); the forward
                          per-job launch only, the others walk 8                */
   double rho2;        /* squared support radius: every sample farther from the
@@ -47,15 +50,32 @@ typedef struct {
 
 /* FP64 vol[x][y][z] -> the bricks of the brick rows bx in [bx0, bx0 + nbx) (vol holds the
  * planes x >= 32 bx0: the whole volume for bx0 = 0, nbx = nb); flags[brick] += its number of
- * non-zero values (flags zeroed by the caller); r2max = largest squared cell-centre radius
- * of a non-zero cell (as the bit pattern of a non-negative double, so unsigned max orders
- * it), zeroed by the caller */
+ * non-zero values (flags zeroed by the caller); r2max[0] = largest squared cell-centre
+ * radius of a non-zero cell (as the bit pattern of a non-negative double, so unsigned max
+ * orders it), r2max[1] = largest |value| as stored in the bricks (the bits of a
+ * non-negative float), both zeroed by the caller */
 int wrtk_brickify(const double* vol, int n, int nb, int bx0, int nbx, int bf16, void* bricks,
                   int* flags, unsigned long long* r2max, hipStream_t st);
 /* boundary = 1: the tasks' bricks touch the cube's faces (cube test and edge clamp) */
 int wrtk_radon_fast(const wrt_radon_fast_args* a, int boundary, hipStream_t st);
 /* out[i] = (float)acc[i] */
 int wrtk_acc_to_f32(const double* acc, float* out, size_t count, hipStream_t st);
+/* out[i] = (float)(acc[i] * qstep): the fixed-point plane sums rounded to FP32 */
+int wrtk_fix_to_f32(const long long* acc, double qstep, float* out, size_t count, hipStream_t st);
+
+/* The fixed-point unit of a volume's plane sums: the power of two qstep >= 16 vmax 2^-50
+ * (vmax the largest |brick value|).  A plane's sum is at most 16 vmax in magnitude (16 n^2
+ * lattice samples of weight delta^2 = 1/n^2), so every section's and every plane's sum is
+ * below 2^50 units: a section's value converts exactly through binary64 (|x| < 2^51) and the
+ * 64-bit sums cannot overflow.  The unit is at most 2^-46 of the largest possible sum. */
+static inline double wrt_plane_qstep(float vmax) {
+  double q = 1.0;
+  const double want = 16.0 * (double)vmax * (1.0 + 1.0 / 1048576.0) / 1125899906842624.0;
+  if (!(want > 0.0)) return 1.0;  /* an all-zero volume */
+  while (q < want) q *= 2.0;
+  while (q * 0.5 >= want) q *= 0.5;
+  return q;
+}
 
 /* Exact transpose of the plane integrals from FP32 plane values (a.g, a.gscale, a.part set;
  * a.rho2 >= 3 so that no row is clipped): every task (brick, half, group) stores its half
@@ -80,8 +100,8 @@ int wrtk_noalloc_zero(void* p, size_t words, hipStream_t st);
  * tasks; counts[0], counts[1] = their numbers */
 int wrtk_noalloc_tasks(const int* flags, int n, int nb, int ngroups, int* tasks, int* counts,
                        hipStream_t st);
-/* acc[i] = (double)(float)acc[i] */
-int wrtk_noalloc_round(double* acc, size_t count, hipStream_t st);
+/* acc[i] (fixed point, units of qstep) -> the double (float)(acc[i] * qstep), in place */
+int wrtk_noalloc_round(long long* acc, double qstep, size_t count, hipStream_t st);
 
 /* ---- the persistent batch launch: many jobs' tasks in one grid ---- */
 enum { WRT_ITEM_RADON = 0, WRT_ITEM_ADJOINT = 1 };
@@ -115,11 +135,12 @@ typedef struct {
   int item, a, b, c;
 } wrt_batch_task;
 
-/* an FP64 accumulator of a radon item and its FP32 output */
+/* the fixed-point plane sums of a radon item and its FP32 output */
 typedef struct {
-  double* acc;
+  long long* acc;
   float* out;
   long long count;
+  double qstep;
 } wrt_batch_seg;
 
 /* zero every segment's accumulator and the task counter */

@@ -242,6 +242,7 @@ static int plan_build(wrt_ctx* c, wrt_job* const* jobs, int njobs, batch_plan** 
                       : (double)fd[nitems].nvox * fd[nitems].item.andir;
     if (fd[nitems].item.kind == WRT_ITEM_RADON) {
       segs[nseg].acc = fd[nitems].acc;
+      segs[nseg].qstep = fd[nitems].qstep;
       segs[nseg].out = fd[nitems].out;
       segs[nseg].count = fd[nitems].nout;
       ++nseg;

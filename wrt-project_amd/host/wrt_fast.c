@@ -46,7 +46,8 @@ typedef struct {
   int n_interior, n_boundary, dgroup;
   int band_rows;    /* rows per band of the walk (4, 8 or 16) */
   int tuned;        /* band_rows and dgroup settled for this load */
-  double* acc;      /* [nshift][ndir] */
+  long long* acc;   /* [nshift][ndir] plane sums, fixed point in units of qstep */
+  double qstep;     /* wrt_plane_qstep of the loaded volume */
   float* out;       /* [nshift][ndir] */
   double* staging;  /* n^3 FP64 input */
   void* sorted;     /* ascending shifts, then their rows (sorted_shifts) */
@@ -405,9 +406,10 @@ static int radon_fast_create(wrt_job* j) {
   const size_t nout = (size_t)d->nshift * ndir;
   const size_t ngroups = (ndir + 7) / 8;
   if (hipMalloc(&f->bricks, f->nbricks * WRT_BRICK * WRT_BRICK * WRT_BRICK * eb) != hipSuccess ||
-      hipMalloc((void**)&f->flags, f->nbricks * sizeof(int) + 16) != hipSuccess ||
+      /* the counts, padding to 8 bytes, the support radius and the largest |value| */
+      hipMalloc((void**)&f->flags, f->nbricks * sizeof(int) + 24) != hipSuccess ||
       hipMalloc((void**)&f->tasks, 2 * f->nbricks * ngroups * sizeof(int)) != hipSuccess ||
-      hipMalloc((void**)&f->acc, nout * sizeof(double)) != hipSuccess ||
+      hipMalloc((void**)&f->acc, nout * sizeof(long long)) != hipSuccess ||
       hipMalloc((void**)&f->out, nout * sizeof(float)) != hipSuccess ||
       hipMalloc((void**)&f->staging, (size_t)d->n * d->n * d->n * sizeof(double)) != hipSuccess ||
       hipMalloc(&f->sorted, sorted_shift_bytes(d->nshift)) != hipSuccess) {
@@ -462,16 +464,17 @@ static int radon_fast_load(wrt_job* j, const double* in) {
   hipStream_t st = j->st;
   const size_t n3 = (size_t)d->n * d->n * d->n;
   HIPCHK(hipMemcpyAsync(f->staging, in, n3 * sizeof(double), hipMemcpyHostToDevice, st));
-  /* flags, then (8-byte aligned) the squared support radius as ordered bits */
+  /* flags, then (8-byte aligned) the squared support radius and the largest |value|, as
+   * ordered bits */
   const size_t r2off = (f->nbricks * sizeof(int) + 7) & ~(size_t)7;
   unsigned long long* d_r2 = (unsigned long long*)((char*)f->flags + r2off);
-  HIPCHK(hipMemsetAsync(f->flags, 0, r2off + 8, st));
+  HIPCHK(hipMemsetAsync(f->flags, 0, r2off + 16, st));
   int rc = wrtk_brickify(f->staging, d->n, f->nb, 0, f->nb, d->prec == WRT_BF16, f->bricks,
                          f->flags, d_r2, st);
   if (rc != WRT_OK) return rc;
-  int* flags = (int*)malloc(r2off + 8);
+  int* flags = (int*)malloc(r2off + 16);
   if (!flags) return WRT_E_NOMEM;
-  if (hipMemcpyAsync(flags, f->flags, r2off + 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+  if (hipMemcpyAsync(flags, f->flags, r2off + 16, hipMemcpyDeviceToHost, st) != hipSuccess ||
       hipStreamSynchronize(st) != hipSuccess) {
     free(flags);
     return WRT_E_HIP;
@@ -483,6 +486,12 @@ static int radon_fast_load(wrt_job* j, const double* in) {
     memcpy(&r2, (char*)flags + r2off, sizeof(double));
     const double rho = sqrt(r2) + sqrt(3.0) / d->n;
     f->rho2 = rho * rho * (1.0 + 1e-12) + 1e-12;
+    unsigned long long vb;
+    memcpy(&vb, (char*)flags + r2off + 8, sizeof(vb));
+    const unsigned vb32 = (unsigned)vb;
+    float vmax;
+    memcpy(&vmax, &vb32, sizeof(vmax));
+    f->qstep = wrt_plane_qstep(vmax);
   }
   const int ndir = d->nphi * d->ntheta;
   int nonempty = 0;
@@ -545,7 +554,7 @@ static int radon_fast_pass(wrt_job* j, hipStream_t st, int timed) {
   const wrt_job_desc* d = &j->d;
   const size_t ndir = (size_t)d->nphi * d->ntheta;
   const size_t nout = (size_t)d->nshift * ndir;
-  int rc = wrtk_fill_f64(f->acc, 0.0, nout, st);
+  int rc = wrtk_fill_f64((double*)f->acc, 0.0, nout, st); /* zero bits: integer 0 */
   if (rc != WRT_OK) return rc;
   if (timed) HIPCHK(hipEventRecord(j->e1, st));
   wrt_radon_fast_args a;
@@ -561,10 +570,11 @@ static int radon_fast_pass(wrt_job* j, hipStream_t st, int timed) {
   a.sidx = (const int*)((const char*)f->sorted + (size_t)d->nshift * sizeof(double));
   a.nshift = d->nshift;
   a.acc = f->acc;
+  a.qmul = (1.0 / d->n) * (1.0 / d->n) / f->qstep;
   a.rho2 = f->rho2;
   a.band_rows = f->band_rows;
   /* face bricks on the side stream, interior bricks on the job's stream, concurrently:
-   * both add into the same FP64 accumulators with atomics */
+   * both add into the same 64-bit plane sums with integer atomics */
   hipStream_t sb = f->side ? f->side : st;
   if (f->side) {
     HIPCHK(hipEventRecord(f->fork, st));
@@ -582,7 +592,7 @@ static int radon_fast_pass(wrt_job* j, hipStream_t st, int timed) {
   if (rc != WRT_OK) return rc;
   if (f->side) HIPCHK(hipStreamWaitEvent(st, f->join, 0));
   if (timed) HIPCHK(hipEventRecord(j->e3, st));
-  return wrtk_acc_to_f32(f->acc, f->out, nout, st);
+  return wrtk_fix_to_f32(f->acc, f->qstep, f->out, nout, st);
 }
 
 /* seconds of one pass with the current band shape and direction group (events of its own:
@@ -1071,17 +1081,22 @@ int wrt_fast_tuning(const wrt_job* j, int* band_rows, int* dgroup) {
   return WRT_OK;
 }
 
-/* the FP32 plane values of a fast radon job from its FP64 accumulators, i.e. the rounding
- * the per-job launch does on the device, done here for the rounds of the serve loop (which
- * leave the FP32 buffer alone) */
+/* the FP32 plane values of a fast radon job from its fixed-point plane sums, i.e. the
+ * rounding the per-job launch does on the device, done here for the rounds of the serve loop
+ * (which leave the FP32 buffer alone) */
 int wrt_fast_fetch_acc(wrt_job* j, double* out) {
   if (!j || !j->fast || !out || j->d.op != WRT_OP_RADON) return WRT_E_ARG;
   const wrt_job_desc* d = &j->d;
+  const radon_fast* f = (const radon_fast*)j->fast;
   const size_t count = (size_t)d->nshift * d->nphi * d->ntheta;
-  HIPCHK(hipMemcpyAsync(out, ((radon_fast*)j->fast)->acc, count * sizeof(double),
-                        hipMemcpyDeviceToHost, j->st));
+  /* the 8-byte sums land in out's slots and are converted in place */
+  HIPCHK(hipMemcpyAsync(out, f->acc, count * sizeof(long long), hipMemcpyDeviceToHost, j->st));
   HIPCHK(hipStreamSynchronize(j->st));
-  for (size_t i = 0; i < count; ++i) out[i] = (double)(float)out[i];
+  for (size_t i = 0; i < count; ++i) {
+    long long v;
+    memcpy(&v, &out[i], sizeof(v));
+    out[i] = (double)(float)((double)v * f->qstep);
+  }
   return WRT_OK;
 }
 
@@ -1110,6 +1125,7 @@ int wrt_fast_describe(wrt_job* j, wrt_fast_desc* fd) {
     it->ra.sidx = (const int*)((const char*)f->sorted + (size_t)d->nshift * sizeof(double));
     it->ra.nshift = d->nshift;
     it->ra.acc = f->acc;
+    it->ra.qmul = (1.0 / d->n) * (1.0 / d->n) / f->qstep;
     it->ra.rho2 = f->rho2;
     fd->brick = f->ne_brick;
     fd->weight = f->ne_count;
@@ -1117,6 +1133,7 @@ int wrt_fast_describe(wrt_job* j, wrt_fast_desc* fd) {
     fd->nbrick = f->n_ne;
     fd->dgroup = f->dgroup;
     fd->acc = f->acc;
+    fd->qstep = f->qstep;
     fd->out = f->out;
     fd->nout = (long long)d->nshift * d->nphi * d->ntheta;
   } else {

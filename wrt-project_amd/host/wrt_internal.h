@@ -67,8 +67,9 @@ typedef struct {
   const int* weight;          /* radon: their non-zero voxel counts */
   const unsigned char* bnd;   /* radon: 1 if the brick touches a cube face */
   int nbrick;
-  double* acc;                /* radon: FP64 accumulators and FP32 output */
-  float* out;
+  long long* acc;             /* radon: fixed-point plane sums (units of qstep) ... */
+  double qstep;
+  float* out;                 /* ... and the FP32 output */
   long long nout;
   long long nvox;             /* adjoint: n^3 */
   int dgroup;                 /* radon: direction group of the per-job launch */

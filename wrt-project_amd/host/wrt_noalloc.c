@@ -52,7 +52,7 @@ static void na_plan(int n, int ndir, int nshift, int ngroups, na_layout* L) {
   L->flags = o;
   o = up256(o + nbr * sizeof(int));
   L->r2 = o;
-  o += 8;
+  o += 16; /* the squared support radius, then the largest |value| */
   L->counts = o;
   o = up256(o + 2 * sizeof(int));
   L->tasks = o;
@@ -144,12 +144,16 @@ int wrt_radon_grid_noalloc(wrt_ctx* c, const double* vol, int n, const double* p
     RCHK(wrtk_brickify((const double*)(base + L.slab), n, nb, bx, 1, 0, base + L.bricks,
                        (int*)(base + L.flags), (unsigned long long*)(base + L.r2), s));
   }
-  /* the support radius, as the per-job form derives it */
-  unsigned long long r2bits = 0;
-  HIPCHK(hipMemcpyAsync(&r2bits, base + L.r2, sizeof(r2bits), hipMemcpyDeviceToHost, s));
+  /* the support radius and the fixed-point unit, as the per-job form derives them */
+  unsigned long long r2bits[2] = {0, 0};
+  HIPCHK(hipMemcpyAsync(r2bits, base + L.r2, sizeof(r2bits), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   double r2;
-  memcpy(&r2, &r2bits, sizeof(r2));
+  memcpy(&r2, &r2bits[0], sizeof(r2));
+  const unsigned vb32 = (unsigned)r2bits[1];
+  float vmax;
+  memcpy(&vmax, &vb32, sizeof(vmax));
+  const double qstep = wrt_plane_qstep(vmax);
   const double rho = sqrt(r2) + sqrt(3.0) / n;
   const double rho2 = rho * rho * (1.0 + 1e-12) + 1e-12;
   /* task list */
@@ -173,7 +177,8 @@ int wrt_radon_grid_noalloc(wrt_ctx* c, const double* vol, int n, const double* p
   a.shift = (const double*)(base + L.shifts);
   a.sidx = (const int*)(base + L.sidx);
   a.nshift = nshift;
-  a.acc = (double*)(base + L.acc);
+  a.acc = (long long*)(base + L.acc);
+  a.qmul = (1.0 / n) * (1.0 / n) / qstep;
   a.rho2 = rho2;
   HIPCHK(hipEventRecord(c->ev0, s));
   a.tasks = (const int*)(base + L.tasks);
@@ -183,7 +188,7 @@ int wrt_radon_grid_noalloc(wrt_ctx* c, const double* vol, int n, const double* p
   a.ntasks = counts[1];
   RCHK(wrtk_radon_fast(&a, 1, s));
   HIPCHK(hipEventRecord(c->ev1, s));
-  RCHK(wrtk_noalloc_round((double*)(base + L.acc), nplanes, s));
+  RCHK(wrtk_noalloc_round((long long*)(base + L.acc), qstep, nplanes, s));
   HIPCHK(hipEventRecord(c->ev3, s));
   HIPCHK(hipMemcpyAsync(out, base + L.acc, nplanes * sizeof(double), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));

@@ -312,6 +312,7 @@ static int round_launch(wrt_ctx* ctx, round_slot* slots, int me, int express, ac
       it.ra.dgroup = a->G;
       if (!a->zeroed) {
         r->h_segs[nseg].acc = a->fd.acc;
+        r->h_segs[nseg].qstep = a->fd.qstep;
         r->h_segs[nseg].out = a->fd.out;
         r->h_segs[nseg].count = a->fd.nout;
         ++nseg;
@@ -570,7 +571,7 @@ int wrt_serve_run(wrt_ctx* ctx, const wrt_serve_cfg* c, wrt_serve_report* rep,
         for (long long k = 0; k < a->ntasks; ++k) a->left += a->tasks[k].cost;
         if (a->ntasks == 0 && a->fd.item.kind == WRT_ITEM_RADON) {
           /* an all-zero volume: no task, so no prologue zeroes its planes */
-          rc = wrtk_fill_f64(a->fd.acc, 0.0, (size_t)a->fd.nout, ctx->stream);
+          rc = wrtk_fill_f64((double*)a->fd.acc, 0.0, (size_t)a->fd.nout, ctx->stream);
           if (rc == WRT_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = WRT_E_HIP;
           if (rc != WRT_OK) goto done;
         }
