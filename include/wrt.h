@@ -97,7 +97,9 @@ int wrt_zslice_grid(int nslices, double* z);
 /* ------------------------------------------------- host-buffer entry points --- */
 /* Plane integrals over every (shift, phi, theta): replaces plane_integral
  * (radon_grid/src/integral.c:26-59) called from radon_grid/src/radon_grid.c:248-258.
- * out[s][phi][theta]. */
+ * out[s][phi][theta].  The FP32 and BF16 forms need every volume value finite in FP32
+ * (WRT_E_ARG otherwise); the plain FP64 form propagates non-finite values as the reference
+ * does. */
 int wrt_radon_grid(wrt_ctx* ctx, const double* vol, int n, const double* phi, int nphi,
                    const double* theta, int ntheta, const double* shift, int nshift,
                    wrt_prec prec, double* out, wrt_stats* st);
@@ -307,7 +309,8 @@ size_t wrt_job_bytes(const wrt_job_desc* d);
 size_t wrt_job_input_count(const wrt_job_desc* d);
 size_t wrt_job_output_count(const wrt_job_desc* d);
 int wrt_job_create(wrt_ctx* ctx, const wrt_job_desc* d, wrt_job** job);
-/* host -> device input (converted to the job's precision/layout) */
+/* host -> device input (converted to the job's precision/layout); a RADON job in FP32 or
+ * BF16 refuses a volume with a value that is not finite in FP32 (WRT_E_ARG) */
 int wrt_job_load(wrt_job* job, const double* in);
 /* synthetic input from the seeded generators below: an ellipsoid phantom for RADON (cell
  * centred) and RAY (node centred), Gaussian data for the other ops */

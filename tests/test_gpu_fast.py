@@ -1,6 +1,7 @@
 """GPU tests of the fast plane-integral form (FP32 values / BF16 bricks, FP32 lane and wave
-sums added to FP64 plane accumulators, the reference's index decisions: fixed point with an
-exact binary64 path near cell boundaries) against the plain FP64 form and the oracle; integer
+sums added into exact 64-bit fixed-point plane sums, the reference's index decisions: fixed
+point with an exact binary64 path near cell boundaries) against the plain FP64 form and the
+oracle; integer
 volumes pin every index decision (test_fast_f32_index_decisions_exact), for every band shape
 the autotuner can pick.
 
@@ -280,3 +281,24 @@ def _exact_case(ctx, n, support):
     d2 = (1.0 / n) ** 2
     assert np.array_equal(np.rint(got / d2), np.rint(ref / d2))
     assert np.max(np.abs(ref / d2 - np.rint(ref / d2))) < 1e-6
+
+
+@pytest.mark.parametrize("bad", [np.nan, np.inf, -np.inf, 1e39])
+def test_fast_forms_refuse_values_not_finite_in_fp32(ctx, bad):
+    # the fixed-point plane sums need a finite largest |value|: the FP32 and BF16 forms and
+    # the entry point that allocates nothing return WRT_E_ARG (1e39 is finite in binary64
+    # but not in FP32); the plain FP64 form keeps the reference's arithmetic
+    vol, phi, th, s = _phantom_case(ctx, 40, 8, 4, 17, seed=6)
+    vol[20, 21, 22] = bad
+    for prec in ("f32", "bf16"):
+        with pytest.raises(wrt.WrtError) as e:
+            ctx.radon_grid(vol, phi, th, s, prec=prec)
+        assert e.value.code == wrt.WRT_E_ARG, prec
+    with pytest.raises(wrt.WrtError) as e:
+        ctx.radon_grid_noalloc(vol, phi, th, s)
+    assert e.value.code == wrt.WRT_E_ARG
+    # the context still works afterwards
+    vol[20, 21, 22] = 0.5
+    got, _ = ctx.radon_grid(vol, phi, th, s, prec="f32")
+    ref, _ = ctx.radon_grid(vol, phi, th, s, prec="f64")
+    assert orc.rel_l2(got, ref) <= 1e-5

@@ -139,14 +139,17 @@ __global__ __launch_bounds__(256) void k_brickify(const double* __restrict__ vol
     double v = 0.0;
     if (x < n && y < n && z < n) v = vol[((size_t)(x - x0) * n + y) * n + z];
     const float f = (float)v;
+    float fa;
     if (bf16) {
       const unsigned short h = f32_to_bf16_rne(f);
       ((unsigned short*)bricks)[i] = h;
-      vmax = fmaxf(vmax, fabsf(__uint_as_float((unsigned)h << 16)));
+      fa = fabsf(__uint_as_float((unsigned)h << 16));
     } else {
       ((float*)bricks)[i] = f;
-      vmax = fmaxf(vmax, fabsf(f));
+      fa = fabsf(f);
     }
+    // a NaN counts as infinite, so that the caller sees every non-finite value
+    vmax = fmaxf(vmax, fa <= 3.4028234663852886e38f ? fa : __builtin_inff());
     // one count per wavefront: the 64 lanes of a wave always share a brick
     const unsigned long long nz = __ballot(v != 0.0);
     if ((threadIdx.x & 63) == 0 && nz) atomicAdd(&flags[brick], (int)__popcll(nz));

@@ -53,7 +53,7 @@ typedef struct {
  * non-zero values (flags zeroed by the caller); r2max[0] = largest squared cell-centre
  * radius of a non-zero cell (as the bit pattern of a non-negative double, so unsigned max
  * orders it), r2max[1] = largest |value| as stored in the bricks (the bits of a
- * non-negative float), both zeroed by the caller */
+ * non-negative float; infinity if a value is not finite in FP32), both zeroed by the caller */
 int wrtk_brickify(const double* vol, int n, int nb, int bx0, int nbx, int bf16, void* bricks,
                   int* flags, unsigned long long* r2max, hipStream_t st);
 /* boundary = 1: the tasks' bricks touch the cube's faces (cube test and edge clamp) */

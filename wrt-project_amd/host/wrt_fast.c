@@ -25,6 +25,7 @@
  * z-nodes, rounded to FP32.
  */
 #include <stdlib.h>
+#include <float.h>
 #include <math.h>
 #include <string.h>
 
@@ -491,6 +492,10 @@ static int radon_fast_load(wrt_job* j, const double* in) {
     const unsigned vb32 = (unsigned)vb;
     float vmax;
     memcpy(&vmax, &vb32, sizeof(vmax));
+    if (!(vmax <= FLT_MAX)) { /* a value that is not finite in FP32 */
+      free(flags);
+      return WRT_E_ARG;
+    }
     f->qstep = wrt_plane_qstep(vmax);
   }
   const int ndir = d->nphi * d->ntheta;

@@ -18,6 +18,7 @@
  * call's contents; calls that overlap on one GPU (two host threads, two contexts) share it
  * and race, as do overlapping calls from several threads on the static host tables.
  */
+#include <float.h>
 #include <math.h>
 #include <string.h>
 
@@ -153,6 +154,7 @@ int wrt_radon_grid_noalloc(wrt_ctx* c, const double* vol, int n, const double* p
   const unsigned vb32 = (unsigned)r2bits[1];
   float vmax;
   memcpy(&vmax, &vb32, sizeof(vmax));
+  if (!(vmax <= FLT_MAX)) return WRT_E_ARG; /* a value that is not finite in FP32 */
   const double qstep = wrt_plane_qstep(vmax);
   const double rho = sqrt(r2) + sqrt(3.0) / n;
   const double rho2 = rho * rho * (1.0 + 1e-12) + 1e-12;
