@@ -412,8 +412,8 @@ static void launch_ray_T(const float* g, int n, const int* zi, const double* zc,
   if (per < 1) per = 1;
   const dim3 grid((unsigned)per, (unsigned)nz);
   const size_t lds = LDS ? (size_t)n * ray_pitch(n) * sizeof(double) : 0;
-  // the smallest stride >= 4 coprime to nshift (rays 4 shifts apart: at C1's 257 shifts
-  // over 128 cells, 2 cells apart; a stride of 8 measured the same)
+  // the smallest stride >= 4 coprime to nshift (rays 4 shifts apart: with 257 shifts over
+  // 128 cells, 2 cells apart; a stride of 8 measured the same)
   int sstride = 4;
   for (;; ++sstride) {
     int a = sstride, b = nshift;

@@ -503,7 +503,8 @@ static int radon_fast_load(wrt_job* j, const double* in) {
   for (size_t b = 0; b < f->nbricks; ++b) nonempty += flags[b] != 0;
   /* enough tasks to fill 256 CUs several times, at least 16 directions (one per wave).
    * More directions per task even out the ends of the 16 waves' shares and stage fewer
-   * bricks (C2, 256^3 with 32768 directions: 1.085 s per pass with groups of 512, 1.024 s
+   * bricks (256^3 with 256 phi x 128 theta = 32768 directions and 257 shifts: 1.085 s per
+   * pass with groups of 512, 1.024 s
    * with 1024, 1.005 s with 2048, 1.015 s with 4096), but longer tasks interleave the
    * served jobs that run together more coarsely (serve p99 650 ms against 550 ms with the
    * 256^3 jobs' 8192 directions in groups of 1600 rather than 512).  So at least 16 groups
