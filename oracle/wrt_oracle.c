@@ -17,8 +17,12 @@
  * sources built under oracle/_ref (plane_integral, zero_interp, ray_transform,
  * cube_trilinear_interp: bit-exact), numpy leggauss and the README theta values (GL),
  * scipy CubicSpline(bc_type='natural') (spline), exact quadratics (polynomial),
- * closed-form ellipsoid plane/line integrals and constant-data adjoint (4*pi*c),
- * and the dot-product test for the transposes.  The reduction on independent grids (no
+ * closed-form ellipsoid plane/line integrals and constant-data adjoint (4*pi*c), the
+ * backprojection's orientation, sign and [z][x][y] order (data s * n_k gives (4 pi / 3) x_k
+ * on each axis k), the reduction's z stencil (constant rays: the top cell's closed form with
+ * its zeroed third slice; cubic-in-z rays: the sign of the forward stencil's remainder), each
+ * with planted mistakes that the check catches, and the dot-product test for the
+ * transposes.  The reduction on independent grids (no
  * reference counterpart) is pinned to the reference form on identical grids (bit for bit),
  * to reflected full-turn rays, to cubics in phi and to an off-centre Gaussian's closed-form
  * plane integrals, with planted stencil mistakes that fail the last check.
