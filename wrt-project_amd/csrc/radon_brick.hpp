@@ -14,8 +14,8 @@
 // power-of-two scaling or a corrected reciprocal), so only the value rounding (FP32/BF16)
 // and the summation order differ from the plain form.  Samples whose cell lies in the
 // brick add its FP32 value to the lane's FP32 sum; a section's lane sums are reduced over the
-// wavefront in FP32 and added to the plane's FP64 accumulator (one atomic per section and
-// brick).
+// wavefront in FP32, rounded to the plane's 64-bit fixed-point unit and added to the plane's
+// integer accumulator (one atomic per section and brick).
 //
 // Transpose mode works on half bricks (32 x 32 x 16 cells) of 64-bit integer accumulators:
 // every sample whose cell lies in the half brick adds the section's value, quantized to a
@@ -139,7 +139,7 @@ struct radon_T_lds {
 
 // One task: brick `brick` against directions [grp*dgroup, (grp+1)*dgroup).
 // Forward (T = false, L a radon_lds): the brick of a.bricks is staged and every section's
-// sum is added to its plane's FP64 accumulator.  Transpose (T = true, L a radon_T_lds): the
+// sum is added to its plane's fixed-point accumulator.  Transpose (T = true, L a radon_T_lds): the
 // half brick zhalf (cells with z in [16*zhalf, 16*zhalf + 16) of the brick) starts at zero
 // and every sample of a section whose cell lies in it adds the section's quantized value
 // rint(a.g[plane] * a.gscale) to that cell, so that after the task the accumulators hold
@@ -164,11 +164,13 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
   int (*const rowlo)[136] = L.rowlo;
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  // the job's tables in the global address space (the batch kernel reads the argument
-  // block from memory, where the compiler does not know it)
-  const WRT_GLOBAL double* const g_dirs = (const WRT_GLOBAL double*)a.dirs;
-  const WRT_GLOBAL double* const g_shift = (const WRT_GLOBAL double*)a.shift;
-  const WRT_GLOBAL int* const g_sidx = (const WRT_GLOBAL int*)a.sidx;
+  // The job's tables, which no kernel writes while it runs, in the constant address space:
+  // their wave-uniform reads become scalar loads through the scalar cache (vector loads of a
+  // uniform address waited on at every section otherwise; the batch kernel reads the argument
+  // block from memory, where the compiler does not know the address space at all).
+  const WRT_CONST double* const g_dirs = (const WRT_CONST double*)a.dirs;
+  const WRT_CONST double* const g_shift = (const WRT_CONST double*)a.shift;
+  const WRT_CONST int* const g_sidx = (const WRT_CONST int*)a.sidx;
   const int nb = a.nb, n = a.n;
   const int bxi = brick / (nb * nb), byi = (brick / nb) % nb, bzi = brick % nb;
   const int b0x = bxi * FB, b0y = byi * FB, b0z = bzi * FB + (T ? FT_Z * zhalf : 0);
@@ -262,7 +264,7 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
     if (lane == 0) d = atomicAdd(&L.next_dir, 1);
     d = __builtin_amdgcn_readfirstlane(d);
     if (d >= d_end) break;
-    const WRT_GLOBAL double* D = g_dirs + 9 * (size_t)d;
+    const WRT_CONST double* D = g_dirs + 9 * (size_t)d;
     const double nrm[3] = {uniform_f64(D[0]), uniform_f64(D[1]), uniform_f64(D[2])};
     const double av[3] = {uniform_f64(D[3]), uniform_f64(D[4]), uniform_f64(D[5])};
     const double bv[3] = {uniform_f64(D[6]), uniform_f64(D[7]), uniform_f64(D[8])};
@@ -354,7 +356,7 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
       const int jo = g_sidx[j];  // the plane's row in the caller's shift order
       unsigned long long gq = 0ull;  // transpose: the section's quantized value
       if constexpr (T) {
-        const float gv = ((const WRT_GLOBAL float*)a.g)[(size_t)jo * a.ndir + d];
+        const float gv = ((const WRT_CONST float*)a.g)[(size_t)jo * a.ndir + d];
         if (gv == 0.f) continue;
         gq = (unsigned long long)(long long)rint((double)gv * a.gscale);
         if (gq == 0ull) continue;
@@ -471,8 +473,9 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
         }
       }
       // One sample: Q the lane's fixed-point coordinates, R0 + t its row and cb - nrl + step*m
-      // its column (for the exact decision).  Forward: adds the cell's value to bs; transpose:
-      // adds the section's quantized value to the cell.
+      // its column (for the exact decision); ok and off give its cell (below).  The forward
+      // walk adds the cell's value to its lane sum; the transpose adds the section's quantized
+      // value to the cell.
       // The cell of brick offsets (lx, ly, lz): all three in [0, 32) (z: [0, 16) in transpose
       // mode, tested as 2*lz < 32; a negative offset sets high bits of the OR).  ok and the
       // cell's byte offset in the LDS brick (forward: slot 0 holds the zero that rejected
@@ -486,19 +489,17 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
           off = ok ? off : 0u;
         }
       };
-      auto sample = [&](const long long (&Q)[3], int t, int cb, int nrl, int step, int m,
-                        float& bs) {
+      auto locate = [&](const long long (&Q)[3], int t, int cb, int nrl, int step, int m,
+                        bool& ok, unsigned& off) {
         const unsigned u = min(min((unsigned)Q[0], (unsigned)Q[1]), (unsigned)Q[2]);
         // the fixed point's cell (away from a boundary the fixed point floors where the
         // reference truncates: they differ only for -1 < q < 0, x < -1, where the floor's
         // cell -1 lies outside every brick and the reference's cube test gives 0) ...
-        bool ok;
-        unsigned off;
         cell((int)(Q[0] >> 32), (int)(Q[1] >> 32), (int)(Q[2] >> 32), ok, off);
         const bool near = u < 2048u;
         // a branch of the whole wave on the ballot (a scalar branch on VCC, no exec mask to
         // save and restore in every step); each lane takes the exact result only if it is near
-        if (__builtin_amdgcn_ballot_w64(near) != 0ull) {
+        if (__builtin_expect(__builtin_amdgcn_ballot_w64(near) != 0ull, 0)) {
           // ... replaced near a cell boundary (fraction of Q below 2^-21) by the reference's
           // rounding, in binary64 (the per-column products b*i2 rounded as the reference
           // rounds delta*v2*i2); the step index through readfirstlane keeps the column out
@@ -542,13 +543,16 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
           ok = near ? okx : ok;
           off = near ? offx : off;
         }
+      };
+      auto sample = [&](const long long (&Q)[3], int t, int cb, int nrl, int step, int m) {
+        bool ok;
+        unsigned off;
+        locate(Q, t, cb, nrl, step, m, ok, off);
         if constexpr (T) {
           if (ok)
             __hip_atomic_fetch_add(
                 (unsigned long long*)__builtin_assume_aligned((char*)L.acc + off, 8), gq,
                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        } else {
-          bs += *(const float*)((const char*)L.sb + off);
         }
       };
       // Rows past R1 and columns past the lattice end need no test: a plane's cube section
@@ -593,7 +597,7 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
             for (int c = 0; c < 3; ++c)
               Q[c] = Qb[c] + (long long)tq * Afx[c] + (long long)(c0 - C0) * Dfx[c];
             for (int m = 0; m < steps2; ++m) {
-              sample(Q, t, cb, nrl, 16, m, bs);
+              sample(Q, t, cb, nrl, 16, m);
 #pragma unroll
               for (int c = 0; c < 3; ++c) Q[c] += 2 * DS[c];
             }
@@ -608,11 +612,23 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
 #pragma unroll
           for (int c = 0; c < 3; ++c)
             Q[c] = Tb[c] + (long long)tq * Afx[c] + (long long)nrl * nDfx[c];
-          for (int m = 0; m < steps; ++m) {
-            sample(Q, t, k_lane, nrl, BC, m, bs);
+          // Software-pipelined: step m's LDS read is issued before step m + 1's cell decision
+          // and added after it, so that the read's latency overlaps that work.  A rejected
+          // sample reads the zero slot (off = 0).
+          bool ok;
+          unsigned off;
+          locate(Q, t, k_lane, nrl, BC, 0, ok, off);
+          // opaque, so that the loop steps Q instead of forming step 1 from the band's terms
+#pragma unroll
+          for (int c = 0; c < 3; ++c) asm volatile("" : "+v"(Q[c]));
+          for (int m = 1; m < steps; ++m) {
+            const float v = *(const float*)((const char*)L.sb + off);
 #pragma unroll
             for (int c = 0; c < 3; ++c) Q[c] += DS[c];
+            locate(Q, t, k_lane, nrl, BC, m, ok, off);
+            bs += v;
           }
+          bs += *(const float*)((const char*)L.sb + off);
         }
       }
       if constexpr (!T) {
