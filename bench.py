@@ -604,31 +604,38 @@ def kernel_sources_sha():
 
 def _profile_figure(path, key, tile=None):
     """the figure `key:` of a committed counter summary, if it was taken on the current
-    kernel sources and, given tile = (band_rows, dgroup), with that tile (None otherwise)"""
-    vals = {}
-    try:
-        for line in open(path):
-            if ":" in line and not line.startswith((" ", "#", "|")):
-                k, v = line.split(":", 1)
-                vals[k.strip()] = v.strip()
-    except OSError:
-        return None
-    if vals.get("kernel_sources_sha256") != kernel_sources_sha() or key not in vals:
-        return None
-    if tile is not None and (vals.get("band_rows"), vals.get("dgroup")) != tuple(map(str, tile)):
-        return None
-    return float(vals[key])
+    kernel sources and, given tile = (band_rows, dgroup), with that tile (None otherwise).
+    The summary of the default tile (8 rows, 4096 directions) is `path`; that of another tile
+    the tuner may pick carries the tile in its name (`..._8x2048.md`)."""
+    paths = [path]
+    if tile is not None:
+        paths.append(path[:-3] + f"_{tile[0]}x{tile[1]}.md")
+    for p in paths:
+        vals = {}
+        try:
+            for line in open(p):
+                if ":" in line and not line.startswith((" ", "#", "|")):
+                    k, v = line.split(":", 1)
+                    vals[k.strip()] = v.strip()
+        except OSError:
+            continue
+        if vals.get("kernel_sources_sha256") != kernel_sources_sha() or key not in vals:
+            continue
+        if tile is not None and (vals.get("band_rows"), vals.get("dgroup")) != tuple(map(str, tile)):
+            continue
+        return float(vals[key])
+    return None
 
 
 def traffic_from_profile(tile):
-    """HBM bytes per pass from the committed counter summary (None if absent, stale or
-    taken with another tile shape)."""
+    """HBM bytes per pass from the committed counter summary of this tile (None if absent
+    or stale)."""
     return _profile_figure(TRAFFIC_FILE, "traffic_bytes_per_pass", tile)
 
 
 def valu_from_profile(tile):
-    """VALU wavefront instructions per pass from the committed counter summary (None if
-    absent, stale or taken with another tile shape)"""
+    """VALU wavefront instructions per pass from the committed counter summary of this tile
+    (None if absent or stale)"""
     return _profile_figure(VALU_FILE, "valu_insts_per_pass", tile)
 
 

@@ -503,14 +503,14 @@ static int radon_fast_load(wrt_job* j, const double* in) {
   for (size_t b = 0; b < f->nbricks; ++b) nonempty += flags[b] != 0;
   /* enough tasks to fill 256 CUs several times, at least 16 directions (one per wave).
    * More directions per task even out the ends of the 16 waves' shares and stage fewer
-   * bricks (256^3 with 256 phi x 128 theta = 32768 directions and 257 shifts: 1.085 s per
-   * pass with groups of 512, 1.024 s
-   * with 1024, 1.005 s with 2048, 1.015 s with 4096), but longer tasks interleave the
-   * served jobs that run together more coarsely (serve p99 650 ms against 550 ms with the
-   * 256^3 jobs' 8192 directions in groups of 1600 rather than 512).  So at least 16 groups
-   * per brick: groups of at most max(512, ndir / 16) directions, and never above 2048. */
-  long long cap = ndir / 16;
-  cap = cap < 512 ? 512 : (cap > 2048 ? 2048 : cap);
+   * bricks: 256^3 with 256 phi x 128 theta = 32768 directions and 257 shifts takes 0.728 s
+   * per pass with groups of 1024, 0.6755 s with 2048 and 0.6695 s with 4096 (about 400
+   * non-empty bricks: 3200 tasks of 4096 directions on 256 CUs).  So at least 8 groups per
+   * brick: groups of at most max(512, ndir / 8) directions, and never above 4096 (the tuner
+   * also times half the group).  Served jobs run in the persistent batch kernel, whose
+   * planner sizes its own groups. */
+  long long cap = ndir / 8;
+  cap = cap < 512 ? 512 : (cap > 4096 ? 4096 : cap);
   long long want = (long long)ndir * (nonempty ? nonempty : 1) / 2048;
   int dg = (int)(want < 16 ? 16 : (want > cap ? cap : want));
   dg = (dg + 15) & ~15;
