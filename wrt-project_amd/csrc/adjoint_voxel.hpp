@@ -5,7 +5,8 @@
 // (-1 + 1/n + i*2/n, radon3d_ajoint.c:248-250): the sum over directions of
 // (2pi/nphi) w_theta S(x . n), S the natural cubic spline of the direction's row over the
 // shift grid and zero for |s| > 1.  Each interval's cubic is an FP32 float4 rounded from the
-// FP64 spline, in the interval's unit variable tt = (s - x_i)/h; s is formed in FP32, and
+// FP64 spline, in the interval's unit variable tt = (s - x_i)/h; s is formed in FP32 as
+// z n_z + (x n_x + y n_y) (the bracket shared by a thread's voxels, which differ in z), and
 // u = (s - s0)/ds with one fma gives the interval i (truncated, clamped) and tt = u - i.  An
 // interval decision that differs from the FP64 one near a knot changes the value only at
 // the rounding level (the spline is C2 across knots); the cut |s| <= 1 is not continuous
@@ -52,10 +53,9 @@ __device__ __forceinline__ void adjoint_tile_voxels(long long tile, int lane, in
 template <int V, int MODE, class DirP, class TrigP, class CubP>
 __device__ __forceinline__ void adjoint_dir_sum(DirP __restrict__ dirs, TrigP __restrict__ trig,
                                                 int ndir, CubP __restrict__ cub, int ns,
-                                                float s0, float inv_ds, const double* xd,
-                                                const double* yd, const double* zd,
-                                                const float* x, const float* y, const float* z,
-                                                float* out) {
+                                                float s0, float inv_ds, double xd, double yd,
+                                                const double* zd, float x, float y,
+                                                const float* z, float* out) {
   constexpr bool INSIDE = MODE >= 1;
   float acc[V], comp[V];
 #pragma unroll
@@ -75,18 +75,25 @@ __device__ __forceinline__ void adjoint_dir_sum(DirP __restrict__ dirs, TrigP __
     int iv[V][4];
     bool inv[V][4];
     float4 Cv[V][4];
+    float sxy[4];  // x n_x + y n_y, shared by the thread's voxels
+#pragma unroll
+    for (int u = 0; u < 4; ++u) sxy[u] = fmaf(x, D[u].x, y * D[u].y);
 #pragma unroll
     for (int v = 0; v < V; ++v)
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        const float s = fmaf(x[v], D[u].x, fmaf(y[v], D[u].y, z[v] * D[u].z));
+        const float s = fmaf(z[v], D[u].z, sxy[u]);
         bool in = true;
         if (!INSIDE) {
           in = fabsf(s) <= 1.0f;
-          if (fabsf(fabsf(s) - 1.0f) <= 1e-5f) {
+          // the FP64 re-decision as a branch of the whole wave on the ballot (a scalar
+          // branch, no exec mask saved and restored per evaluation); each lane takes its
+          // result only if its s is near the cut
+          const bool nr = fabsf(fabsf(s) - 1.0f) <= 1e-5f;
+          if (__builtin_expect(__builtin_amdgcn_ballot_w64(nr) != 0ull, 0)) {
             const double4 T = trig[d + u];
-            const double sd = xd[v] * T.x * T.y + yd[v] * T.x * T.z + zd[v] * T.w;
-            in = !(fabs(sd) > 1.0);
+            const double sd = xd * T.x * T.y + yd * T.x * T.z + zd[v] * T.w;
+            in = nr ? !(fabs(sd) > 1.0) : in;
           }
         }
         const float uu = fmaf(s, inv_ds, u0);
@@ -127,16 +134,18 @@ __device__ __forceinline__ void adjoint_dir_sum(DirP __restrict__ dirs, TrigP __
   }
   for (; d < ndir; ++d) {
     const float4 D = dirs[d];
+    const float sxy = fmaf(x, D.x, y * D.y);
 #pragma unroll
     for (int v = 0; v < V; ++v) {
-      const float s = fmaf(x[v], D.x, fmaf(y[v], D.y, z[v] * D.z));
+      const float s = fmaf(z[v], D.z, sxy);
       bool in = true;
       if (!INSIDE) {
         in = fabsf(s) <= 1.0f;
-        if (fabsf(fabsf(s) - 1.0f) <= 1e-5f) {
+        const bool nr = fabsf(fabsf(s) - 1.0f) <= 1e-5f;
+        if (__builtin_expect(__builtin_amdgcn_ballot_w64(nr) != 0ull, 0)) {
           const double4 T = trig[d];
-          const double sd = xd[v] * T.x * T.y + yd[v] * T.x * T.z + zd[v] * T.w;
-          in = !(fabs(sd) > 1.0);
+          const double sd = xd * T.x * T.y + yd * T.x * T.z + zd[v] * T.w;
+          in = nr ? !(fabs(sd) > 1.0) : in;
         }
       }
       if (in) {
@@ -156,8 +165,8 @@ __device__ __forceinline__ void adjoint_dir_sum(DirP __restrict__ dirs, TrigP __
   for (int v = 0; v < V; ++v) out[v] = acc[v];
 }
 
-// V voxels per thread, k[0..V-1] of the [z][x][y] output; dirs / trig / cub as in
-// wrt_batch_item.  Called by all 64 lanes of a wavefront (the inside test is a ballot).
+// V voxels per thread, k[0..V-1] of the [z][x][y] output, all with the x and y of k[0] (as
+// adjoint_tile_voxels gives them); dirs / trig / cub as in wrt_batch_item.  Called by all 64 lanes of a wavefront (the inside test is a ballot).
 // Directions go in fours and every voxel's four cubic gathers are issued before any is
 // used, so a thread keeps 4V table loads in flight; each voxel's sum still runs in
 // direction order, so the result of a voxel does not depend on V or on the tiling.  The
@@ -172,21 +181,21 @@ __device__ __forceinline__ void adjoint_voxels_f32(DirP __restrict__ dirs, TrigP
                                                    int ndir, CubP __restrict__ cub, int ns,
                                                    float s0, float inv_ds, float ds, int n,
                                                    const long long* k, float* out) {
-  double xd[V], yd[V], zd[V];
-  float x[V], y[V], z[V];
+  // the thread's voxels share x and y (the tiles give them one column of z)
+  const int iy = (int)(k[0] % n);
+  const int ix = (int)((k[0] / n) % n);
+  const double xd = (-1.0) + (1.0 / n) + ix * (2.0 / n);
+  const double yd = (-1.0) + (1.0 / n) + iy * (2.0 / n);
+  const float x = (float)xd, y = (float)yd;
+  double zd[V];
+  float z[V];
   bool inside = true;
 #pragma unroll
   for (int v = 0; v < V; ++v) {
-    const int iy = (int)(k[v] % n);
-    const int ix = (int)((k[v] / n) % n);
     const int iz = (int)(k[v] / ((long long)n * n));
-    xd[v] = (-1.0) + (1.0 / n) + ix * (2.0 / n);
-    yd[v] = (-1.0) + (1.0 / n) + iy * (2.0 / n);
     zd[v] = (-1.0) + (1.0 / n) + iz * (2.0 / n);
-    x[v] = (float)xd[v];
-    y[v] = (float)yd[v];
     z[v] = (float)zd[v];
-    inside = inside && xd[v] * xd[v] + yd[v] * yd[v] + zd[v] * zd[v] <= (1.0 - 1e-4) * (1.0 - 1e-4);
+    inside = inside && xd * xd + yd * yd + zd[v] * zd[v] <= (1.0 - 1e-4) * (1.0 - 1e-4);
   }
   // the grid covers [-1 + 5e-5, 1 - 5e-5] (wave-uniform)
   const bool covers = s0 <= -1.0f + 5e-5f && fmaf((float)(ns - 1), ds, s0) >= 1.0f - 5e-5f;
