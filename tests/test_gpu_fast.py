@@ -144,6 +144,31 @@ def test_fast_adjoint_f32_cut_decisions(ctx, n, nphi, nth):
     assert np.max(np.abs(got - ref)) <= 1e-4 * np.max(np.abs(ref)), np.max(np.abs(got - ref))
 
 
+@pytest.mark.parametrize("ns", [65, 2049, 2050])
+def test_fast_adjoint_f32_row_lengths(ctx, ns):
+    # The per-job launch stages chunks of the directions' cubic rows in LDS, a multiple of 4
+    # directions whose rows fit 8192 float4: 65 shifts give chunks of 128 directions (35
+    # directions: one partial chunk, 3 single directions past the last four), 2049 shifts
+    # chunks of exactly 4 (9 chunks), and 2050 rows too long for LDS (the gathers read the
+    # table in memory, as the persistent batch kernel's always do).  Noisy rows against the
+    # batch kernel, bit for bit; smooth rows against the plain FP64 kernel (noisy rows on a
+    # fine grid have slopes of order 1/ds, where the FP32 s alone errs by 1e-4 relative).
+    n = 20
+    phi, th, w, s = ctx.radon_grid_init(7, 5, ns)
+    j = ctx.job("adjoint", "f32", n=n, phi=phi, theta=th, theta_w=w, shift=s)
+    j.load(wrt.gaussian((len(s), len(phi), len(th)), seed=ns))
+    j.run()
+    alone = j.fetch()
+    ctx.batch_run([j])
+    assert np.array_equal(j.fetch(), alone)
+    j.close()
+    radon, _ = ctx.radon_grid(wrt.phantom(n, True, 10, ns, 0.0), phi, th, s, prec="f64")
+    ref, _ = ctx.radon_adjoint(radon, phi, th, w, s, n, prec="f64")
+    got, _ = ctx.radon_adjoint(radon, phi, th, w, s, n, prec="f32")
+    assert orc.rel_l2(got, ref) <= 1e-5, orc.rel_l2(got, ref)
+    assert np.max(np.abs(got - ref)) <= 1e-5 * np.max(np.abs(ref)), np.max(np.abs(got - ref))
+
+
 @pytest.mark.parametrize("n", [32, 40, 64])
 def test_fast_f32_structured_directions_on_faces(ctx, n):
     # Non-zero values up to the cube's faces, axis-aligned and diagonal directions, and

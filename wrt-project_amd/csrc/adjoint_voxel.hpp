@@ -50,24 +50,26 @@ __device__ __forceinline__ void adjoint_tile_voxels(long long tile, int lane, in
 // grid covers [-1 + 5e-5, 1 - 5e-5], so u = (s - s0)/ds of such an s lies in [0, ns - 1) and
 // needs no clamp: the interval is trunc(u) and tt = fract(u) (exact: u - floor(u) is
 // representable).  Every mode gives the same result, bit for bit, with fewer instructions.
+//
+// The directions [d_begin, d_end) of the sum, with the running sums acc and their Kahan
+// compensations comp carried in and out: d_begin is a multiple of 4, and d_end is too unless
+// it is ndir (the single directions past the last group of four are added only there), so
+// that a sum taken in several ranges is the sum taken at once, bit for bit.  The cubic row
+// of direction d is cub + (d - cub_d0) * (ns - 1): the whole table (cub_d0 = 0), or a chunk
+// of rows staged in LDS that starts at direction cub_d0.
 template <int V, int MODE, class DirP, class TrigP, class CubP>
-__device__ __forceinline__ void adjoint_dir_sum(DirP __restrict__ dirs, TrigP __restrict__ trig,
-                                                int ndir, CubP __restrict__ cub, int ns,
-                                                float s0, float inv_ds, double xd, double yd,
-                                                const double* zd, float x, float y,
-                                                const float* z, float* out) {
+__device__ __forceinline__ void adjoint_dir_range(DirP __restrict__ dirs, TrigP __restrict__ trig,
+                                                  int d_begin, int d_end, int ndir,
+                                                  CubP __restrict__ cub, int cub_d0, int ns,
+                                                  float s0, float inv_ds, double xd, double yd,
+                                                  const double* zd, float x, float y,
+                                                  const float* z, float* acc, float* comp) {
   constexpr bool INSIDE = MODE >= 1;
-  float acc[V], comp[V];
-#pragma unroll
-  for (int v = 0; v < V; ++v) {
-    acc[v] = 0.f;
-    comp[v] = 0.f;
-  }
   const int last = ns - 2;
   const float lastf = (float)last;  // exact: ns < 2^24
   const float u0 = -s0 * inv_ds;  // u = s * inv_ds + u0 = (s - s0) / ds
-  int d = 0;
-  for (; d + 4 <= ndir; d += 4) {
+  int d = d_begin;
+  for (; d + 4 <= d_end; d += 4) {
     float4 D[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) D[u] = dirs[d + u];  // uniform across the wavefront
@@ -112,7 +114,7 @@ __device__ __forceinline__ void adjoint_dir_sum(DirP __restrict__ dirs, TrigP __
     // each direction's row of cubics from its scalar base, the interval as a 32-bit offset
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      const auto row = cub + (long long)(d + u) * (ns - 1);
+      const auto row = cub + (long long)(d + u - cub_d0) * (ns - 1);
 #pragma unroll
       for (int v = 0; v < V; ++v) Cv[v][u] = row[(unsigned)iv[v][u]];
     }
@@ -132,6 +134,7 @@ __device__ __forceinline__ void adjoint_dir_sum(DirP __restrict__ dirs, TrigP __
       acc[v] = tk;
     }
   }
+  if (d_end < ndir) return;
   for (; d < ndir; ++d) {
     const float4 D = dirs[d];
     const float sxy = fmaf(x, D.x, y * D.y);
@@ -152,7 +155,7 @@ __device__ __forceinline__ void adjoint_dir_sum(DirP __restrict__ dirs, TrigP __
         const float uu = fmaf(s, inv_ds, u0);
         const int i = (int)__builtin_amdgcn_fmed3f(uu, 0.0f, lastf);
         const float t = uu - (float)i;
-        const float4 C = cub[(long long)d * (ns - 1) + (unsigned)i];
+        const float4 C = cub[(long long)(d - cub_d0) * (ns - 1) + (unsigned)i];
         const float val = fmaf(fmaf(fmaf(C.w, t, C.z), t, C.y), t, C.x);
         const float yk = fmaf(D.w, val, -comp[v]);
         const float tk = acc[v] + yk;
@@ -161,12 +164,57 @@ __device__ __forceinline__ void adjoint_dir_sum(DirP __restrict__ dirs, TrigP __
       }
     }
   }
-#pragma unroll
-  for (int v = 0; v < V; ++v) out[v] = acc[v];
 }
 
+// The voxels' coordinates (the thread's voxels share x and y: the tiles give them one column
+// of z) and the mode of the wavefront: 2 if every voxel lies within radius 1 - 1e-4 and the
+// grid covers [-1 + 5e-5, 1 - 5e-5], 1 if only the former (and MODE1), else 0.
+template <int V, bool MODE1 = true>
+struct adjoint_voxel_set {
+  double xd, yd, zd[V];
+  float x, y, z[V];
+  int mode;
+  __device__ __forceinline__ adjoint_voxel_set(const long long* k, int n, int ns, float s0,
+                                               float ds) {
+    const int iy = (int)(k[0] % n);
+    const int ix = (int)((k[0] / n) % n);
+    xd = (-1.0) + (1.0 / n) + ix * (2.0 / n);
+    yd = (-1.0) + (1.0 / n) + iy * (2.0 / n);
+    x = (float)xd;
+    y = (float)yd;
+    bool inside = true;
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      const int iz = (int)(k[v] / ((long long)n * n));
+      zd[v] = (-1.0) + (1.0 / n) + iz * (2.0 / n);
+      z[v] = (float)zd[v];
+      inside = inside && xd * xd + yd * yd + zd[v] * zd[v] <= (1.0 - 1e-4) * (1.0 - 1e-4);
+    }
+    // the grid covers [-1 + 5e-5, 1 - 5e-5] (wave-uniform)
+    const bool covers = s0 <= -1.0f + 5e-5f && fmaf((float)(ns - 1), ds, s0) >= 1.0f - 5e-5f;
+    const bool all_inside = __builtin_amdgcn_ballot_w64(!inside) == 0ull;
+    mode = all_inside && covers ? 2 : (MODE1 && all_inside ? 1 : 0);
+  }
+  // the directions [d_begin, d_end) in the wavefront's mode
+  template <class DirP, class TrigP, class CubP>
+  __device__ __forceinline__ void range(DirP dirs, TrigP trig, int d_begin, int d_end, int ndir,
+                                        CubP cub, int cub_d0, int ns, float s0, float inv_ds,
+                                        float* acc, float* comp) const {
+    if (mode == 2)
+      adjoint_dir_range<V, 2>(dirs, trig, d_begin, d_end, ndir, cub, cub_d0, ns, s0, inv_ds, xd,
+                              yd, zd, x, y, z, acc, comp);
+    else if (MODE1 && mode == 1)
+      adjoint_dir_range<V, 1>(dirs, trig, d_begin, d_end, ndir, cub, cub_d0, ns, s0, inv_ds, xd,
+                              yd, zd, x, y, z, acc, comp);
+    else
+      adjoint_dir_range<V, 0>(dirs, trig, d_begin, d_end, ndir, cub, cub_d0, ns, s0, inv_ds, xd,
+                              yd, zd, x, y, z, acc, comp);
+  }
+};
+
 // V voxels per thread, k[0..V-1] of the [z][x][y] output, all with the x and y of k[0] (as
-// adjoint_tile_voxels gives them); dirs / trig / cub as in wrt_batch_item.  Called by all 64 lanes of a wavefront (the inside test is a ballot).
+// adjoint_tile_voxels gives them); dirs / trig / cub as in wrt_batch_item.  Called by all 64
+// lanes of a wavefront (the inside test is a ballot).
 // Directions go in fours and every voxel's four cubic gathers are issued before any is
 // used, so a thread keeps 4V table loads in flight; each voxel's sum still runs in
 // direction order, so the result of a voxel does not depend on V or on the tiling.  The
@@ -181,31 +229,104 @@ __device__ __forceinline__ void adjoint_voxels_f32(DirP __restrict__ dirs, TrigP
                                                    int ndir, CubP __restrict__ cub, int ns,
                                                    float s0, float inv_ds, float ds, int n,
                                                    const long long* k, float* out) {
-  // the thread's voxels share x and y (the tiles give them one column of z)
-  const int iy = (int)(k[0] % n);
-  const int ix = (int)((k[0] / n) % n);
-  const double xd = (-1.0) + (1.0 / n) + ix * (2.0 / n);
-  const double yd = (-1.0) + (1.0 / n) + iy * (2.0 / n);
-  const float x = (float)xd, y = (float)yd;
-  double zd[V];
-  float z[V];
-  bool inside = true;
+  const adjoint_voxel_set<V, MODE1> vs(k, n, ns, s0, ds);
+  float acc[V], comp[V];
 #pragma unroll
   for (int v = 0; v < V; ++v) {
-    const int iz = (int)(k[v] / ((long long)n * n));
-    zd[v] = (-1.0) + (1.0 / n) + iz * (2.0 / n);
-    z[v] = (float)zd[v];
-    inside = inside && xd * xd + yd * yd + zd[v] * zd[v] <= (1.0 - 1e-4) * (1.0 - 1e-4);
+    acc[v] = 0.f;
+    comp[v] = 0.f;
   }
-  // the grid covers [-1 + 5e-5, 1 - 5e-5] (wave-uniform)
-  const bool covers = s0 <= -1.0f + 5e-5f && fmaf((float)(ns - 1), ds, s0) >= 1.0f - 5e-5f;
-  const bool all_inside = __builtin_amdgcn_ballot_w64(!inside) == 0ull;
-  if (all_inside && covers)
-    adjoint_dir_sum<V, 2>(dirs, trig, ndir, cub, ns, s0, inv_ds, xd, yd, zd, x, y, z, out);
-  else if (MODE1 && all_inside)
-    adjoint_dir_sum<V, 1>(dirs, trig, ndir, cub, ns, s0, inv_ds, xd, yd, zd, x, y, z, out);
-  else
-    adjoint_dir_sum<V, 0>(dirs, trig, ndir, cub, ns, s0, inv_ds, xd, yd, zd, x, y, z, out);
+  vs.range(dirs, trig, 0, ndir, ndir, cub, 0, ns, s0, inv_ds, acc, comp);
+#pragma unroll
+  for (int v = 0; v < V; ++v) out[v] = acc[v];
+}
+
+// The LDS-staged form over a workgroup of ADJ_LDS_WAVES wavefronts (all of its threads call
+// it): wavefront w < ntile takes tile tile0 + w, and the workgroup walks the directions in
+// chunks of `chunk` directions (a multiple of 4 with chunk * (ns - 1) <= PF * 1024) whose
+// cubic rows it copies into tab, so that every evaluation's 16-byte gather is an LDS read
+// (gathers from memory, one vector load through the CU's L1 per evaluation, bound the
+// unstaged form).  Each thread loads its PF float4 of the next chunk into registers while
+// the current one is evaluated and stores them between two barriers.  The sums are the
+// unstaged form's, bit for bit (adjoint_dir_range).  Returns with the barriers passed and
+// the outputs stored; a caller that reuses tab waits first.
+#define ADJ_LDS_F4 8192
+#define ADJ_LDS_WAVES 16
+template <int V, bool MODE1, int PF, class DirP, class TrigP, class CubP, class OutP>
+__device__ __forceinline__ void adjoint_staged_tiles(DirP dirs, TrigP trig, int ndir, CubP cub,
+                                                     int ns, float s0, float inv_ds, float ds,
+                                                     int n, long long tile0, int ntile,
+                                                     int chunk, WRT_LDS float4* tab, OutP out) {
+  constexpr int STRIDE = ADJ_LDS_WAVES * 64;
+  static_assert(PF * STRIDE <= ADJ_LDS_F4, "a chunk fits the table");
+  const int tid = threadIdx.x;
+  const int w = tid >> 6;
+  // wave-uniform: a wavefront past the last tile stages and waits with the others
+  const bool active = w < ntile;
+  long long k[V];
+  bool valid[V];
+  adjoint_tile_voxels<V>(tile0 + (active ? w : 0), tid & 63, n, k, valid);
+  const adjoint_voxel_set<V, MODE1> vs(k, n, ns, s0, ds);
+  float acc[V], comp[V];
+#pragma unroll
+  for (int v = 0; v < V; ++v) {
+    acc[v] = 0.f;
+    comp[v] = 0.f;
+  }
+  const int row = ns - 1;
+  // the thread's share of a chunk: elements tid + p * 1024, p < PF, in registers q0..q7
+  // (named, not an array, which the compiler keeps in scratch memory); an index past the
+  // chunk reads its last element and is not stored
+  static_assert(PF == 4 || PF == 8, "4 or 8 float4 per thread and chunk");
+  float4 q0, q1, q2, q3, q4, q5, q6, q7;
+  auto fetch = [&](decltype(cub) src, int cnt) {
+    q0 = src[min(tid + 0 * STRIDE, cnt - 1)];
+    q1 = src[min(tid + 1 * STRIDE, cnt - 1)];
+    q2 = src[min(tid + 2 * STRIDE, cnt - 1)];
+    q3 = src[min(tid + 3 * STRIDE, cnt - 1)];
+    if constexpr (PF == 8) {
+      q4 = src[min(tid + 4 * STRIDE, cnt - 1)];
+      q5 = src[min(tid + 5 * STRIDE, cnt - 1)];
+      q6 = src[min(tid + 6 * STRIDE, cnt - 1)];
+      q7 = src[min(tid + 7 * STRIDE, cnt - 1)];
+    }
+  };
+  auto store = [&](int p, const float4& v, int cnt) {
+    if (tid + p * STRIDE < cnt) tab[tid + p * STRIDE] = v;
+  };
+  fetch(cub, min(chunk, ndir) * row);
+  for (int d0 = 0; d0 < ndir; d0 += chunk) {
+    const int d1 = min(d0 + chunk, ndir);
+    const int cnt = (d1 - d0) * row;
+    __syncthreads();  // every wavefront is done with the previous chunk
+    store(0, q0, cnt);
+    store(1, q1, cnt);
+    store(2, q2, cnt);
+    store(3, q3, cnt);
+    if constexpr (PF == 8) {
+      store(4, q4, cnt);
+      store(5, q5, cnt);
+      store(6, q6, cnt);
+      store(7, q7, cnt);
+    }
+    __syncthreads();
+    // the next chunk, in flight while this one is evaluated
+    if (d1 < ndir) fetch(cub + (long long)d1 * row, (min(d1 + chunk, ndir) - d1) * row);
+    if (active)
+      vs.range(dirs, trig, d0, d1, ndir, (const WRT_LDS float4*)tab, d0, ns, s0, inv_ds, acc,
+               comp);
+  }
+  if (active) {
+#pragma unroll
+    for (int v = 0; v < V; ++v)
+      if (valid[v]) out[k[v]] = acc[v];
+  }
+}
+
+// directions per chunk of the staged form with PF float4 per thread, for rows of ns shifts
+// (0: the rows do not fit)
+__host__ __device__ inline int adjoint_chunk(int ns, int pf) {
+  return ns < 2 ? 0 : 4 * (pf * ADJ_LDS_WAVES * 64 / (4 * (ns - 1)));
 }
 
 #endif

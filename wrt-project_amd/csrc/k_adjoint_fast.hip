@@ -59,13 +59,34 @@ __global__ __launch_bounds__(256) void k_adjoint_f32(const float4* __restrict__ 
     if (valid[v]) out[k[v]] = r[v];
 }
 
+// the LDS-staged form (adjoint_staged_tiles): one tile per wavefront, 16 per workgroup
+__global__ __launch_bounds__(ADJ_LDS_WAVES * 64) void k_adjoint_f32_lds(
+    const float4* __restrict__ dirs, const double4* __restrict__ trig, int ndir,
+    const float4* __restrict__ cub, int ns, float s0, float inv_ds, float ds, int n,
+    long long ntiles, int chunk, float* __restrict__ out) {
+  __shared__ float4 tab[ADJ_LDS_F4];
+  const long long tile0 = (long long)blockIdx.x * ADJ_LDS_WAVES;
+  adjoint_staged_tiles<ADJ_VPT, true, 8>(dirs, trig, ndir, cub, ns, s0, inv_ds, ds, n, tile0,
+                                      (int)min((long long)ADJ_LDS_WAVES, ntiles - tile0), chunk,
+                                      (WRT_LDS float4*)tab, out);
+}
+
 extern "C" int wrtk_adjoint_f32(const void* dirs, const void* trig, int ndir, const void* cub,
                                 int ns, double s0, double ds, int n, float* out, hipStream_t st) {
   if (n <= 0) return WRT_OK;
   const long long ntiles = wrt_adj_tiles(n, ADJ_VPT);
-  k_adjoint_f32<<<dim3((unsigned)((ntiles + 3) / 4)), 256, 0, st>>>(
-      (const float4*)dirs, (const double4*)trig, ndir, (const float4*)cub, ns, (float)s0,
-      (float)(1.0 / ds), (float)ds, n, ntiles, out);
+  // directions per chunk: a multiple of 4 whose rows fit the LDS table
+  const int chunk = adjoint_chunk(ns, 8);
+  if (chunk >= 4) {
+    k_adjoint_f32_lds<<<dim3((unsigned)((ntiles + ADJ_LDS_WAVES - 1) / ADJ_LDS_WAVES)),
+                        ADJ_LDS_WAVES * 64, 0, st>>>(
+        (const float4*)dirs, (const double4*)trig, ndir, (const float4*)cub, ns, (float)s0,
+        (float)(1.0 / ds), (float)ds, n, ntiles, chunk, out);
+  } else {  // rows longer than 2048 intervals
+    k_adjoint_f32<<<dim3((unsigned)((ntiles + 3) / 4)), 256, 0, st>>>(
+        (const float4*)dirs, (const double4*)trig, ndir, (const float4*)cub, ns, (float)s0,
+        (float)(1.0 / ds), (float)ds, n, ntiles, out);
+  }
   WRT_LAUNCH_CHECK();
   return WRT_OK;
 }

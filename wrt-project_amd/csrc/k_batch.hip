@@ -7,9 +7,10 @@
 // the list is exhausted, so uneven tasks balance across the CUs; the host orders the list
 // longest first.  A plane-integral task is the brick task of the per-job launch
 // (radon_brick.hpp) with the job's arguments; a backprojection task is up to
-// WRT_ADJ_TASK_TILES voxel tiles of one job, one per wavefront, WRT_ADJ_VPT voxels per thread
-// (adjoint_voxel.hpp: more voxels per thread than the per-job launch's two keep more gathers
-// in flight at the one workgroup per CU the brick task's LDS allows).  Results are the
+// WRT_ADJ_TASK_TILES voxel tiles of one job, one per wavefront, WRT_ADJ_VPT voxels per
+// thread, with the directions' cubic rows staged in the brick's LDS (adjoint_staged_tiles,
+// in chunks of WRT_ADJ_PF float4 per thread, half the per-job launch's: the one register
+// allocation of this kernel has no room for more).  Results are the
 // per-job launches', bit for bit: every voxel's sum runs in the same order, and the plane
 // sums are exact 64-bit fixed-point sums, whatever the order of their sections.
 #include "adjoint_voxel.hpp"
@@ -36,7 +37,9 @@ __global__ __launch_bounds__(256) void k_batch_epilogue(const wrt_batch_seg* __r
 __global__ __launch_bounds__(FR_WAVES * 64) void k_batch(const wrt_batch_item* __restrict__ items,
                                                          const wrt_batch_task* __restrict__ tasks,
                                                          int ntasks, int* __restrict__ counter) {
-  __shared__ radon_lds L;
+  // 16-byte aligned: a backprojection task stages float4 cubic rows in L.sb
+  __shared__ __attribute__((aligned(16))) radon_lds L;
+  static_assert(sizeof(L.sb) >= ADJ_LDS_F4 * sizeof(float4), "the staged rows fit L.sb");
   __shared__ int s_task;
   for (;;) {
     __syncthreads();  // the previous task is done with L and s_task
@@ -53,8 +56,15 @@ __global__ __launch_bounds__(FR_WAVES * 64) void k_batch(const wrt_batch_item* _
       // register allocation of this kernel)
       radon_brick_task<false>(it.ra, t.a, t.b, it.h, it.dc, it.inv_dc, L, t.c != 0,
                               it.ra.bf16 != 0, it.pow2 != 0);
+    } else if (adjoint_chunk(it.ans, WRT_ADJ_PF) >= 4) {
+      // tile a + w for wavefront w < b of the task's tiles, the cubic rows staged in LDS
+      adjoint_staged_tiles<WRT_ADJ_VPT, false, WRT_ADJ_PF>(
+          (const WRT_CONST float4*)it.adirs, (const WRT_CONST double4*)it.atrig, it.andir,
+          (const WRT_GLOBAL float4*)it.acub, it.ans, (float)it.as0, (float)(1.0 / it.ads),
+          (float)it.ads, it.an, (long long)t.a, t.b, adjoint_chunk(it.ans, WRT_ADJ_PF),
+          (WRT_LDS float4*)L.sb, (WRT_GLOBAL float*)it.aout);
     } else {
-      // tile a + w for wavefront w < b of the task's tiles (one tile per wavefront)
+      // rows too long for LDS: tile a + w for wavefront w < b, gathers from memory
       const int w = threadIdx.x >> 6;
       if (w < t.b) {
         long long k[WRT_ADJ_VPT];

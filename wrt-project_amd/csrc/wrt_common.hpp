@@ -13,14 +13,17 @@
 // Address spaces for pointers a kernel reads from memory (the persistent batch kernel's
 // argument blocks), where the compiler cannot infer them as it does for kernel arguments:
 // global (vector loads and atomics without the flat path's extra waits) and constant
-// (wave-uniform reads of tables the kernel never writes become scalar loads).
+// (wave-uniform reads of tables the kernel never writes become scalar loads); and LDS, for
+// a staged table handed to code that also reads tables from memory (LDS reads, not flat).
 // (The host pass, which only checks device code, sees them as plain pointers.)
 #if defined(__HIP_DEVICE_COMPILE__)
 #define WRT_GLOBAL __attribute__((address_space(1)))
 #define WRT_CONST __attribute__((address_space(4)))
+#define WRT_LDS __attribute__((address_space(3)))
 #else
 #define WRT_GLOBAL
 #define WRT_CONST
+#define WRT_LDS
 #endif
 
 // fmod(a, d) for a >= 0, d > 0, exact (equal to the C library's fmod).  q = trunc(a/d) is

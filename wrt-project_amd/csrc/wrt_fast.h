@@ -125,8 +125,11 @@ static inline long long wrt_adj_tiles(int n, int vpt) {
   return txy * txy * (long long)((n + 4 * vpt - 1) / (4 * vpt));
 }
 /* voxels per thread of a backprojection task in the persistent kernel; a task is up to one
- * tile per wavefront of its 1024-thread workgroup */
-#define WRT_ADJ_VPT 3
+ * tile per wavefront of its 1024-thread workgroup, which stages the directions' cubic rows
+ * in LDS in chunks of WRT_ADJ_PF float4 per thread (4096 float4; 2 voxels and 4 float4 per
+ * thread keep the kernel within 128 registers with no spill) */
+#define WRT_ADJ_VPT 2
+#define WRT_ADJ_PF 4
 #define WRT_ADJ_TASK_TILES 16
 
 /* radon: (item, brick, direction group, 1 if the brick touches a cube face);
