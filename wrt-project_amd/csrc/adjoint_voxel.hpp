@@ -242,7 +242,7 @@ __device__ __forceinline__ void adjoint_voxels_f32(DirP __restrict__ dirs, TrigP
 }
 
 // The LDS-staged form over a workgroup of ADJ_LDS_WAVES wavefronts (all of its threads call
-// it): wavefront w < ntile takes tile tile0 + w, and the workgroup walks the directions in
+// it): wavefront w < ntile takes tile order[tile0 + w], and the workgroup walks the directions in
 // chunks of `chunk` directions (a multiple of 4 with chunk * (ns - 1) <= PF * 1024) whose
 // cubic rows it copies into tab, so that every evaluation's 16-byte gather is an LDS read
 // (gathers from memory, one vector load through the CU's L1 per evaluation, bound the
@@ -252,11 +252,13 @@ __device__ __forceinline__ void adjoint_voxels_f32(DirP __restrict__ dirs, TrigP
 // the outputs stored; a caller that reuses tab waits first.
 #define ADJ_LDS_F4 8192
 #define ADJ_LDS_WAVES 16
-template <int V, bool MODE1, int PF, class DirP, class TrigP, class CubP, class OutP>
+template <int V, bool MODE1, int PF, class DirP, class TrigP, class CubP, class OrdP,
+          class OutP>
 __device__ __forceinline__ void adjoint_staged_tiles(DirP dirs, TrigP trig, int ndir, CubP cub,
                                                      int ns, float s0, float inv_ds, float ds,
-                                                     int n, long long tile0, int ntile,
-                                                     int chunk, WRT_LDS float4* tab, OutP out) {
+                                                     int n, OrdP order, long long tile0,
+                                                     int ntile, int chunk, WRT_LDS float4* tab,
+                                                     OutP out) {
   constexpr int STRIDE = ADJ_LDS_WAVES * 64;
   static_assert(PF * STRIDE <= ADJ_LDS_F4, "a chunk fits the table");
   const int tid = threadIdx.x;
@@ -265,7 +267,7 @@ __device__ __forceinline__ void adjoint_staged_tiles(DirP dirs, TrigP trig, int 
   const bool active = w < ntile;
   long long k[V];
   bool valid[V];
-  adjoint_tile_voxels<V>(tile0 + (active ? w : 0), tid & 63, n, k, valid);
+  adjoint_tile_voxels<V>(order[tile0 + (active ? w : 0)], tid & 63, n, k, valid);
   const adjoint_voxel_set<V, MODE1> vs(k, n, ns, s0, ds);
   float acc[V], comp[V];
 #pragma unroll

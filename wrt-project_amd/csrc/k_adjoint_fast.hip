@@ -35,10 +35,9 @@ extern "C" int wrtk_adjoint_cubics(const double* x, int ns, const double* tab, v
   return WRT_OK;
 }
 
-// voxels per thread of the per-job launch
-#ifndef ADJ_VPT
-#define ADJ_VPT 2
-#endif
+// voxels per thread of the per-job launch: the persistent kernel's, whose tiling the job's
+// tile order lists
+#define ADJ_VPT WRT_ADJ_VPT
 
 // one tile of 4 x 4 x 4*ADJ_VPT voxels per wavefront (ADJ_VPT voxels per thread), four per
 // workgroup
@@ -63,16 +62,17 @@ __global__ __launch_bounds__(256) void k_adjoint_f32(const float4* __restrict__ 
 __global__ __launch_bounds__(ADJ_LDS_WAVES * 64) void k_adjoint_f32_lds(
     const float4* __restrict__ dirs, const double4* __restrict__ trig, int ndir,
     const float4* __restrict__ cub, int ns, float s0, float inv_ds, float ds, int n,
-    long long ntiles, int chunk, float* __restrict__ out) {
+    const int* __restrict__ order, long long ntiles, int chunk, float* __restrict__ out) {
   __shared__ float4 tab[ADJ_LDS_F4];
   const long long tile0 = (long long)blockIdx.x * ADJ_LDS_WAVES;
-  adjoint_staged_tiles<ADJ_VPT, true, 8>(dirs, trig, ndir, cub, ns, s0, inv_ds, ds, n, tile0,
-                                      (int)min((long long)ADJ_LDS_WAVES, ntiles - tile0), chunk,
-                                      (WRT_LDS float4*)tab, out);
+  adjoint_staged_tiles<ADJ_VPT, true, 8>(dirs, trig, ndir, cub, ns, s0, inv_ds, ds, n, order,
+                                         tile0, (int)min((long long)ADJ_LDS_WAVES, ntiles - tile0),
+                                         chunk, (WRT_LDS float4*)tab, out);
 }
 
 extern "C" int wrtk_adjoint_f32(const void* dirs, const void* trig, int ndir, const void* cub,
-                                int ns, double s0, double ds, int n, float* out, hipStream_t st) {
+                                int ns, double s0, double ds, int n, const int* order, float* out,
+                                hipStream_t st) {
   if (n <= 0) return WRT_OK;
   const long long ntiles = wrt_adj_tiles(n, ADJ_VPT);
   // directions per chunk: a multiple of 4 whose rows fit the LDS table
@@ -81,7 +81,7 @@ extern "C" int wrtk_adjoint_f32(const void* dirs, const void* trig, int ndir, co
     k_adjoint_f32_lds<<<dim3((unsigned)((ntiles + ADJ_LDS_WAVES - 1) / ADJ_LDS_WAVES)),
                         ADJ_LDS_WAVES * 64, 0, st>>>(
         (const float4*)dirs, (const double4*)trig, ndir, (const float4*)cub, ns, (float)s0,
-        (float)(1.0 / ds), (float)ds, n, ntiles, chunk, out);
+        (float)(1.0 / ds), (float)ds, n, order, ntiles, chunk, out);
   } else {  // rows longer than 2048 intervals
     k_adjoint_f32<<<dim3((unsigned)((ntiles + 3) / 4)), 256, 0, st>>>(
         (const float4*)dirs, (const double4*)trig, ndir, (const float4*)cub, ns, (float)s0,

@@ -57,11 +57,13 @@ __global__ __launch_bounds__(FR_WAVES * 64) void k_batch(const wrt_batch_item* _
       radon_brick_task<false>(it.ra, t.a, t.b, it.h, it.dc, it.inv_dc, L, t.c != 0,
                               it.ra.bf16 != 0, it.pow2 != 0);
     } else if (adjoint_chunk(it.ans, WRT_ADJ_PF) >= 4) {
-      // tile a + w for wavefront w < b of the task's tiles, the cubic rows staged in LDS
+      // tile aorder[a + w] for wavefront w < b of the task's tiles, the cubic rows staged in
+      // LDS
       adjoint_staged_tiles<WRT_ADJ_VPT, false, WRT_ADJ_PF>(
           (const WRT_CONST float4*)it.adirs, (const WRT_CONST double4*)it.atrig, it.andir,
           (const WRT_GLOBAL float4*)it.acub, it.ans, (float)it.as0, (float)(1.0 / it.ads),
-          (float)it.ads, it.an, (long long)t.a, t.b, adjoint_chunk(it.ans, WRT_ADJ_PF),
+          (float)it.ads, it.an, (const WRT_CONST int*)it.aorder, (long long)t.a, t.b,
+          adjoint_chunk(it.ans, WRT_ADJ_PF),
           (WRT_LDS float4*)L.sb, (WRT_GLOBAL float*)it.aout);
     } else {
       // rows too long for LDS: tile a + w for wavefront w < b, gathers from memory

@@ -65,11 +65,12 @@ int wrtk_ray_T_plain(const double* g, int n, const double* z, int nz, const doub
 
 /* FP32 backprojection: per-interval FP32 cubics from the FP64 packed spline table, then one
  * thread per voxel over float4 directions (n.x, n.y, n.z, weight); uniform shift grid.
- * trig [ndir][4] = sin(theta) cos(phi) sin(phi) cos(theta) in FP64 (cut decisions) */
+ * trig [ndir][4] = sin(theta) cos(phi) sin(phi) cos(theta) in FP64 (cut decisions); order
+ * the voxel tiles in launch order (wrt_adj_tile_order) */
 int wrtk_adjoint_cubics(const double* x, int ns, const double* tab, void* cub, long long rows,
                         hipStream_t st);
 int wrtk_adjoint_f32(const void* dirs, const void* trig, int ndir, const void* cub, int ns,
-                     double s0, double ds, int n, float* out, hipStream_t st);
+                     double s0, double ds, int n, const int* order, float* out, hipStream_t st);
 
 /* FP32 rays: vol32 is the FP32 volume in [z][x][y] order; zi / zc the slices' index and
  * fraction as the reference decides them (zi < 0: |z| > 1); out [z][s][phi] FP32 */

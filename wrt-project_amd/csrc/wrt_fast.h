@@ -114,6 +114,7 @@ typedef struct {
   const void* atrig;        /* adjoint: [ndir] double4 (sin t, cos p, sin p, cos t)      */
   const void* acub;         /* adjoint: [ndir][ns-1] float4 cubics                       */
   float* aout;              /* adjoint: n^3 [z][x][y]                                    */
+  const int* aorder;        /* adjoint: the tiles in launch order (wrt_adj_tile_order)     */
   int andir, ans, an;
   double as0, ads;          /* adjoint: uniform shift grid s_i = s0 + i*ds               */
 } wrt_batch_item;
@@ -124,6 +125,34 @@ static inline long long wrt_adj_tiles(int n, int vpt) {
   const long long txy = (long long)((n + 3) / 4);
   return txy * txy * (long long)((n + 4 * vpt - 1) / (4 * vpt));
 }
+/* The tiles of an n^3 backprojection with vpt voxels per thread in launch order: first the
+ * tiles with a voxel at radius 1 - 1e-4 or beyond (the general mode, which takes about 1.5
+ * times the others' work), then the rest, each in index order.  The 16 tiles of a staged
+ * workgroup wait for each other at every chunk of directions, so a workgroup of one kind
+ * does not idle behind the other kind's work; the longer kind goes first. */
+static inline void wrt_adj_tile_order(int n, int vpt, int* order) {
+  const int ntx = (n + 3) / 4, ntz = (n + 4 * vpt - 1) / (4 * vpt);
+  const double r2 = (1.0 - 1e-4) * (1.0 - 1e-4);
+  long long k = 0;
+  for (int pass = 0; pass < 2; ++pass)
+    for (int tz = 0; tz < ntz; ++tz)
+      for (int tx = 0; tx < ntx; ++tx)
+        for (int ty = 0; ty < ntx; ++ty) {
+          /* the largest |coordinate| of the tile's voxel centres along each axis (indices
+           * clamped into the volume, as the kernel clamps them) */
+          double m2 = 0.0;
+          const int lo[3] = {4 * tx, 4 * ty, 4 * vpt * tz}, len[3] = {4, 4, 4 * vpt};
+          for (int c = 0; c < 3; ++c) {
+            const int a = lo[c] < n - 1 ? lo[c] : n - 1;
+            const int b = lo[c] + len[c] - 1 < n - 1 ? lo[c] + len[c] - 1 : n - 1;
+            const double xa = (-1.0) + (1.0 / n) + a * (2.0 / n);
+            const double xb = (-1.0) + (1.0 / n) + b * (2.0 / n);
+            m2 += xa * xa > xb * xb ? xa * xa : xb * xb;
+          }
+          if ((m2 <= r2) == (pass == 1)) order[k++] = (tz * ntx + tx) * ntx + ty;
+        }
+}
+
 /* voxels per thread of a backprojection task in the persistent kernel; a task is up to one
  * tile per wavefront of its 1024-thread workgroup, which stages the directions' cubic rows
  * in LDS in chunks of WRT_ADJ_PF float4 per thread (4096 float4; 2 voxels and 4 float4 per

@@ -66,6 +66,7 @@ typedef struct {
   void* dirs;      /* [ndir] float4 (n.x, n.y, n.z, (2pi/nphi) w_theta) */
   double* trig;    /* [ndir][4] sin t, cos p, sin p, cos t: exact cut decisions near |s| = 1 */
   float* out;      /* n^3 [z][x][y] */
+  int* order;      /* the voxel tiles in launch order (wrt_adj_tile_order) */
   double s0, ds;
 } adjoint_fast;
 
@@ -195,7 +196,7 @@ int wrt_fast_supported(const wrt_job_desc* d) {
 static size_t adjoint_fast_bytes(const wrt_job_desc* d) {
   const size_t ndir = (size_t)d->nphi * d->ntheta;
   return (size_t)d->nshift * ndir * 8 + ndir * (size_t)(d->nshift - 1) * 16 + ndir * 48 +
-         (size_t)d->n * d->n * d->n * 4;
+         (size_t)d->n * d->n * d->n * 4 + (size_t)wrt_adj_tiles(d->n, WRT_ADJ_VPT) * 4;
 }
 
 static void adjoint_fast_destroy(wrt_job* j) {
@@ -205,6 +206,7 @@ static void adjoint_fast_destroy(wrt_job* j) {
   hipFree(f->dirs);
   hipFree(f->trig);
   hipFree(f->out);
+  hipFree(f->order);
   free(f);
   j->fast = NULL;
 }
@@ -222,12 +224,16 @@ static int adjoint_fast_create(wrt_job* j) {
       hipMalloc((void**)&f->trig, ndir * 32) != hipSuccess ||
       hipMalloc((void**)&f->out, (size_t)d->n * d->n * d->n * 4) != hipSuccess)
     return WRT_E_NOMEM;
+  const size_t ntiles = (size_t)wrt_adj_tiles(d->n, WRT_ADJ_VPT);
+  if (hipMalloc((void**)&f->order, ntiles * 4) != hipSuccess) return WRT_E_NOMEM;
   /* unit normals (sin t cos p, sin t sin p, cos t) and weights (2pi/nphi) w_t, host libm */
   float* h = (float*)malloc(ndir * 16);
   double* tr = (double*)malloc(ndir * 32);
-  if (!h || !tr) {
+  int* ord = (int*)malloc(ntiles * 4);
+  if (!h || !tr || !ord) {
     free(h);
     free(tr);
+    free(ord);
     return WRT_E_NOMEM;
   }
   const double dphi = 2 * M_PI / d->nphi;
@@ -245,13 +251,16 @@ static int adjoint_fast_create(wrt_job* j) {
       v[2] = (float)t[3];
       v[3] = (float)(dphi * d->theta_w[it]);
     }
+  wrt_adj_tile_order(d->n, WRT_ADJ_VPT, ord);
   int rc = WRT_OK;
   if (hipMemcpyAsync(f->dirs, h, ndir * 16, hipMemcpyHostToDevice, j->st) != hipSuccess ||
       hipMemcpyAsync(f->trig, tr, ndir * 32, hipMemcpyHostToDevice, j->st) != hipSuccess ||
+      hipMemcpyAsync(f->order, ord, ntiles * 4, hipMemcpyHostToDevice, j->st) != hipSuccess ||
       hipStreamSynchronize(j->st) != hipSuccess)
     rc = WRT_E_HIP;
   free(h);
   free(tr);
+  free(ord);
   return rc;
 }
 
@@ -272,7 +281,7 @@ static int adjoint_fast_enqueue(wrt_job* j) {
   const wrt_job_desc* d = &j->d;
   HIPCHK(hipEventRecord(j->e1, j->st));
   const int rc = wrtk_adjoint_f32(f->dirs, f->trig, d->nphi * d->ntheta, f->cub, d->nshift, f->s0, f->ds,
-                                  d->n, f->out, j->st);
+                                  d->n, f->order, f->out, j->st);
   if (rc != WRT_OK) return rc;
   HIPCHK(hipEventRecord(j->e3, j->st));
   return WRT_OK;
@@ -1149,6 +1158,7 @@ int wrt_fast_describe(wrt_job* j, wrt_fast_desc* fd) {
     it->atrig = f->trig;
     it->acub = f->cub;
     it->aout = f->out;
+    it->aorder = f->order;
     it->andir = d->nphi * d->ntheta;
     it->ans = d->nshift;
     it->an = d->n;
