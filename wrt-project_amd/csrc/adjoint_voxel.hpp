@@ -18,6 +18,8 @@
 #define WRT_ADJOINT_VOXEL_HPP
 #include "wrt_common.hpp"
 
+#include <type_traits>
+
 // The voxels go to wavefronts in tiles of 4 (x) x 4 (y) x 4V (z): lane l takes x offset
 // (l >> 2) & 3, y offset l & 3 and z offsets (l >> 4) + 4v, v < V.  A tile is compact, so
 // for each direction its voxels' shifts fall in a few neighbouring spline intervals (the
@@ -61,10 +63,10 @@ template <int V, int MODE, class DirP, class TrigP, class CubP>
 __device__ __forceinline__ void adjoint_dir_range(DirP __restrict__ dirs, TrigP __restrict__ trig,
                                                   int d_begin, int d_end, int ndir,
                                                   CubP __restrict__ cub, int cub_d0, int ns,
-                                                  float s0, float inv_ds, double xd, double yd,
-                                                  const double* zd, float x, float y,
-                                                  const float* z, float* acc, float* comp) {
-  constexpr bool INSIDE = MODE >= 1;
+                                                  float s0, float inv_ds, bool covers,
+                                                  double xd, double yd, const double* zd,
+                                                  float x, float y, const float* z, float* acc,
+                                                  float* comp) {
   const int last = ns - 2;
   const float lastf = (float)last;  // exact: ns < 2^24
   const float u0 = -s0 * inv_ds;  // u = s * inv_ds + u0 = (s - s0) / ds
@@ -73,68 +75,96 @@ __device__ __forceinline__ void adjoint_dir_range(DirP __restrict__ dirs, TrigP 
     float4 D[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) D[u] = dirs[d + u];  // uniform across the wavefront
-    float sv[V][4];
-    int iv[V][4];
-    bool inv[V][4];
-    float4 Cv[V][4];
+    float sa[V][4];
     float sxy[4];  // x n_x + y n_y, shared by the thread's voxels
 #pragma unroll
     for (int u = 0; u < 4; ++u) sxy[u] = fmaf(x, D[u].x, y * D[u].y);
 #pragma unroll
     for (int v = 0; v < V; ++v)
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const float s = fmaf(z[v], D[u].z, sxy[u]);
-        bool in = true;
-        if (!INSIDE) {
-          in = fabsf(s) <= 1.0f;
-          // the FP64 re-decision as a branch of the whole wave on the ballot (a scalar
-          // branch, no exec mask saved and restored per evaluation); each lane takes its
-          // result only if its s is near the cut
-          const bool nr = fabsf(fabsf(s) - 1.0f) <= 1e-5f;
-          if (__builtin_expect(__builtin_amdgcn_ballot_w64(nr) != 0ull, 0)) {
-            const double4 T = trig[d + u];
-            const double sd = xd * T.x * T.y + yd * T.x * T.z + zd[v] * T.w;
-            in = nr ? !(fabs(sd) > 1.0) : in;
+      for (int u = 0; u < 4; ++u) sa[v][u] = fmaf(z[v], D[u].z, sxy[u]);
+    // the four directions in mode EM (the modes above; EM >= 1 only where every s of the
+    // wavefront is in range and far from the cut)
+    auto group = [&](auto em_tag) {
+      constexpr int EM = decltype(em_tag)::value;
+      constexpr bool INSIDE = EM >= 1;
+      float sv[V][4];
+      int iv[V][4];
+      bool inv[V][4];
+      float4 Cv[V][4];
+#pragma unroll
+      for (int v = 0; v < V; ++v)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const float s = sa[v][u];
+          bool in = true;
+          if (!INSIDE) {
+            in = fabsf(s) <= 1.0f;
+            // the FP64 re-decision as a branch of the whole wave on the ballot (a scalar
+            // branch, no exec mask saved and restored per evaluation); each lane takes its
+            // result only if its s is near the cut
+            const bool nr = fabsf(fabsf(s) - 1.0f) <= 1e-5f;
+            if (__builtin_expect(__builtin_amdgcn_ballot_w64(nr) != 0ull, 0)) {
+              const double4 T = trig[d + u];
+              const double sd = xd * T.x * T.y + yd * T.x * T.z + zd[v] * T.w;
+              in = nr ? !(fabs(sd) > 1.0) : in;
+            }
           }
+          const float uu = fmaf(s, inv_ds, u0);
+          if (EM == 2) {
+            iv[v][u] = (int)uu;
+            sv[v][u] = __builtin_amdgcn_fractf(uu);
+          } else {
+            // the interval, truncated and clamped into [0, last] (a valid interval even where
+            // s is cut): truncating the clamped uu is the same, in one med3 and one conversion
+            const int i = (int)__builtin_amdgcn_fmed3f(uu, 0.0f, lastf);
+            sv[v][u] = uu - (float)i;  // tt in [0, 1] where the sample is kept
+            iv[v][u] = i;
+          }
+          inv[v][u] = in;
         }
-        const float uu = fmaf(s, inv_ds, u0);
-        if (MODE == 2) {
-          iv[v][u] = (int)uu;
-          sv[v][u] = __builtin_amdgcn_fractf(uu);
-        } else {
-          // the interval, truncated and clamped into [0, last] (a valid interval even where
-          // s is cut): truncating the clamped uu is the same, in one med3 and one conversion
-          const int i = (int)__builtin_amdgcn_fmed3f(uu, 0.0f, lastf);
-          sv[v][u] = uu - (float)i;  // tt in [0, 1] where the sample is kept
-          iv[v][u] = i;
-        }
-        inv[v][u] = in;
-      }
-    // each direction's row of cubics from its scalar base, the interval as a 32-bit offset
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const auto row = cub + (long long)(d + u - cub_d0) * (ns - 1);
-#pragma unroll
-      for (int v = 0; v < V; ++v) Cv[v][u] = row[(unsigned)iv[v][u]];
-    }
-#pragma unroll
-    for (int v = 0; v < V; ++v) {
-      float g = 0.f;  // the four directions' terms, in direction order
+      // each direction's row of cubics from its scalar base, the interval as a 32-bit offset
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        const float t = sv[v][u];
-        const float4 C = Cv[v][u];
-        const float val = fmaf(fmaf(fmaf(C.w, t, C.z), t, C.y), t, C.x);
-        g = fmaf(D[u].w, INSIDE || inv[v][u] ? val : 0.f, g);
+        const auto row = cub + (long long)(d + u - cub_d0) * (ns - 1);
+#pragma unroll
+        for (int v = 0; v < V; ++v) Cv[v][u] = row[(unsigned)iv[v][u]];
       }
-      const float yk = g - comp[v];
-      const float tk = acc[v] + yk;
-      comp[v] = (tk - acc[v]) - yk;
-      acc[v] = tk;
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        float g = 0.f;  // the four directions' terms, in direction order
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const float t = sv[v][u];
+          const float4 C = Cv[v][u];
+          const float val = fmaf(fmaf(fmaf(C.w, t, C.z), t, C.y), t, C.x);
+          g = fmaf(D[u].w, INSIDE || inv[v][u] ? val : 0.f, g);
+        }
+        const float yk = g - comp[v];
+        const float tk = acc[v] + yk;
+        comp[v] = (tk - acc[v]) - yk;
+        acc[v] = tk;
+      }
+    };
+    if constexpr (MODE >= 1) {
+      group(std::integral_constant<int, MODE>());
+    } else {
+      // In the general mode a group whose every s (FP32, error under 1e-6) lies within
+      // 1 - 2e-4 of 0 keeps every sample, is not near the cut, and on a covering grid has u
+      // in [0, ns - 1): it takes mode 2's instructions, with the same result bit for bit.
+      bool far = covers;
+#pragma unroll
+      for (int v = 0; v < V; ++v)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) far = far && fabsf(sa[v][u]) < 1.0f - 2e-4f;
+      if (__builtin_amdgcn_ballot_w64(!far) == 0ull)
+        group(std::integral_constant<int, 2>());
+      else
+        group(std::integral_constant<int, 0>());
     }
   }
   if (d_end < ndir) return;
+  constexpr bool INSIDE = MODE >= 1;
   for (; d < ndir; ++d) {
     const float4 D = dirs[d];
     const float sxy = fmaf(x, D.x, y * D.y);
@@ -174,6 +204,7 @@ struct adjoint_voxel_set {
   double xd, yd, zd[V];
   float x, y, z[V];
   int mode;
+  bool covers;
   __device__ __forceinline__ adjoint_voxel_set(const long long* k, int n, int ns, float s0,
                                                float ds) {
     const int iy = (int)(k[0] % n);
@@ -191,7 +222,7 @@ struct adjoint_voxel_set {
       inside = inside && xd * xd + yd * yd + zd[v] * zd[v] <= (1.0 - 1e-4) * (1.0 - 1e-4);
     }
     // the grid covers [-1 + 5e-5, 1 - 5e-5] (wave-uniform)
-    const bool covers = s0 <= -1.0f + 5e-5f && fmaf((float)(ns - 1), ds, s0) >= 1.0f - 5e-5f;
+    covers = s0 <= -1.0f + 5e-5f && fmaf((float)(ns - 1), ds, s0) >= 1.0f - 5e-5f;
     const bool all_inside = __builtin_amdgcn_ballot_w64(!inside) == 0ull;
     mode = all_inside && covers ? 2 : (MODE1 && all_inside ? 1 : 0);
   }
@@ -201,14 +232,14 @@ struct adjoint_voxel_set {
                                         CubP cub, int cub_d0, int ns, float s0, float inv_ds,
                                         float* acc, float* comp) const {
     if (mode == 2)
-      adjoint_dir_range<V, 2>(dirs, trig, d_begin, d_end, ndir, cub, cub_d0, ns, s0, inv_ds, xd,
-                              yd, zd, x, y, z, acc, comp);
+      adjoint_dir_range<V, 2>(dirs, trig, d_begin, d_end, ndir, cub, cub_d0, ns, s0, inv_ds,
+                              covers, xd, yd, zd, x, y, z, acc, comp);
     else if (MODE1 && mode == 1)
-      adjoint_dir_range<V, 1>(dirs, trig, d_begin, d_end, ndir, cub, cub_d0, ns, s0, inv_ds, xd,
-                              yd, zd, x, y, z, acc, comp);
+      adjoint_dir_range<V, 1>(dirs, trig, d_begin, d_end, ndir, cub, cub_d0, ns, s0, inv_ds,
+                              covers, xd, yd, zd, x, y, z, acc, comp);
     else
-      adjoint_dir_range<V, 0>(dirs, trig, d_begin, d_end, ndir, cub, cub_d0, ns, s0, inv_ds, xd,
-                              yd, zd, x, y, z, acc, comp);
+      adjoint_dir_range<V, 0>(dirs, trig, d_begin, d_end, ndir, cub, cub_d0, ns, s0, inv_ds,
+                              covers, xd, yd, zd, x, y, z, acc, comp);
   }
 };
 
