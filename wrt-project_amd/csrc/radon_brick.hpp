@@ -255,6 +255,7 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
     hcf[c] = uniform_f32((float)fmin(hi[c], 1.0) + RI_TOL);
   }
   const float rho2wf = uniform_f32((float)(a.rho2 + 1e-4 * (1.0 + a.rho2)));
+  const double rho_w = sqrt(a.rho2) * (1.0 + 1e-12);
 
   const int d_end = min((grp + 1) * a.dgroup, a.ndir);
   // waves take the group's directions one at a time from a shared counter, so that they
@@ -292,14 +293,17 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
     // shifts with s in [pn0 - eps, pn1 + eps] (the ascending copy of the grid, bisected).  A
     // plane that misses the closed box has every lattice point outside it, and a point's
     // binary64 rounding (below 1e-15 here) is all that could bring its cell into the brick.
+    // The range is also clipped to |s| <= rho (slightly widened): a plane farther from the
+    // origin than the support radius meets only zero cells.
     const double eps = 1e-9;
+    const double slo = fmax(pn0 - eps, -rho_w), shi = fmin(pn1 + eps, rho_w);
     int j0, j1;
     {
       int l = 0, r = a.nshift;
-      while (l < r) { const int m = (l + r) >> 1; if (g_shift[m] < pn0 - eps) l = m + 1; else r = m; }
+      while (l < r) { const int m = (l + r) >> 1; if (g_shift[m] < slo) l = m + 1; else r = m; }
       j0 = l;
       l = 0; r = a.nshift;
-      while (l < r) { const int m = (l + r) >> 1; if (g_shift[m] <= pn1 + eps) l = m + 1; else r = m; }
+      while (l < r) { const int m = (l + r) >> 1; if (g_shift[m] <= shi) l = m + 1; else r = m; }
       j1 = l - 1;
     }
     if (j0 > j1) continue;
@@ -344,14 +348,20 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
       nDfx[c] = -Dfx[c];
       DS[c] = (long long)BC * Dfx[c];
     }
-    // Lanes 0, 1, 2 form the fixed-point base of axes x, y, z at (row R0, column C0) for each
-    // section as one FMA s * (n_c h) + K_c, K_c = ((a_c R0 + b_c C0) + 1) h - b0_c: |q| < 200
-    // and |K_c| < 4 h + 32, so the base errs by under 1e-11 cells for n < 16384.
-    const int ax = lane < 3 ? lane : 0;
-    const double l_nh = (ax == 0 ? nrm[0] : (ax == 1 ? nrm[1] : nrm[2])) * h;
-    const double l_K = (((ax == 0 ? av[0] : (ax == 1 ? av[1] : av[2])) * (double)R0 +
-                         (ax == 0 ? bv[0] : (ax == 1 ? bv[1] : bv[2])) * (double)C0) + 1.0) * h -
-                       (ax == 0 ? fb0x : (ax == 1 ? fb0y : fb0z));
+    // The fixed-point base of axis c at (row R0, column C0) for each section is one FMA onto
+    // 1.5 * 2^52: s * (n_c h 2^32) + (K_c 2^32 + 1.5 * 2^52 + 1024), K_c = ((a_c R0 + b_c C0)
+    // + 1) h - b0_c.  The base q lies within 200 cells of the brick (|q 2^32| < 2^40), so the
+    // FMA rounds to an integer-valued double in [2^52, 2^53) whose bit pattern less the
+    // constant's is the 64-bit fixed-point value with its 1024-unit bias (below); with the
+    // constant's own rounding it errs by at most 2^-32 cells.
+    const double fb0[3] = {fb0x, fb0y, fb0z};
+    double nh32[3], K32[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      nh32[c] = nrm[c] * h * 4294967296.0;  // exact scalings by powers of two
+      K32[c] = ((((av[c] * (double)R0 + bv[c] * (double)C0) + 1.0) * h - fb0[c]) * 4294967296.0 +
+                6755399441055744.0) + 1024.0;
+    }
     for (int j = j0; j <= j1; ++j) {
       const int jo = g_sidx[j];  // the plane's row in the caller's shift order
       unsigned long long gq = 0ull;  // transpose: the section's quantized value
@@ -362,8 +372,6 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
         if (gq == 0ull) continue;
       }
       const double s = g_shift[j];
-      // the plane meets the support sphere only for s^2 <= rho^2
-      if (a.rho2 - s * s < 0.0) continue;
       const float sf = (float)s;
       float aL[3], aU[3];
 #pragma unroll
@@ -451,27 +459,15 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
       // (x rounded as radon_grid/src/integral.c:47-51 rounds it, then (x + 1) / (2/n)) the
       // error is at most (1 + t + col - C0) 2^-32 (t < 118 rows; col - C0 < 160: a brick's
       // section spans at most 64 sqrt(3) + 6 columns, plus a step's overshoot) plus the
-      // roundings of q and of Qb (below 1e-10 for n < 16384): under 2^-23.  Q carries a bias
-      // of 2^-22 (1024 units), so wherever its fraction is at least 2^-21 the reference's
+      // roundings of q (below 1e-13) and of Qb (2^-32): under 2^-23.  Q carries a bias of
+      // 2^-22 (1024 units), so wherever its fraction is at least 2^-21 the reference's
       // quotient lies inside (k + 2^-23, k + 1 - 2^-23) for the integer part k of Q, which
       // is then the reference's truncated index (less b0); the few samples nearer a cell
-      // boundary take the exact binary64 decision instead.  Lane c (0..2) converts axis c;
-      // q lies within 160 cells of the brick, so its integer part fits int32 and its fraction
-      // rounded to 32 bits (at most 2^32 - 1 after the clamp) uint32.
+      // boundary take the exact binary64 decision instead.
       long long Qb[3];
-      {
-        const double q = fma(s, l_nh, l_K);
-        const double fl = floor(q);
-        const double fr = fmin(rint((q - fl) * 4294967296.0), 4294967295.0);
-        const int qi = (int)fl;
-        const unsigned qf = (unsigned)fr;
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const unsigned hi32 = (unsigned)__builtin_amdgcn_readlane(qi, c);
-          const unsigned lo32 = (unsigned)__builtin_amdgcn_readlane((int)qf, c);
-          Qb[c] = (long long)((((unsigned long long)hi32) << 32) | lo32) + 1024ll;
-        }
-      }
+      for (int c = 0; c < 3; ++c)
+        Qb[c] = __double_as_longlong(fma(s, nh32[c], K32[c])) - 0x4338000000000000ll;
       // One sample: Q the lane's fixed-point coordinates, R0 + t its row and cb - nrl + step*m
       // its column (for the exact decision); ok and off give its cell (below).  The forward
       // walk adds the cell's value to its lane sum; the transpose adds the section's quantized
@@ -580,7 +576,11 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
       // 2.90 s for bands of 16 rows taken 2 apart (their longer rows' steps waste more).
       int rlo_next = rowlo[wv][BR * k_first + r_lane];
       for (int k = k_first; k <= k_last; ++k) {
-        const int t = BR * k + r_lane;  // this lane's row within the window
+        // this lane's row within the window, passed through an empty asm so that its
+        // products below stay signed 32 x 32 -> 64 multiply-adds (one instruction each)
+        // instead of the two unsigned ones the compiler picks for a known non-negative factor
+        int t = BR * k + r_lane;
+        asm volatile("" : "+v"(t));
         const int nrl = (rlo_next << 16) >> 16;  // minus the band row's first column
         if (k < k_last) rlo_next = rowlo[wv][t + BR];
         const int steps = 1 + __builtin_amdgcn_readlane(bsteps01, BR * (k % NBH) + k / NBH);
@@ -590,12 +590,10 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
 #pragma unroll
           for (int dcol = 0; dcol < 2; ++dcol) {
             const int cb = dcol + 2 * k_lane, c0 = cb - nrl;  // c0: first column i2 of the lane
-            int tq = t;  // as in the forward walk below
-            asm volatile("" : "+v"(tq));
             long long Q[3];
 #pragma unroll
             for (int c = 0; c < 3; ++c)
-              Q[c] = Qb[c] + (long long)tq * Afx[c] + (long long)(c0 - C0) * Dfx[c];
+              Q[c] = Qb[c] + (long long)t * Afx[c] + (long long)(c0 - C0) * Dfx[c];
             for (int m = 0; m < steps2; ++m) {
               sample(Q, t, cb, nrl, 16, m);
 #pragma unroll
@@ -603,15 +601,10 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
             }
           }
         } else {
-          // t passes through an empty asm so that its products stay signed 32 x 32 -> 64
-          // multiply-adds (one instruction each) instead of the two unsigned ones the
-          // compiler picks for a known non-negative factor
-          int tq = t;
-          asm volatile("" : "+v"(tq));
           long long Q[3];
 #pragma unroll
           for (int c = 0; c < 3; ++c)
-            Q[c] = Tb[c] + (long long)tq * Afx[c] + (long long)nrl * nDfx[c];
+            Q[c] = Tb[c] + (long long)t * Afx[c] + (long long)nrl * nDfx[c];
           // Software-pipelined: step m's LDS read is issued before step m + 1's cell decision
           // and added after it, so that the read's latency overlaps that work.  A rejected
           // sample reads the zero slot (off = 0).
