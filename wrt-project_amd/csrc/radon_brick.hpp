@@ -506,8 +506,12 @@ __device__ __forceinline__ void radon_brick_task(const wrt_radon_fast_args& a, i
           asm volatile("" : "+v"(tv), "+v"(nv));
           const double f2 = (double)(cb - nv + step * __builtin_amdgcn_readfirstlane(m));
           const double f1 = (double)(R0 + tv);
-          const double Rx = s * nrm[0] + av[0] * f1, Ry = s * nrm[1] + av[1] * f1,
-                       Rz = s * nrm[2] + av[2] * f1;
+          // the shift, opaque here too, so that its products stay in this branch rather than
+          // in every section's setup
+          double sv = s;
+          asm volatile("" : "+s"(sv));
+          const double Rx = sv * nrm[0] + av[0] * f1, Ry = sv * nrm[1] + av[1] * f1,
+                       Rz = sv * nrm[2] + av[2] * f1;
           const double x = Rx + bv[0] * f2, y = Ry + bv[1] * f2, z = Rz + bv[2] * f2;
           double qx, qy, qz;
           if (POW2) {
