@@ -102,8 +102,15 @@ __device__ __forceinline__ float wave_sum_f32(float v) {
   v += dpp_move_f32<0x112, 0xf>(v);  // row_shr:2
   v += dpp_move_f32<0x114, 0xf>(v);  // row_shr:4
   v += dpp_move_f32<0x118, 0xf>(v);  // row_shr:8
-  v += dpp_move_f32<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
-  v += dpp_move_f32<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+  // The broadcasts as one DPP add each, written out: the rows outside the row mask keep
+  // their value (the compiler forms these as a zero move, a DPP move and an add).  The
+  // s_nop gives the DPP read of v the two wait states it needs after v's last write.
+  asm volatile(
+      "s_nop 1\n\t"
+      "v_add_f32_dpp %0, %0, %0 row_bcast:15 row_mask:0xa bank_mask:0xf\n\t"
+      "s_nop 1\n\t"
+      "v_add_f32_dpp %0, %0, %0 row_bcast:31 row_mask:0xc bank_mask:0xf"
+      : "+v"(v));
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 
