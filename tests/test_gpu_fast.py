@@ -71,6 +71,26 @@ def test_fast_f32_random_volume_and_odd_grids(ctx):
     assert orc.rel_l2(got, ref) <= 1e-5
 
 
+def test_fast_f32_planes_past_the_support(ctx):
+    # Non-zero values only within radius 0.4 of the centre (support radius rho about 0.42):
+    # each direction's bisected shift range is clipped to |s| <= rho, so the planes farther
+    # out are never walked and must come back exactly zero, as the plain form's sums of
+    # zero cells are; the planes inside must match the plain form.
+    n = 48
+    c = -1.0 + (2.0 * np.arange(n) + 1.0) / n
+    X, Y, Z = np.meshgrid(c, c, c, indexing="ij")
+    rng = np.random.default_rng(17)
+    vol = np.where(X * X + Y * Y + Z * Z <= 0.16, 1.0 + rng.standard_normal((n, n, n)), 0.0)
+    phi, th, w, s = ctx.radon_grid_init(16, 8, n + 1)
+    ref, _ = ctx.radon_grid(vol, phi, th, s, prec="f64")
+    got, _ = ctx.radon_grid(vol, phi, th, s, prec="f32")
+    far = np.abs(s) > 0.5
+    assert far.any() and (~far).any()
+    assert np.all(ref[far] == 0.0) and np.all(got[far] == 0.0)
+    assert orc.rel_l2(got, ref) <= 1e-5, orc.rel_l2(got, ref)
+    assert orc.max_rel(got, ref) <= 1e-5, orc.max_rel(got, ref)
+
+
 def test_fast_bf16_bricks(ctx):
     vol, phi, th, s = _phantom_case(ctx, 64, 32, 16, 65, seed=2, sigma=0.01)
     ref, _ = ctx.radon_grid(vol, phi, th, s, prec="f64")
