@@ -152,15 +152,29 @@ __device__ __forceinline__ void adjoint_dir_range(DirP __restrict__ dirs, TrigP 
       // In the general mode a group whose every s (FP32, error under 1e-6) lies within
       // 1 - 2e-4 of 0 keeps every sample, is not near the cut, and on a covering grid has u
       // in [0, ns - 1): it takes mode 2's instructions, with the same result bit for bit.
-      bool far = covers;
+      // A group whose every s lies beyond 1 + 1e-5 (so the cut holds, FP64 decision
+      // included) adds g = 0: only its Kahan step with g = 0 is taken.
+      bool far = covers, out = true;
 #pragma unroll
       for (int v = 0; v < V; ++v)
 #pragma unroll
-        for (int u = 0; u < 4; ++u) far = far && fabsf(sa[v][u]) < 1.0f - 2e-4f;
-      if (__builtin_amdgcn_ballot_w64(!far) == 0ull)
+        for (int u = 0; u < 4; ++u) {
+          far = far && fabsf(sa[v][u]) < 1.0f - 2e-4f;
+          out = out && fabsf(sa[v][u]) > 1.0f + 1e-5f;
+        }
+      if (__builtin_amdgcn_ballot_w64(!far) == 0ull) {
         group(std::integral_constant<int, 2>());
-      else
+      } else if (__builtin_amdgcn_ballot_w64(!out) == 0ull) {
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          const float yk = 0.f - comp[v];
+          const float tk = acc[v] + yk;
+          comp[v] = (tk - acc[v]) - yk;
+          acc[v] = tk;
+        }
+      } else {
         group(std::integral_constant<int, 0>());
+      }
     }
   }
   if (d_end < ndir) return;

@@ -156,8 +156,8 @@ static inline void wrt_adj_tile_order(int n, int vpt, int* order) {
 /* voxels per thread of a backprojection task in the persistent kernel; a task is up to one
  * tile per wavefront of its 1024-thread workgroup, which stages the directions' cubic rows
  * in LDS in chunks of WRT_ADJ_PF float4 per thread (4096 float4; 2 voxels and 4 float4 per
- * thread keep the kernel within 128 registers, with one 8-byte value spilled outside the
- * loops) */
+ * thread keep the kernel within 128 registers, with a few values spilled and reloaded only
+ * outside the sample loops) */
 #define WRT_ADJ_VPT 2
 #define WRT_ADJ_PF 4
 #define WRT_ADJ_TASK_TILES 16
